@@ -1,0 +1,130 @@
+/*
+ * isce2_amd.h — C ABI of the MI355X dense offset-tracking engine.
+ *
+ * The engine computes, for every window, what PyCuAmpcor's cuAmpcorChunk::run
+ * computes (contrib/PyCuAmpcor/src/cuAmpcorChunk.cpp:12-217): the integer
+ * correlation peak, SNR and covariance on the raw correlation surface, and
+ * the sub-pixel offset from the oversampled surface.
+ *
+ * Conventions
+ *   - x = row = down = azimuth, y = column = across = range; row-major images.
+ *   - Every pointer argument of ampcor_run* is DEVICE memory on the context's
+ *     device; sizes are element counts; pitches are in elements.
+ *   - Functions return 0 (AMPCOR_OK) or a negative AMPCOR_E* code and never
+ *     exit the process; ampcor_last_error() describes the last failure.
+ *   - A context is not thread-safe; use one per host thread.
+ */
+#ifndef ISCE2_AMD_H
+#define ISCE2_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AMPCOR_OK            0
+#define AMPCOR_E_ARG        -1   /* invalid argument / size */
+#define AMPCOR_E_RANGE      -2   /* a window reads outside its image */
+#define AMPCOR_E_DEVICE     -3   /* HIP runtime error */
+#define AMPCOR_E_NOMEM      -4   /* device allocation failed */
+#define AMPCOR_E_UNSUPPORTED -5  /* configuration outside the engine's limits */
+
+typedef struct ampcor_ctx ampcor_ctx;
+
+/* Per-window parameters. Mirrors the fields of cuAmpcorParameter that the
+ * per-window pipeline reads (contrib/PyCuAmpcor/src/cuAmpcorParameter.h:32-158);
+ * the values are the *Raw user settings, before setupParameters. */
+typedef struct ampcor_params {
+    int32_t window_h;      /* windowSizeHeightRaw */
+    int32_t window_w;      /* windowSizeWidthRaw */
+    int32_t hs_down;       /* halfSearchRangeDownRaw */
+    int32_t hs_across;     /* halfSearchRangeAcrossRaw */
+    int32_t raw_ovs;       /* rawDataOversamplingFactor */
+    int32_t zoom;          /* zoomWindowSize (corrSurfaceZoomInWindow), clamped as setupParameters */
+    int32_t surf_ovs;      /* oversamplingFactor (corrSurfaceOverSamplingFactor) */
+    int32_t surf_method;   /* oversamplingMethod: 0 FFT, 1 sinc */
+    int32_t corr_stat;     /* corrStatWindowSize */
+    int32_t deramp;        /* derampMethod: 0 amplitude chips, 1 deramp, other: complex chips */
+    int32_t algorithm;     /* 0 frequency domain, 1 time domain (identical sums) */
+    int32_t image_complex; /* 1: complex64 rasters, 0: float32 rasters */
+} ampcor_params;
+
+/* Derived sizes, as cuAmpcorParameter::setupParameters computes them
+ * (contrib/PyCuAmpcor/src/cuAmpcorParameter.cpp:70-98). */
+typedef struct ampcor_sizes {
+    int32_t zoom, half_zoom;
+    int32_t search_h, search_w;          /* raw search chip */
+    int32_t zoom_in_h, zoom_in_w;        /* raw zoom-in search chip */
+    int32_t os_win_h, os_win_w;          /* oversampled template */
+    int32_t os_search_h, os_search_w;    /* oversampled zoom-in search chip */
+    int32_t raw_corr_h, raw_corr_w;      /* raw correlation surface */
+    int32_t os_corr_h, os_corr_w;        /* oversampled correlation surface (before trim) */
+    int32_t surf_out;                    /* oversampled surface side */
+    int32_t corr_raw_zoom_h, corr_raw_zoom_w; /* SNR patch */
+    int32_t fused;                       /* 1 if the fused per-window kernels cover this config */
+} ampcor_sizes;
+
+/* Execution modes of ampcor_run_windows. */
+#define AMPCOR_MODE_AUTO    0   /* fused when supported, else staged */
+#define AMPCOR_MODE_STAGED  1   /* one batched kernel per pipeline stage (any size) */
+#define AMPCOR_MODE_FUSED   2   /* per-window fused kernels (window <= 64 class) */
+
+/* Create / destroy an engine context on HIP device `device`.
+ * Replaces cuAmpcorController's ctor/dtor and gpuDeviceInit
+ * (contrib/PyCuAmpcor/src/cuAmpcorController.cpp:19-29,43; cudaUtil.cpp:8-27). */
+int ampcor_create(int32_t device, ampcor_ctx **out);
+int ampcor_destroy(ampcor_ctx *ctx);
+
+/* Validate parameters and compute derived sizes; builds FFT plans.
+ * Replaces cuAmpcorParameter::setupParameters (cuAmpcorParameter.cpp:70-109). */
+int ampcor_setup(ampcor_ctx *ctx, const ampcor_params *params, ampcor_sizes *sizes_out);
+
+/* Run the per-window pipeline for n windows.
+ * Replaces cuAmpcorController::runAmpcor's chunk loop and cuAmpcorChunk::run
+ * (cuAmpcorController.cpp:107-137; cuAmpcorChunk.cpp:12-217), with images
+ * already resident in device memory (GDALImage::loadToDevice is the caller's).
+ *   ref_img/sec_img : complex64 (interleaved float pairs) or float32 images,
+ *                     [h][pitch] row-major.
+ *   ref_d/ref_a/sec_d/sec_a : int32[n] start pixels of each window's reference
+ *                     chip and secondary search chip (cuAmpcorParameter.cpp:155-205).
+ *   out_offset : float[n][2] (down, across); out_snr : float[n];
+ *   out_cov : float[n][3] (cov_down, cov_across, cov_cross).
+ *   chunk : windows per staged batch (0 = engine default); mode : AMPCOR_MODE_*.
+ *   stream : hipStream_t (NULL = default stream). The call is asynchronous
+ *   with respect to the host; synchronize the stream before reading outputs. */
+int ampcor_run_windows(ampcor_ctx *ctx,
+                       const void *ref_img, int64_t ref_h, int64_t ref_w, int64_t ref_pitch,
+                       const void *sec_img, int64_t sec_h, int64_t sec_w, int64_t sec_pitch,
+                       const int32_t *ref_d, const int32_t *ref_a,
+                       const int32_t *sec_d, const int32_t *sec_a, int32_t n,
+                       float *out_offset, float *out_snr, float *out_cov,
+                       int32_t chunk, int32_t mode, void *stream);
+
+/* Debug/parity entry: same as ampcor_run_windows but also returns the integer
+ * stage results per window (int32[n][2] each): raw-surface argmax (i_offsetInit
+ * before adjustment), zoom start (i_offsetInitAdjusted), zoom shift
+ * (i_maxLocShift) and oversampled-surface argmax (i_offsetZoomIn), plus the raw
+ * peak value (r_maxval). Names follow CUAMPCOR_DEBUG (cuAmpcorChunk.cpp:82-198).
+ * Any of the extra pointers may be NULL. */
+int ampcor_run_windows_debug(ampcor_ctx *ctx,
+                       const void *ref_img, int64_t ref_h, int64_t ref_w, int64_t ref_pitch,
+                       const void *sec_img, int64_t sec_h, int64_t sec_w, int64_t sec_pitch,
+                       const int32_t *ref_d, const int32_t *ref_a,
+                       const int32_t *sec_d, const int32_t *sec_a, int32_t n,
+                       float *out_offset, float *out_snr, float *out_cov,
+                       int32_t *out_maxloc, int32_t *out_start, int32_t *out_shift,
+                       int32_t *out_zoom_idx, float *out_maxval,
+                       int32_t chunk, int32_t mode, void *stream);
+
+/* Text of the last error on this context (or of the last ampcor_create failure
+ * when ctx is NULL). */
+const char *ampcor_last_error(const ampcor_ctx *ctx);
+
+/* Library version string. */
+const char *ampcor_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ISCE2_AMD_H */

@@ -1,0 +1,88 @@
+// Host-side engine state shared by the staged and fused pipelines.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+#include "../../include/isce2_amd.h"
+#include "fft_lds.h"
+
+namespace ampcor {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+#define HIP_OK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            throw ::ampcor::Error(AMPCOR_E_DEVICE, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+// Device buffer owned by the engine (grown on demand, kept across calls).
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    void reserve(size_t n);
+    ~DevBuf();
+    template <class U> U *as() const { return static_cast<U *>(ptr); }
+};
+
+// FFT plan: radix schedule + FP32 and FP64 twiddle tables on the device.
+struct FftPlan {
+    int n = 0;
+    FftDesc desc32{}, desc64{};
+    DevBuf tw32, tw64;
+};
+
+struct Sizes {
+    ampcor_sizes s{};
+};
+
+struct Engine {
+    int device = 0;
+    ampcor_params p{};
+    ampcor_sizes sz{};
+    bool configured = false;
+    std::string last_error;
+    std::map<int, std::unique_ptr<FftPlan>> plans;
+    std::map<std::string, std::unique_ptr<DevBuf>> bufs;
+    DevBuf sinc_table;
+    int sinc_dec = 4096, sinc_len = 8;
+
+    const FftPlan &plan(int n);
+    template <class U> U *buf(const std::string &name, size_t count) {
+        auto &b = bufs[name];
+        if (!b) b.reset(new DevBuf());
+        b->reserve(count * sizeof(U));
+        return b->as<U>();
+    }
+    void setup(const ampcor_params &params);
+    void ensure_sinc_table();
+
+    struct RunArgs {
+        const void *ref; int64_t ref_h, ref_w, ref_pitch;
+        const void *sec; int64_t sec_h, sec_w, sec_pitch;
+        const int32_t *ref_d, *ref_a, *sec_d, *sec_a;
+        int32_t n;
+        float *offset, *snr, *cov;
+        int32_t *maxloc, *start, *shift, *zoom_idx;
+        float *maxval;
+        int32_t chunk, mode;
+        hipStream_t stream;
+    };
+    void run(const RunArgs &a);
+    void run_staged(const RunArgs &a);
+    void run_fused(const RunArgs &a);
+};
+
+// FFT schedule for length n (radices in pass order; direct DFT if a prime > 17 remains)
+FftDesc make_schedule(int n);
+// whether the fused kernels cover a configuration
+bool fused_supported(const ampcor_params &p, const ampcor_sizes &s);
+
+}  // namespace ampcor

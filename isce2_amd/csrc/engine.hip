@@ -1,0 +1,425 @@
+// Engine host code: parameters, FFT plans, device buffers, the staged pipeline
+// and the C ABI declared in include/isce2_amd.h.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include "engine.h"
+#include "stage_kernels.h"
+
+namespace ampcor {
+
+// ---------------------------------------------------------------------------
+void DevBuf::reserve(size_t n) {
+    if (n <= bytes) return;
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+    size_t want = std::max<size_t>(n, 256);
+    if (hipMalloc(&ptr, want) != hipSuccess) {
+        ptr = nullptr;
+        (void)hipGetLastError();
+        throw Error(AMPCOR_E_NOMEM, "device allocation of " + std::to_string(want) + " bytes failed");
+    }
+    bytes = want;
+}
+DevBuf::~DevBuf() {
+    if (ptr) (void)hipFree(ptr);
+}
+
+FftDesc make_schedule(int n) {
+    FftDesc d{};
+    d.n = n;
+    int m = n;
+    const int pow2[] = {16, 8, 4, 2};
+    const int odd[] = {3, 5, 7, 11, 13, 17};
+    for (int r : pow2)
+        while (m % r == 0 && m > 1) { d.radix[d.npass++] = r; m /= r; }
+    for (int r : odd)
+        while (m % r == 0) { d.radix[d.npass++] = r; m /= r; }
+    if (m != 1 || d.npass > 14) {
+        d.npass = 0;
+        d.direct = 1;
+    }
+    return d;
+}
+
+const FftPlan &Engine::plan(int n) {
+    auto it = plans.find(n);
+    if (it != plans.end()) return *it->second;
+    std::unique_ptr<FftPlan> p(new FftPlan());
+    p->n = n;
+    p->desc32 = p->desc64 = make_schedule(n);
+    std::vector<float> t32(2 * n);
+    std::vector<double> t64(2 * n);
+    for (int k = 0; k < n; ++k) {
+        // e^{-2 pi i k/n} in FP64; sin/cos of reduced angles for accuracy
+        const long double a = -2.0L * 3.14159265358979323846264338327950288L * k / n;
+        const double c = double(cosl(a)), s = double(sinl(a));
+        t64[2 * k] = c; t64[2 * k + 1] = s;
+        t32[2 * k] = float(c); t32[2 * k + 1] = float(s);
+    }
+    p->tw32.reserve(t32.size() * sizeof(float));
+    p->tw64.reserve(t64.size() * sizeof(double));
+    HIP_OK(hipMemcpy(p->tw32.ptr, t32.data(), t32.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIP_OK(hipMemcpy(p->tw64.ptr, t64.data(), t64.size() * sizeof(double), hipMemcpyHostToDevice));
+    p->desc32.tw = p->tw32.ptr;
+    p->desc64.tw = p->tw64.ptr;
+    auto &ref = *p;
+    plans[n] = std::move(p);
+    return ref;
+}
+
+// cuAmpcorParameter::setupParameters (cuAmpcorParameter.cpp:70-98), with each
+// axis sized by its own half search range (the reference swaps them, :90-91).
+void Engine::setup(const ampcor_params &q) {
+    if (q.window_h <= 0 || q.window_w <= 0) throw Error(AMPCOR_E_ARG, "window size must be positive");
+    if (q.hs_down <= 0 || q.hs_across <= 0) throw Error(AMPCOR_E_ARG, "half search range must be positive");
+    if (q.raw_ovs <= 0 || q.surf_ovs <= 0) throw Error(AMPCOR_E_ARG, "oversampling factors must be positive");
+    if (q.zoom <= 0) throw Error(AMPCOR_E_ARG, "corrSurfaceZoomInWindow must be positive");
+    if (q.corr_stat <= 0) throw Error(AMPCOR_E_ARG, "corrStatWindowSize must be positive");
+    ampcor_sizes s{};
+    s.zoom = std::min(q.zoom, std::min(q.hs_across, q.hs_down) * 2 * q.raw_ovs);
+    s.half_zoom = s.zoom / (2 * q.raw_ovs);
+    s.corr_raw_zoom_h = std::min(q.corr_stat, 2 * q.hs_down + 1);
+    s.corr_raw_zoom_w = std::min(q.corr_stat, 2 * q.hs_across + 1);
+    s.search_h = q.window_h + 2 * q.hs_down;
+    s.search_w = q.window_w + 2 * q.hs_across;
+    s.zoom_in_h = q.window_h + 2 * s.half_zoom;
+    s.zoom_in_w = q.window_w + 2 * s.half_zoom;
+    s.os_win_h = q.window_h * q.raw_ovs;
+    s.os_win_w = q.window_w * q.raw_ovs;
+    s.os_search_h = s.zoom_in_h * q.raw_ovs;
+    s.os_search_w = s.zoom_in_w * q.raw_ovs;
+    s.raw_corr_h = s.search_h - q.window_h + 1;
+    s.raw_corr_w = s.search_w - q.window_w + 1;
+    s.os_corr_h = s.os_search_h - s.os_win_h + 1;
+    s.os_corr_w = s.os_search_w - s.os_win_w + 1;
+    s.surf_out = s.zoom * q.surf_ovs;
+    if (s.os_corr_h - 1 != s.zoom || s.os_corr_w - 1 != s.zoom)
+        throw Error(AMPCOR_E_ARG, "corrSurfaceZoomInWindow (" + std::to_string(q.zoom) +
+                                      ") must be a multiple of 2*rawDataOversamplingFactor");
+    if (s.half_zoom <= 0) throw Error(AMPCOR_E_ARG, "zoom-in window smaller than 2*rawDataOversamplingFactor");
+    if (std::max({s.search_h, s.search_w, s.os_search_h, s.os_search_w, s.surf_out}) > 4096)
+        throw Error(AMPCOR_E_UNSUPPORTED, "chip or surface side above 4096");
+    s.fused = fused_supported(q, s) ? 1 : 0;
+    p = q;
+    sz = s;
+    configured = true;
+}
+
+void Engine::ensure_sinc_table() {
+    if (sinc_table.ptr) return;
+    // cuSincOverSamplerR2R table (cuSincOverSampler.cu:21-80, constants cuSincOverSampler.h:27-57)
+    const double beta = 0.75, relfiltlen = 6.0, pedestal = 0.0, PI = 3.14159265359;
+    sinc_len = int(relfiltlen / beta + 0.5);
+    sinc_dec = 4096;
+    const int ncoef = sinc_len * sinc_dec;
+    const double wgthgt = (1.0 - pedestal) / 2.0, soff = (ncoef - 1.0) / 2.0;
+    std::vector<float> t(ncoef + 1);
+    for (int i = 0; i <= ncoef; ++i) {
+        const double wa = i - soff;
+        const double wgt = (1.0 - wgthgt) + wgthgt * std::cos(PI * wa / soff);
+        const double x = wa * beta / sinc_dec * PI;
+        const double f = x != 0.0 ? std::sin(x) / x : 1.0;
+        t[i] = float(f * wgt);
+    }
+    sinc_table.reserve(t.size() * sizeof(float));
+    HIP_OK(hipMemcpy(sinc_table.ptr, t.data(), t.size() * sizeof(float), hipMemcpyHostToDevice));
+}
+
+// ---------------------------------------------------------------------------
+// staged pipeline: launch helpers
+// ---------------------------------------------------------------------------
+namespace {
+constexpr int kElemPerBlock = 4096;   // complex elements of one FFT block (x2 buffers in LDS)
+
+template <class T, int DIR>
+void fft_rows(const FftPlan &pl, cx<T> *data, int64_t nrows, hipStream_t st) {
+    const FftDesc &d = std::is_same<T, float>::value ? pl.desc32 : pl.desc64;
+    const int g = std::max(1, kElemPerBlock / pl.n);
+    const size_t lds = 2 * size_t(g) * pl.n * sizeof(cx<T>);
+    const int64_t nblk = (nrows + g - 1) / g;
+    for (int64_t b0 = 0; b0 < nblk; b0 += 1 << 30) {
+        const int64_t nb = std::min<int64_t>(nblk - b0, 1 << 30);
+        hipLaunchKernelGGL((k_fft_rows<T, DIR>), dim3(unsigned(nb)), dim3(256), lds, st,
+                           data + b0 * g * pl.n, int(std::min<int64_t>(nrows - b0 * g, nb * g)), g, d);
+    }
+}
+
+template <class T, int DIR>
+void fft_cols(const FftPlan &pl, cx<T> *data, int nimg, int w, hipStream_t st) {
+    const FftDesc &d = std::is_same<T, float>::value ? pl.desc32 : pl.desc64;
+    const int c = std::max(1, std::min(w, std::min(16, kElemPerBlock / pl.n)));
+    const size_t lds = 2 * size_t(c) * pl.n * sizeof(cx<T>);
+    hipLaunchKernelGGL((k_fft_cols<T, DIR>), dim3(idivup(w, c), nimg), dim3(256), lds, st, data, w, c, d);
+}
+
+// 2D FFT of nimg images h x w (rows then columns), in place
+template <class T, int DIR>
+void fft2(Engine &e, cx<T> *data, int nimg, int h, int w, hipStream_t st) {
+    fft_rows<T, DIR>(e.plan(w), data, int64_t(nimg) * h, st);
+    fft_cols<T, DIR>(e.plan(h), data, nimg, w, st);
+}
+
+inline dim3 grid2(int per_img, int nimg) { return dim3(std::min(idivup(per_img, 256), 64), nimg); }
+}  // namespace
+
+// cuAmpcorChunk::run (cuAmpcorChunk.cpp:12-217) over chunks of windows.
+void Engine::run_staged(const RunArgs &a) {
+    using T = float;
+    const hipStream_t st = a.stream;
+    const int W_h = p.window_h, W_w = p.window_w;
+    const int S_h = sz.search_h, S_w = sz.search_w;
+    const int chunk = a.chunk > 0 ? a.chunk : 1024;
+    const int mode = p.image_complex ? (p.deramp == 0 ? 1 : 0) : 2;
+    // correlation transform sizes: the search chip sizes (cuCorrFrequency.cu:16-43)
+    const int L1h = S_h, L1w = S_w, L2h = sz.os_search_h, L2w = sz.os_search_w;
+    const int Z = sz.zoom, So = sz.surf_out;
+    const int nb = std::min(chunk, a.n);
+
+    cx<T> *c_ref = buf<cx<T>>("c_ref", size_t(nb) * W_h * W_w);
+    T *r_ref = buf<T>("r_ref", size_t(nb) * W_h * W_w);
+    cx<T> *c_sec = buf<cx<T>>("c_sec", size_t(nb) * S_h * S_w);
+    T *r_sec = buf<T>("r_sec", size_t(nb) * S_h * S_w);
+    const size_t zmax = std::max(size_t(L1h) * L1w, size_t(L2h) * L2w);
+    cx<T> *zc = buf<cx<T>>("zc", size_t(nb) * zmax);
+    const size_t satmax = std::max(size_t(S_h + 1) * (S_w + 1), size_t(L2h + 1) * (L2w + 1));
+    double *sat1 = buf<double>("sat1", size_t(nb) * satmax);
+    double *sat2 = buf<double>("sat2", size_t(nb) * satmax);
+    T *corr = buf<T>("corr", size_t(nb) * sz.raw_corr_h * sz.raw_corr_w);
+    int32_t *loc = buf<int32_t>("loc", size_t(nb) * 2);
+    T *val = buf<T>("val", nb);
+    int32_t *start = buf<int32_t>("start", size_t(nb) * 2);
+    int32_t *shift = buf<int32_t>("shift", size_t(nb) * 2);
+    int32_t *zidx = buf<int32_t>("zidx", size_t(nb) * 2);
+    cx<T> *c_ref_os = buf<cx<T>>("c_ref_os", size_t(nb) * sz.os_win_h * sz.os_win_w);
+    T *r_ref_os = buf<T>("r_ref_os", size_t(nb) * sz.os_win_h * sz.os_win_w);
+    cx<T> *c_zoom = buf<cx<T>>("c_zoom", size_t(nb) * sz.zoom_in_h * sz.zoom_in_w);
+    cx<T> *c_sec_os = buf<cx<T>>("c_sec_os", size_t(nb) * L2h * L2w);
+    T *r_sec_os = buf<T>("r_sec_os", size_t(nb) * L2h * L2w);
+    T *corr_os = buf<T>("corr_os", size_t(nb) * sz.os_corr_h * sz.os_corr_w);
+    cx<T> *c_surf_in = buf<cx<T>>("c_surf_in", size_t(nb) * Z * Z);
+    cx<T> *c_surf = buf<cx<T>>("c_surf", size_t(nb) * So * So);
+    T *r_surf = buf<T>("r_surf", size_t(nb) * So * So);
+    T *r_adj = buf<T>("r_adj", size_t(nb) * Z * Z);
+    if (p.surf_method) ensure_sinc_table();
+
+    for (int w0 = 0; w0 < a.n; w0 += nb) {
+        const int n = std::min(nb, a.n - w0);
+        // chip load + amplitude + mean (cuAmpcorChunk.cpp:18-39)
+        hipLaunchKernelGGL(k_gather<T>, grid2(W_h * W_w, n), dim3(256), 0, st, a.ref, a.ref_pitch,
+                           a.ref_d + w0, a.ref_a + w0, W_h, W_w, mode, c_ref);
+        hipLaunchKernelGGL(k_abs<T>, dim3(idivup(n * W_h * W_w, 256)), dim3(256), 0, st, c_ref, r_ref,
+                           int64_t(n) * W_h * W_w);
+        hipLaunchKernelGGL(k_sub_mean<T>, dim3(n), dim3(256), 0, st, r_ref, W_h * W_w);
+        hipLaunchKernelGGL(k_gather<T>, grid2(S_h * S_w, n), dim3(256), 0, st, a.sec, a.sec_pitch,
+                           a.sec_d + w0, a.sec_a + w0, S_h, S_w, mode, c_sec);
+        hipLaunchKernelGGL(k_abs<T>, dim3(idivup(n * S_h * S_w, 256)), dim3(256), 0, st, c_sec, r_sec,
+                           int64_t(n) * S_h * S_w);
+        // raw correlation (cuAmpcorChunk.cpp:48-52)
+        hipLaunchKernelGGL(k_pack_corr<T>, grid2(L1h * L1w, n), dim3(256), 0, st, r_ref, W_h, W_w, r_sec, S_h, S_w,
+                           zc, L1h);
+        fft2<T, -1>(*this, zc, n, L1h, L1w, st);
+        hipLaunchKernelGGL(k_spec_product<T>, grid2(L1h * L1w, n), dim3(256), 0, st, zc, L1h,
+                           T(1.0 / (double(L1h) * L1w)));
+        fft2<T, +1>(*this, zc, n, L1h, L1w, st);
+        hipLaunchKernelGGL(k_crop_real<T>, grid2(sz.raw_corr_h * sz.raw_corr_w, n), dim3(256), 0, st, zc, L1h,
+                           corr, sz.raw_corr_h, sz.raw_corr_w);
+        // normalize (cuAmpcorChunk.cpp:60)
+        hipLaunchKernelGGL(k_sat<T>, dim3(n), dim3(256), 0, st, r_sec, S_h, S_w, sat1, sat2);
+        hipLaunchKernelGGL(k_normalize<T>, dim3(n), dim3(256), 0, st, corr, sz.raw_corr_h, sz.raw_corr_w, r_ref,
+                           W_h, W_w, sat1, sat2, S_h, S_w);
+        // argmax, covariance, SNR, zoom start/shift (cuAmpcorChunk.cpp:69-106)
+        hipLaunchKernelGGL(k_maxloc<T>, dim3(n), dim3(256), 0, st, corr, sz.raw_corr_h, sz.raw_corr_w, loc, val);
+        hipLaunchKernelGGL(k_stats<T>, dim3(idivup(n, 64)), dim3(64), 0, st, corr, sz.raw_corr_h, sz.raw_corr_w, n,
+                           loc, val, W_h * W_w, sz.corr_raw_zoom_h, sz.corr_raw_zoom_w, p.hs_down, p.hs_across,
+                           sz.half_zoom, a.cov + 3 * size_t(w0), a.snr + w0, start, shift);
+        // reference oversampling (cuAmpcorChunk.cpp:115-126)
+        if (p.deramp == 1) hipLaunchKernelGGL(k_deramp<T>, dim3(n), dim3(256), 0, st, c_ref, W_h, W_w);
+        fft2<T, +1>(*this, c_ref, n, W_h, W_w, st);
+        hipLaunchKernelGGL(k_quadpad<T>, grid2(sz.os_win_h * sz.os_win_w, n), dim3(256), 0, st, c_ref, W_h, W_w,
+                           c_ref_os, sz.os_win_h, sz.os_win_w);
+        fft2<T, -1>(*this, c_ref_os, n, sz.os_win_h, sz.os_win_w, st);
+        hipLaunchKernelGGL(k_abs<T>, dim3(idivup(n * sz.os_win_h * sz.os_win_w, 256)), dim3(256), 0, st, c_ref_os,
+                           r_ref_os, int64_t(n) * sz.os_win_h * sz.os_win_w);
+        hipLaunchKernelGGL(k_sub_mean<T>, dim3(n), dim3(256), 0, st, r_ref_os, sz.os_win_h * sz.os_win_w);
+        // secondary zoom-in crop and oversampling (cuAmpcorChunk.cpp:134-137)
+        hipLaunchKernelGGL(k_crop_start<T>, grid2(sz.zoom_in_h * sz.zoom_in_w, n), dim3(256), 0, st, c_sec, S_h,
+                           S_w, start, c_zoom, sz.zoom_in_h, sz.zoom_in_w);
+        if (p.deramp == 1)
+            hipLaunchKernelGGL(k_deramp<T>, dim3(n), dim3(256), 0, st, c_zoom, sz.zoom_in_h, sz.zoom_in_w);
+        fft2<T, +1>(*this, c_zoom, n, sz.zoom_in_h, sz.zoom_in_w, st);
+        hipLaunchKernelGGL(k_quadpad<T>, grid2(L2h * L2w, n), dim3(256), 0, st, c_zoom, sz.zoom_in_h, sz.zoom_in_w,
+                           c_sec_os, L2h, L2w);
+        fft2<T, -1>(*this, c_sec_os, n, L2h, L2w, st);
+        hipLaunchKernelGGL(k_abs<T>, dim3(idivup(n * L2h * L2w, 256)), dim3(256), 0, st, c_sec_os, r_sec_os,
+                           int64_t(n) * L2h * L2w);
+        // oversampled correlation + normalize (cuAmpcorChunk.cpp:148-161)
+        hipLaunchKernelGGL(k_pack_corr<T>, grid2(L2h * L2w, n), dim3(256), 0, st, r_ref_os, sz.os_win_h,
+                           sz.os_win_w, r_sec_os, L2h, L2w, zc, L2h);
+        fft2<T, -1>(*this, zc, n, L2h, L2w, st);
+        hipLaunchKernelGGL(k_spec_product<T>, grid2(L2h * L2w, n), dim3(256), 0, st, zc, L2h,
+                           T(1.0 / (double(L2h) * L2w)));
+        fft2<T, +1>(*this, zc, n, L2h, L2w, st);
+        hipLaunchKernelGGL(k_crop_real<T>, grid2(sz.os_corr_h * sz.os_corr_w, n), dim3(256), 0, st, zc, L2h,
+                           corr_os, sz.os_corr_h, sz.os_corr_w);
+        hipLaunchKernelGGL(k_sat<T>, dim3(n), dim3(256), 0, st, r_sec_os, L2h, L2w, sat1, sat2);
+        hipLaunchKernelGGL(k_normalize<T>, dim3(n), dim3(256), 0, st, corr_os, sz.os_corr_h, sz.os_corr_w,
+                           r_ref_os, sz.os_win_h, sz.os_win_w, sat1, sat2, L2h, L2w);
+        // trim + surface oversampling (cuAmpcorChunk.cpp:169-186)
+        if (p.surf_method) {
+            hipLaunchKernelGGL(k_trim_real<T>, grid2(Z * Z, n), dim3(256), 0, st, corr_os, sz.os_corr_h,
+                               sz.os_corr_w, r_adj, Z);
+            hipLaunchKernelGGL(k_sinc<T>, dim3(1, n), dim3(256), 0, st, r_adj, Z, r_surf, So, shift,
+                               p.surf_ovs * p.raw_ovs, sinc_table.as<float>(), p.surf_ovs, sinc_dec, sinc_len);
+        } else {
+            hipLaunchKernelGGL(k_trim_complex<T>, grid2(Z * Z, n), dim3(256), 0, st, corr_os, sz.os_corr_h,
+                               sz.os_corr_w, c_surf_in, Z);
+            fft2<T, +1>(*this, c_surf_in, n, Z, Z, st);
+            hipLaunchKernelGGL(k_quadpad<T>, grid2(So * So, n), dim3(256), 0, st, c_surf_in, Z, Z, c_surf, So, So);
+            fft2<T, -1>(*this, c_surf, n, So, So, st);
+            hipLaunchKernelGGL(k_real<T>, dim3(idivup(n * So * So, 256)), dim3(256), 0, st, c_surf, r_surf,
+                               int64_t(n) * So * So);
+        }
+        // sub-pixel argmax and final offset (cuAmpcorChunk.cpp:194-207)
+        hipLaunchKernelGGL(k_maxloc<T>, dim3(n), dim3(256), 0, st, r_surf, So, So, zidx, (T *)nullptr);
+        hipLaunchKernelGGL(k_subpixel, dim3(idivup(n, 64)), dim3(64), 0, st, start, zidx, n,
+                           1.0f / float(p.surf_ovs * p.raw_ovs), p.hs_down, p.hs_across, a.offset + 2 * size_t(w0));
+        if (a.maxloc) HIP_OK(hipMemcpyAsync(a.maxloc + 2 * size_t(w0), loc, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToDevice, st));
+        if (a.start) HIP_OK(hipMemcpyAsync(a.start + 2 * size_t(w0), start, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToDevice, st));
+        if (a.shift) HIP_OK(hipMemcpyAsync(a.shift + 2 * size_t(w0), shift, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToDevice, st));
+        if (a.zoom_idx) HIP_OK(hipMemcpyAsync(a.zoom_idx + 2 * size_t(w0), zidx, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToDevice, st));
+        if (a.maxval) HIP_OK(hipMemcpyAsync(a.maxval + w0, val, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+        HIP_OK(hipGetLastError());
+    }
+}
+
+void Engine::run(const RunArgs &a) {
+    if (!configured) throw Error(AMPCOR_E_ARG, "ampcor_setup has not been called");
+    if (a.n < 0) throw Error(AMPCOR_E_ARG, "negative window count");
+    if (a.n == 0) return;
+    if (!a.ref || !a.sec || !a.ref_d || !a.ref_a || !a.sec_d || !a.sec_a || !a.offset || !a.snr || !a.cov)
+        throw Error(AMPCOR_E_ARG, "null pointer argument");
+    if (a.ref_pitch < a.ref_w || a.sec_pitch < a.sec_w) throw Error(AMPCOR_E_ARG, "pitch smaller than width");
+    HIP_OK(hipSetDevice(device));
+    // bounds: every chip must lie inside its image (device-side start arrays are
+    // checked here with one small copy; the caller's checkPixelInImageRange is stricter)
+    std::vector<int32_t> h(4 * size_t(a.n));
+    HIP_OK(hipMemcpyAsync(h.data(), a.ref_d, sizeof(int32_t) * a.n, hipMemcpyDeviceToHost, a.stream));
+    HIP_OK(hipMemcpyAsync(h.data() + a.n, a.ref_a, sizeof(int32_t) * a.n, hipMemcpyDeviceToHost, a.stream));
+    HIP_OK(hipMemcpyAsync(h.data() + 2 * size_t(a.n), a.sec_d, sizeof(int32_t) * a.n, hipMemcpyDeviceToHost, a.stream));
+    HIP_OK(hipMemcpyAsync(h.data() + 3 * size_t(a.n), a.sec_a, sizeof(int32_t) * a.n, hipMemcpyDeviceToHost, a.stream));
+    HIP_OK(hipStreamSynchronize(a.stream));
+    for (int i = 0; i < a.n; ++i) {
+        const int64_t rd = h[i], ra = h[a.n + i], sd = h[2 * size_t(a.n) + i], sa = h[3 * size_t(a.n) + i];
+        if (rd < 0 || ra < 0 || rd + p.window_h > a.ref_h || ra + p.window_w > a.ref_w ||
+            sd < 0 || sa < 0 || sd + sz.search_h > a.sec_h || sa + sz.search_w > a.sec_w)
+            throw Error(AMPCOR_E_RANGE, "window " + std::to_string(i) + " reads outside its image");
+    }
+    const bool fused = (a.mode == AMPCOR_MODE_FUSED) || (a.mode == AMPCOR_MODE_AUTO && sz.fused);
+    if (a.mode == AMPCOR_MODE_FUSED && !sz.fused)
+        throw Error(AMPCOR_E_UNSUPPORTED, "the fused kernels do not cover this configuration");
+    if (fused) run_fused(a);
+    else run_staged(a);
+    HIP_OK(hipGetLastError());
+}
+
+}  // namespace ampcor
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+struct ampcor_ctx {
+    ampcor::Engine eng;
+};
+
+namespace {
+std::mutex g_err_mu;
+std::string g_err;
+template <class F>
+int guarded(ampcor_ctx *ctx, F &&f) {
+    try {
+        f();
+        return AMPCOR_OK;
+    } catch (const ampcor::Error &e) {
+        if (ctx) ctx->eng.last_error = e.what();
+        else { std::lock_guard<std::mutex> l(g_err_mu); g_err = e.what(); }
+        return e.code;
+    } catch (const std::exception &e) {
+        if (ctx) ctx->eng.last_error = e.what();
+        else { std::lock_guard<std::mutex> l(g_err_mu); g_err = e.what(); }
+        return AMPCOR_E_DEVICE;
+    }
+}
+}  // namespace
+
+extern "C" {
+
+int ampcor_create(int32_t device, ampcor_ctx **out) {
+    if (!out) return AMPCOR_E_ARG;
+    *out = nullptr;
+    return guarded(nullptr, [&] {
+        int count = 0;
+        HIP_OK(hipGetDeviceCount(&count));
+        if (device < 0 || device >= count)
+            throw ampcor::Error(AMPCOR_E_ARG, "device " + std::to_string(device) + " not available (" +
+                                                  std::to_string(count) + " devices)");
+        HIP_OK(hipSetDevice(device));
+        auto *c = new ampcor_ctx();
+        c->eng.device = device;
+        *out = c;
+    });
+}
+
+int ampcor_destroy(ampcor_ctx *ctx) {
+    if (!ctx) return AMPCOR_E_ARG;
+    (void)hipSetDevice(ctx->eng.device);
+    delete ctx;
+    return AMPCOR_OK;
+}
+
+int ampcor_setup(ampcor_ctx *ctx, const ampcor_params *params, ampcor_sizes *sizes_out) {
+    if (!ctx || !params) return AMPCOR_E_ARG;
+    return guarded(ctx, [&] {
+        HIP_OK(hipSetDevice(ctx->eng.device));
+        ctx->eng.setup(*params);
+        if (sizes_out) *sizes_out = ctx->eng.sz;
+    });
+}
+
+int ampcor_run_windows_debug(ampcor_ctx *ctx, const void *ref_img, int64_t ref_h, int64_t ref_w, int64_t ref_pitch,
+                             const void *sec_img, int64_t sec_h, int64_t sec_w, int64_t sec_pitch,
+                             const int32_t *ref_d, const int32_t *ref_a, const int32_t *sec_d,
+                             const int32_t *sec_a, int32_t n, float *out_offset, float *out_snr, float *out_cov,
+                             int32_t *out_maxloc, int32_t *out_start, int32_t *out_shift, int32_t *out_zoom_idx,
+                             float *out_maxval, int32_t chunk, int32_t mode, void *stream) {
+    if (!ctx) return AMPCOR_E_ARG;
+    return guarded(ctx, [&] {
+        ampcor::Engine::RunArgs a{ref_img, ref_h, ref_w, ref_pitch, sec_img, sec_h, sec_w, sec_pitch,
+                                  ref_d, ref_a, sec_d, sec_a, n, out_offset, out_snr, out_cov,
+                                  out_maxloc, out_start, out_shift, out_zoom_idx, out_maxval,
+                                  chunk, mode, static_cast<hipStream_t>(stream)};
+        ctx->eng.run(a);
+    });
+}
+
+int ampcor_run_windows(ampcor_ctx *ctx, const void *ref_img, int64_t ref_h, int64_t ref_w, int64_t ref_pitch,
+                       const void *sec_img, int64_t sec_h, int64_t sec_w, int64_t sec_pitch, const int32_t *ref_d,
+                       const int32_t *ref_a, const int32_t *sec_d, const int32_t *sec_a, int32_t n,
+                       float *out_offset, float *out_snr, float *out_cov, int32_t chunk, int32_t mode,
+                       void *stream) {
+    return ampcor_run_windows_debug(ctx, ref_img, ref_h, ref_w, ref_pitch, sec_img, sec_h, sec_w, sec_pitch, ref_d,
+                                    ref_a, sec_d, sec_a, n, out_offset, out_snr, out_cov, nullptr, nullptr,
+                                    nullptr, nullptr, nullptr, chunk, mode, stream);
+}
+
+const char *ampcor_last_error(const ampcor_ctx *ctx) {
+    if (ctx) return ctx->eng.last_error.c_str();
+    std::lock_guard<std::mutex> l(g_err_mu);
+    return g_err.c_str();
+}
+
+const char *ampcor_version(void) { return "isce2_amd 0.1.0 (gfx950)"; }
+
+}  // extern "C"

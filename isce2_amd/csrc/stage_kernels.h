@@ -1,0 +1,481 @@
+// Batched per-stage kernels of the staged pipeline (one launch per stage over a
+// chunk of windows, intermediates in HBM). Each kernel names the reference
+// function whose arithmetic it performs (paths under contrib/PyCuAmpcor/src/).
+#pragma once
+#include <cfloat>
+#include "cx.h"
+#include "fft_lds.h"
+
+namespace ampcor {
+
+// ---------------------------------------------------------------------------
+// 1D FFTs over rows / columns of batched row-major images in HBM
+// ---------------------------------------------------------------------------
+// rows: `nrows` contiguous rows of length d.n (row pitch d.n), `g` rows per block
+template <class T, int DIR>
+__global__ void __launch_bounds__(256) k_fft_rows(cx<T> *data, int nrows, int g, FftDesc d) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *a = reinterpret_cast<cx<T> *>(smem_raw);
+    cx<T> *b = a + g * d.n;
+    const int r0 = blockIdx.x * g;
+    const int nr = min(g, nrows - r0);
+    const int tot = nr * d.n;
+    cx<T> *base = data + size_t(r0) * d.n;
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) a[i] = base[i];
+    __syncthreads();
+    cx<T> *res = fft_lds<T, DIR>(a, b, d, nr, 1, d.n);
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) base[i] = res[i];
+}
+
+// columns: images [nimg][h][w] with h = d.n; `c` columns per block
+template <class T, int DIR>
+__global__ void __launch_bounds__(256) k_fft_cols(cx<T> *data, int w, int c, FftDesc d) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *a = reinterpret_cast<cx<T> *>(smem_raw);
+    cx<T> *b = a + c * d.n;
+    const int h = d.n;
+    const int col0 = blockIdx.x * c;
+    const int nc = min(c, w - col0);
+    cx<T> *img = data + size_t(blockIdx.y) * h * w + col0;
+    const int tot = h * nc;
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+        const int r = i / nc, cc = i - r * nc;
+        a[r * c + cc] = img[size_t(r) * w + cc];
+    }
+    __syncthreads();
+    cx<T> *res = fft_lds<T, DIR>(a, b, d, nc, c, 1);
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+        const int r = i / nc, cc = i - r * nc;
+        img[size_t(r) * w + cc] = res[r * c + cc];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// chip gather (cuArraysCopy.cu:70-165): mode 0 complex copy, 1 (|z|,0), 2 real -> (x,0)
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void k_gather(const void *img, int64_t pitch, const int32_t *sd, const int32_t *sa,
+                         int h, int w, int mode, cx<T> *out) {
+    const int win = blockIdx.y;
+    const int tot = h * w;
+    const int64_t r0 = sd[win], c0 = sa[win];
+    cx<T> *o = out + size_t(win) * tot;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += gridDim.x * blockDim.x) {
+        const int r = i / w, c = i - r * w;
+        const int64_t off = (r0 + r) * pitch + c0 + c;
+        cx<T> v;
+        if (mode == 2) {
+            v = {T(static_cast<const float *>(img)[off]), T(0)};
+        } else {
+            const float2 z = static_cast<const float2 *>(img)[off];
+            v = {T(z.x), T(z.y)};
+            if (mode == 1) v = {T(sqrtf(z.x * z.x + z.y * z.y)), T(0)};
+        }
+        o[i] = v;
+    }
+}
+
+// |z| (cuArraysCopy.cu:510-529)
+template <class T>
+__global__ void k_abs(const cx<T> *in, T *out, int64_t total) {
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < total; i += int64_t(gridDim.x) * blockDim.x)
+        out[i] = amag(in[i]);
+}
+
+// block-wide FP64 sum (256 threads)
+__device__ __forceinline__ double block_sum256(double v, double *sh) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[wv] = v;
+    __syncthreads();
+    double s = 0.0;
+    const int nw = (blockDim.x + 63) >> 6;
+    for (int i = 0; i < nw; ++i) s += sh[i];
+    __syncthreads();
+    return s;
+}
+
+// mean subtraction per image, FP64 sum (cuCorrNormalization.cu:210-235)
+template <class T>
+__global__ void __launch_bounds__(256) k_sub_mean(T *data, int size) {
+    __shared__ double sh[8];
+    T *img = data + size_t(blockIdx.x) * size;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < size; i += blockDim.x) s += double(img[i]);
+    s = block_sum256(s, sh);
+    const T mean = T(s / size);
+    for (int i = threadIdx.x; i < size; i += blockDim.x) img[i] -= mean;
+}
+
+// ---------------------------------------------------------------------------
+// correlation via one complex 2D FFT of z = t_pad + i*s (cuCorrFrequency.cu:63-111)
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void k_pack_corr(const T *tmpl, int th, int tw, const T *srch, int sh, int sw,
+                            cx<T> *z, int L) {
+    const int win = blockIdx.y;
+    const int tot = L * L;
+    const T *t = tmpl + size_t(win) * th * tw;
+    const T *s = srch + size_t(win) * sh * sw;
+    cx<T> *o = z + size_t(win) * tot;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += gridDim.x * blockDim.x) {
+        const int r = i / L, c = i - r * L;
+        const T a = (r < th && c < tw) ? t[r * tw + c] : T(0);
+        const T b = (r < sh && c < sw) ? s[r * sw + c] : T(0);
+        o[i] = {a, b};
+    }
+}
+
+// P[k] = conj(T[k]) S[k] / L^2 with T = (Z[k] + conj Z[-k])/2, S = (Z[k] - conj Z[-k])/(2i);
+// P[-k] = conj(P[k]). One thread per (k, -k) pair.
+template <class T>
+__global__ void k_spec_product(cx<T> *z, int L, T scale) {
+    const int win = blockIdx.y;
+    cx<T> *Z = z + size_t(win) * L * L;
+    const int tot = L * L;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += gridDim.x * blockDim.x) {
+        const int r = i / L, c = i - r * L;
+        const int rn = r ? L - r : 0, cn = c ? L - c : 0;
+        const int j = rn * L + cn;
+        if (j < i) continue;   // pair handled by the smaller index
+        const cx<T> a = Z[i], bc = conj(Z[j]);
+        const cx<T> tk = (a + bc) * T(0.5);
+        const cx<T> d = (a - bc) * T(0.5);
+        const cx<T> sk = {d.y, -d.x};   // d / i
+        const cx<T> p = cmulc(tk, sk) * scale;
+        Z[i] = p;
+        if (j != i) Z[j] = conj(p);
+    }
+}
+
+// real part of the top-left oh x ow block
+template <class T>
+__global__ void k_crop_real(const cx<T> *z, int L, T *out, int oh, int ow) {
+    const int win = blockIdx.y;
+    const int tot = oh * ow;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += gridDim.x * blockDim.x) {
+        const int r = i / ow, c = i - r * ow;
+        out[size_t(win) * tot + i] = z[size_t(win) * L * L + r * L + c].x;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// normalization with FP64 summed-area tables (cuCorrNormalizationSAT.cu:150-242,
+// formula of cuCorrNormalization.cu:402-423)
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(256) k_sat(const T *srch, int sh, int sw, double *sat1, double *sat2) {
+    const int win = blockIdx.x;
+    const T *s = srch + size_t(win) * sh * sw;
+    const int p = sw + 1;
+    double *a1 = sat1 + size_t(win) * (sh + 1) * p;
+    double *a2 = sat2 + size_t(win) * (sh + 1) * p;
+    for (int c = threadIdx.x; c < p; c += blockDim.x) { a1[c] = 0.0; a2[c] = 0.0; }
+    for (int r = threadIdx.x; r < sh; r += blockDim.x) {
+        double x1 = 0.0, x2 = 0.0;
+        a1[(r + 1) * p] = 0.0;
+        a2[(r + 1) * p] = 0.0;
+        for (int c = 0; c < sw; ++c) {
+            const double v = double(s[r * sw + c]);
+            x1 += v;
+            x2 += v * v;
+            a1[(r + 1) * p + c + 1] = x1;
+            a2[(r + 1) * p + c + 1] = x2;
+        }
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < p; c += blockDim.x) {
+        double x1 = 0.0, x2 = 0.0;
+        for (int r = 1; r <= sh; ++r) {
+            x1 += a1[r * p + c];
+            x2 += a2[r * p + c];
+            a1[r * p + c] = x1;
+            a2[r * p + c] = x2;
+        }
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_normalize(T *corr, int ch, int cw, const T *tmpl, int th, int tw,
+                                                   const double *sat1, const double *sat2, int sh, int sw) {
+    __shared__ double sh_red[8];
+    const int win = blockIdx.x;
+    const T *t = tmpl + size_t(win) * th * tw;
+    double t2 = 0.0;
+    for (int i = threadIdx.x; i < th * tw; i += blockDim.x) {
+        const double v = double(t[i]);
+        t2 += v * v;
+    }
+    t2 = block_sum256(t2, sh_red);
+    const int p = sw + 1;
+    const double *a1 = sat1 + size_t(win) * (sh + 1) * p;
+    const double *a2 = sat2 + size_t(win) * (sh + 1) * p;
+    const double inv = 1.0 / double(th * tw);
+    T *c = corr + size_t(win) * ch * cw;
+    for (int i = threadIdx.x; i < ch * cw; i += blockDim.x) {
+        const int u = i / cw, v = i - u * cw;
+        const double s1 = a1[(u + th) * p + v + tw] - a1[u * p + v + tw] - a1[(u + th) * p + v] + a1[u * p + v];
+        const double s2 = a2[(u + th) * p + v + tw] - a2[u * p + v + tw] - a2[(u + th) * p + v] + a2[u * p + v];
+        const double norm2 = (s2 - s1 * s1 * inv) * t2;
+        c[i] = T(double(c[i]) / sqrt(norm2 + double(FLT_EPSILON)));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// argmax with the reference tie rule (cuOffset.cu:23-84), one block per surface
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ void argmax_reduce256(T &bv, uint64_t &bk, T *shv, uint64_t *shk) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const T ov = __shfl_xor(bv, o);
+        const uint64_t ok = __shfl_xor(bk, o);
+        if (better(ov, ok, bv, bk)) { bv = ov; bk = ok; }
+    }
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) { shv[wv] = bv; shk[wv] = bk; }
+    __syncthreads();
+    const int nw = (blockDim.x + 63) >> 6;
+    bv = shv[0]; bk = shk[0];
+    for (int i = 1; i < nw; ++i)
+        if (better(shv[i], shk[i], bv, bk)) { bv = shv[i]; bk = shk[i]; }
+    __syncthreads();
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) k_maxloc(const T *surf, int h, int w, int32_t *loc, T *val) {
+    __shared__ T shv[4];
+    __shared__ uint64_t shk[4];
+    const T *s = surf + size_t(blockIdx.x) * h * w;
+    T bv = T(-FLT_MAX);
+    uint64_t bk = ~uint64_t(0);
+    for (int i = threadIdx.x; i < h * w; i += blockDim.x) {
+        const T v = s[i];
+        const uint64_t k = maxloc_key(uint32_t(i));
+        if (v > T(-FLT_MAX) && better(v, k, bv, bk)) { bv = v; bk = k; }
+    }
+    argmax_reduce256(bv, bk, shv, shk);
+    if (threadIdx.x == 0) {
+        const int idx = (bk == ~uint64_t(0)) ? 0 : int(bk & 0xffffffffu);
+        loc[2 * blockIdx.x] = idx / w;
+        loc[2 * blockIdx.x + 1] = idx % w;
+        if (val) val[blockIdx.x] = bv;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// covariance, SNR, zoom start/shift; one thread per window
+// (cuEstimateStats.cu:20-110, cuArraysCopy.cu:249-284, cuCorrNormalization.cu:255-285,
+//  cuOffset.cu:190-241)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void adjust_axis(int maxloc, int old_range, int new_range, int &start, int &shift) {
+    start = maxloc - new_range;
+    shift = 0;
+    const int rbound = 2 * (old_range - new_range);
+    if (start < 0) { shift = -start; start = 0; }
+    else if (start > rbound) { shift = start - rbound; start = rbound; }
+}
+
+template <class T>
+__device__ void window_stats(const T *c, int nx, int ny, int px, int py, double peak, int template_size,
+                             int patch_h, int patch_w, float *cov3, float *snr) {
+    // covariance
+    double cxx = 99.0, cyy = 99.0, cxy = 0.0;
+    if (!(px - 1 < 0 || py - 1 < 0 || px + 1 >= nx || py + 1 >= ny)) {
+        auto g = [&](int dx, int dy) { return double(c[(px + dx) * ny + py + dy]); };
+        double dxx = -(g(1, 0) + g(-1, 0) - 2.0 * g(0, 0)) * template_size;
+        double dyy = -(g(0, 1) + g(0, -1) - 2.0 * g(0, 0)) * template_size;
+        double dxy = (g(1, 1) + g(-1, -1) - g(1, -1) - g(-1, 1)) * 0.25 * template_size;
+        double n2 = fmax(1.0 - peak, 0.0);
+        const double n4 = n2 * n2 * 0.5 * template_size;
+        n2 *= 2.0;
+        const double u = dxy * dxy - dxx * dyy;
+        const double u2 = u * u;
+        if (fabs(u) >= 1e-2) {
+            cxx = (-n2 * u * dyy + n4 * (dyy * dyy + dxy * dxy)) / u2;
+            cyy = (-n2 * u * dxx + n4 * (dxx * dxx + dxy * dxy)) / u2;
+            cxy = (n2 * u * dxy - n4 * (dxx + dyy) * dxy) / u2;
+        }
+    }
+    cov3[0] = float(cxx); cov3[1] = float(cyy); cov3[2] = float(cxy);
+    // SNR over the patch centred on the peak (offset -size/2), zero outside
+    double sum = 0.0;
+    int count = 0;
+    for (int i = 0; i < patch_h; ++i) {
+        const int x = px + i - patch_h / 2;
+        if (x < 0 || x >= nx) continue;
+        for (int j = 0; j < patch_w; ++j) {
+            const int y = py + j - patch_w / 2;
+            if (y < 0 || y >= ny) continue;
+            const double v = double(c[x * ny + y]);
+            sum += v * v;
+            ++count;
+        }
+    }
+    const double mean = (sum - peak * peak) / double(count - 1);
+    *snr = float(peak * peak / mean);
+}
+
+template <class T>
+__global__ void k_stats(const T *corr, int nx, int ny, int n, const int32_t *loc, const T *val, int template_size,
+                        int patch_h, int patch_w, int hs_down, int hs_across, int hz,
+                        float *cov, float *snr, int32_t *start, int32_t *shift) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n) return;
+    const int px = loc[2 * w], py = loc[2 * w + 1];
+    window_stats(corr + size_t(w) * nx * ny, nx, ny, px, py, double(val[w]), template_size, patch_h, patch_w,
+                 cov + 3 * w, snr + w);
+    int s0, h0, s1, h1;
+    adjust_axis(px, hs_down, hz, s0, h0);
+    adjust_axis(py, hs_across, hz, s1, h1);
+    start[2 * w] = s0; start[2 * w + 1] = s1;
+    shift[2 * w] = h0; shift[2 * w + 1] = h1;
+}
+
+// ---------------------------------------------------------------------------
+// deramp method 1 (cuDeramp.cu:343-390) with float2.h's operator*= (imaginary
+// part reuses the updated real part); sums, phases and rotation in FP64
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(256) k_deramp(cx<T> *chips, int h, int w) {
+    __shared__ double sh[8];
+    cx<T> *z = chips + size_t(blockIdx.x) * h * w;
+    double ayr = 0, ayi = 0, axr = 0, axi = 0;
+    for (int i = threadIdx.x; i < h * w; i += blockDim.x) {
+        const int r = i / w, c = i - r * w;
+        const double zr = double(z[i].x), zi = double(z[i].y);
+        if (c < w - 1) {   // z * conj(z_right)
+            const double br = double(z[i + 1].x), bi = double(z[i + 1].y);
+            ayr += zr * br + zi * bi;
+            ayi += zi * br - zr * bi;
+        }
+        if (r < h - 1) {   // z * conj(z_below)
+            const double br = double(z[i + w].x), bi = double(z[i + w].y);
+            axr += zr * br + zi * bi;
+            axi += zi * br - zr * bi;
+        }
+    }
+    ayr = block_sum256(ayr, sh);
+    ayi = block_sum256(ayi, sh);
+    axr = block_sum256(axr, sh);
+    axi = block_sum256(axi, sh);
+    const double phy = atan2(ayi, ayr);
+    const double phx = atan2(axi, axr);
+    for (int i = threadIdx.x; i < h * w; i += blockDim.x) {
+        const int r = i / w, c = i - r * w;
+        const double ph = double(c) * phx + double(r) * phy;
+        double s, co;
+        sincos(ph, &s, &co);
+        const double x = double(z[i].x), y = double(z[i].y);
+        const double nx = x * co - y * s;
+        const double ny = y * co + nx * s;
+        z[i] = {T(nx), T(ny)};
+    }
+}
+
+// quadrant padding h x w -> H x W with scale 1/(h*w) (cuArraysPadding.cu:223-266)
+template <class T>
+__global__ void k_quadpad(const cx<T> *in, int h, int w, cx<T> *out, int H, int W) {
+    const int win = blockIdx.y;
+    const int h2 = h / 2, w2 = w / 2;
+    const T f = T(1) / T(h * w);
+    const cx<T> *a = in + size_t(win) * h * w;
+    cx<T> *o = out + size_t(win) * H * W;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < H * W; i += gridDim.x * blockDim.x) {
+        const int p = i / W, q = i - p * W;
+        int sr = -1, sc = -1;
+        if (p < h2) sr = p; else if (p >= H - h2) sr = p - H + h;
+        if (q < w2) sc = q; else if (q >= W - w2) sc = q - W + w;
+        o[i] = (sr >= 0 && sc >= 0) ? a[sr * w + sc] * f : cx<T>{T(0), T(0)};
+    }
+}
+
+// per-window crop at a varying start (cuArraysCopy.cu:209-224)
+template <class T>
+__global__ void k_crop_start(const cx<T> *in, int ih, int iw, const int32_t *start, cx<T> *out, int oh, int ow) {
+    const int win = blockIdx.y;
+    const int r0 = start[2 * win], c0 = start[2 * win + 1];
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < oh * ow; i += gridDim.x * blockDim.x) {
+        const int r = i / ow, c = i - r * ow;
+        out[size_t(win) * oh * ow + i] = in[size_t(win) * ih * iw + (r0 + r) * iw + c0 + c];
+    }
+}
+
+// top-left zoom x zoom of the oversampled surface -> complex (cuAmpcorChunk.cpp:169, cuOverSampler.cpp:143)
+template <class T>
+__global__ void k_trim_complex(const T *in, int ih, int iw, cx<T> *out, int z) {
+    const int win = blockIdx.y;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < z * z; i += gridDim.x * blockDim.x) {
+        const int r = i / z, c = i - r * z;
+        out[size_t(win) * z * z + i] = {in[size_t(win) * ih * iw + r * iw + c], T(0)};
+    }
+}
+
+template <class T>
+__global__ void k_trim_real(const T *in, int ih, int iw, T *out, int z) {
+    const int win = blockIdx.y;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < z * z; i += gridDim.x * blockDim.x) {
+        const int r = i / z, c = i - r * z;
+        out[size_t(win) * z * z + i] = in[size_t(win) * ih * iw + r * iw + c];
+    }
+}
+
+// real part of a complex array
+template <class T>
+__global__ void k_real(const cx<T> *in, T *out, int64_t total) {
+    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < total; i += int64_t(gridDim.x) * blockDim.x)
+        out[i] = in[i].x;
+}
+
+// windowed-sinc surface oversampling (cuSincOverSampler.cu:84-195): zero output,
+// then the (2*2*covs+1)^2 patch around out/2 + shift*factor
+template <class T>
+__global__ void k_sinc(const T *in, int nin, T *out, int nout, const int32_t *shift, int factor,
+                       const float *filt, int covs, int dec, int intplength) {
+    const int win = blockIdx.y;
+    const int rng = 2 * covs, st = nout / 2 - rng, size = 2 * rng + 1;
+    T *o = out + size_t(win) * nout * nout;
+    const T *a = in + size_t(win) * nin * nin;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < nout * nout; i += gridDim.x * blockDim.x) o[i] = T(0);
+    __syncthreads();   // same block writes zeros then values only for its own indices below
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < size * size; i += gridDim.x * blockDim.x) {
+        const int ix = i / size, iy = i - ix * size;
+        int ox = ix + st + shift[2 * win] * factor;
+        if (ox >= nout) ox -= nout;
+        int oy = iy + st + shift[2 * win + 1] * factor;
+        if (oy >= nout) oy -= nout;
+        const int xo = ox / covs, yo = oy / covs;
+        const int xf = int((float(ox) / covs - xo) * dec);
+        const int yf = int((float(oy) / covs - yo) * dec);
+        double acc = 0.0, wsum = 0.0;
+        for (int p = 0; p < intplength; ++p) {
+            int x = xo - p + intplength / 2;
+            if (x < 0) x += nin;
+            if (x >= nin) x -= nin;
+            const double wx = filt[p * dec + xf];
+            for (int q = 0; q < intplength; ++q) {
+                int y = yo - q + intplength / 2;
+                if (y < 0) y += nin;
+                if (y >= nin) y -= nin;
+                const double wgt = wx * double(filt[q * dec + yf]);
+                wsum += wgt;
+                acc += double(a[x * nin + y]) * wgt;
+            }
+        }
+        o[ox * nout + oy] = T(acc / wsum);
+    }
+}
+
+// final offset (cuOffset.cu:137-146)
+__global__ void k_subpixel(const int32_t *start, const int32_t *zidx, int n, float ratio, int hs_down,
+                           int hs_across, float *offset) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= n) return;
+    offset[2 * w] = ratio * float(zidx[2 * w]) + float(start[2 * w]) - float(hs_down);
+    offset[2 * w + 1] = ratio * float(zidx[2 * w + 1]) + float(start[2 * w + 1]) - float(hs_across);
+}
+
+}  // namespace ampcor

@@ -1,0 +1,123 @@
+"""GPU parity: the HIP engine against the FP64 oracle on the same seeded windows.
+
+Bar: integer stage results (raw-surface argmax, zoom start/shift, oversampled
+argmax) bit-exact; sub-pixel offsets within 0.01 px; SNR and covariance
+within 1e-3 relative (FP32 arithmetic vs FP64) where the integer peaks agree.
+"""
+import numpy as np
+import pytest
+import torch
+
+from isce2_amd.engine import MODE_STAGED, MODE_FUSED, MODE_AUTO, Engine
+from isce2_amd.synth import speckle_pair_shift
+from oracle import ampcor_oracle as O
+
+from .gpu_helpers import compare, run_both
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def eng():
+    e = Engine(0)
+    yield e
+    e.close()
+
+
+@pytest.fixture(scope="module")
+def scene1():
+    # 512x512 cut of the CPU-comparison scene: speckle, shift (+3.25, -1.75), 10 dB noise
+    return speckle_pair_shift(512, 512, shift=(3.25, -1.75), snr_db=10.0, seed=0)
+
+
+def _grid(nd, na, s0, skip, hs):
+    rd, ra, sd, sa, _, _ = O.start_pixels(nd, na, s0, s0, skip, skip, 0, 0, hs, hs)
+    return rd, ra, sd, sa
+
+
+def _report(name, c):
+    print(f"\n[{name}] " + " ".join(f"{k}={v}" for k, v in c.items()), flush=True)
+
+
+def _assert_parity(c, zoom_allow=0):
+    assert c["maxloc_mismatch"] == 0, c
+    assert c["start_mismatch"] == 0 and c["shift_mismatch"] == 0, c
+    assert c["zoom_mismatch"] <= zoom_allow, c
+    assert c["offset_over_tol"] <= zoom_allow, c
+    assert c["snr_max_rel"] < 1e-3, c
+    assert c["cov_sentinel_mismatch"] == 0 and c["cov_max_rel"] < 1e-3, c
+
+
+@pytest.mark.parametrize("deramp", [0, 1])
+def test_staged_default_config(eng, scene1, deramp):
+    ref, sec = scene1
+    cfg = O.AmpcorConfig(hs_down=16, hs_across=16, deramp=deramp)
+    starts = _grid(6, 6, 16, 64, 16)
+    g, o = run_both(eng, ref, sec, cfg, starts, MODE_STAGED, chunk=7)
+    c = compare(g, o)
+    _report(f"staged deramp={deramp}", c)
+    _assert_parity(c, zoom_allow=1)
+
+
+def test_staged_hs20_sinc_and_time_domain(eng, scene1):
+    ref, sec = scene1
+    cfg = O.AmpcorConfig(hs_down=20, hs_across=20, surf_method=1, algorithm=1)
+    starts = _grid(4, 4, 20, 100, 20)
+    g, o = run_both(eng, ref, sec, cfg, starts, MODE_STAGED)
+    c = compare(g, o)
+    _report("staged sinc", c)
+    _assert_parity(c, zoom_allow=1)
+
+
+def test_staged_real_image_and_small_window(eng, scene1):
+    ref, sec = scene1
+    ref_r = np.abs(ref).astype(np.float32)
+    sec_r = np.abs(sec).astype(np.float32)
+    cfg = O.AmpcorConfig(window_h=32, window_w=32, hs_down=8, hs_across=8)
+    starts = _grid(5, 5, 8, 90, 8)
+    g, o = run_both(eng, ref_r, sec_r, cfg, starts, MODE_STAGED)
+    c = compare(g, o)
+    _report("staged real", c)
+    _assert_parity(c, zoom_allow=1)
+
+
+def test_staged_edges_and_errors(eng, scene1):
+    ref, sec = scene1
+    cfg = O.AmpcorConfig(window_h=64, window_w=64, hs_down=16, hs_across=16)
+    H, W = ref.shape
+    S = 64 + 32
+    # windows touching the first and last rows/columns of the images
+    rd = np.array([16, H - 64, 16, H - 64 - 16])
+    ra = np.array([16, W - 64, W - 64, 40])
+    sd = np.array([0, H - S, 0, H - S])
+    sa = np.array([0, W - S, W - S, 10])
+    g, o = run_both(eng, ref, sec, cfg, (rd, ra, sd, sa), MODE_STAGED)
+    c = compare(g, o)
+    _report("staged edges", c)
+    _assert_parity(c, zoom_allow=1)
+    # a chip one pixel past the edge is rejected before any kernel runs
+    dev = torch.device("cuda", 0)
+    t = torch.from_numpy(ref).to(dev)
+    bad = [torch.tensor(v, dtype=torch.int32, device=dev) for v in ([H - 63], [0], [0], [0])]
+    with pytest.raises(IndexError):
+        eng.run_windows(t, t, *bad, mode=MODE_STAGED)
+    # zero windows is a no-op
+    empty = [torch.zeros(0, dtype=torch.int32, device=dev) for _ in range(4)]
+    r = eng.run_windows(t, t, *empty, mode=MODE_STAGED)
+    assert r["offset"].shape == (0, 2)
+
+
+def test_staged_chunk_invariance(eng, scene1):
+    ref, sec = scene1
+    cfg = O.AmpcorConfig(hs_down=16, hs_across=16)
+    starts = _grid(6, 6, 16, 64, 16)
+    from .gpu_helpers import params_from_cfg
+    eng.setup(params_from_cfg(cfg))
+    dev = torch.device("cuda", 0)
+    rt, stt = torch.from_numpy(ref).to(dev), torch.from_numpy(sec).to(dev)
+    st = [torch.as_tensor(np.asarray(s, np.int32)).to(dev) for s in starts]
+    a = eng.run_windows(rt, stt, *st, mode=MODE_STAGED, chunk=1)
+    b = eng.run_windows(rt, stt, *st, mode=MODE_STAGED, chunk=36)
+    torch.cuda.synchronize()
+    for k in ("offset", "snr", "cov"):
+        assert torch.equal(a[k], b[k]), k

@@ -1,0 +1,117 @@
+"""Host-side logic and the C ABI surface (no GPU needed)."""
+import ctypes
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from isce2_amd import PyCuAmpcor, raster
+from isce2_amd import _lib
+from oracle import ampcor_oracle as O
+
+ROOT = Path(__file__).resolve().parents[1]
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    header = (ROOT / "include" / "isce2_amd.h").read_text()
+    declared = set(re.findall(r"^\s*(?:int|const char \*)\s*(ampcor_\w+)\s*\(", header, re.M))
+    assert declared == set(_lib.EXPORTS), declared ^ set(_lib.EXPORTS)
+    for name in declared:
+        assert hasattr(lib, name), name
+    assert lib.ampcor_version().decode().startswith("isce2_amd")
+
+
+def test_create_fails_cleanly_without_device():
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is present")
+    lib = _lib.load()
+    h = ctypes.c_void_p()
+    code = lib.ampcor_create(0, ctypes.byref(h))
+    assert code != 0 and not h.value
+    assert lib.ampcor_last_error(None)
+
+
+def test_attribute_surface_and_defaults():
+    p = PyCuAmpcor()
+    # defaults of cuAmpcorParameter.cpp:18-64 through the PyCuAmpcor.cpp property names
+    assert (p.algorithm, p.deviceID, p.nStreams, p.derampMethod) == (0, 0, 1, 1)
+    assert (p.windowSizeHeight, p.windowSizeWidth, p.halfSearchRangeDown, p.halfSearchRangeAcross) == (64, 64, 20, 20)
+    assert (p.skipSampleDown, p.skipSampleAcross, p.rawDataOversamplingFactor) == (64, 64, 2)
+    assert (p.corrSurfaceZoomInWindow, p.corrSurfaceOverSamplingFactor, p.corrSurfaceOverSamplingMethod) == (16, 16, 0)
+    assert (p.corrStatWindowSize, p.mergeGrossOffset, p.useMmap, p.mmapSize) == (21, 0, 1, 1)
+    assert p.referenceImageName == "reference.slc" and p.offsetImageName == "DenseOffset.off"
+    with pytest.raises(AttributeError):
+        p.corrSufaceOverSamplingMethod = 1          # misspelling seen in a caller
+    with pytest.raises(TypeError):
+        p.windowSizeHeight = 64.0
+    p.windowSizeHeight = np.int64(32)
+    assert p.windowSizeHeight == 32 and isinstance(p.windowSizeHeight, int)
+    p.grossOffsetAcrossStatic = 3                   # legacy alias accepted
+    p.numberWindowDown, p.numberWindowAcross = 3, 4
+    assert p.numberWindows == 12
+    with pytest.raises(AttributeError):
+        p.numberWindows = 5
+
+
+def test_setup_and_start_pixels_match_oracle():
+    p = PyCuAmpcor()
+    p.numberWindowDown, p.numberWindowAcross = 3, 5
+    p.halfSearchRangeDown = p.halfSearchRangeAcross = 3   # zoom clamp 16 -> 12
+    p.referenceStartPixelDownStatic, p.referenceStartPixelAcrossStatic = 11, 7
+    p.skipSampleDown, p.skipSampleAcross = 40, 30
+    p.setupParams()
+    assert p.corrSurfaceZoomInWindow == 12
+    p.setConstantGrossOffset(2, -1)
+    st = p._state
+    rd, ra, sd, sa, gd, ga = O.start_pixels(3, 5, 11, 7, 40, 30, 2, -1, 3, 3)
+    for k, v in zip(("ref_d", "ref_a", "sec_d", "sec_a", "gross_d", "gross_a"), (rd, ra, sd, sa, gd, ga)):
+        np.testing.assert_array_equal(st[k], v)
+    vd = np.arange(15, dtype=np.int32)
+    va = -np.arange(15, dtype=np.int32)
+    p.setVaryingGrossOffset(vd, va)
+    np.testing.assert_array_equal(p._state["sec_d"], rd + vd - 3)
+    np.testing.assert_array_equal(p._state["sec_a"], ra + va - 3)
+
+
+def test_setup_errors_instead_of_exit():
+    p = PyCuAmpcor()
+    p.numberWindowDown = 0
+    with pytest.raises(ValueError, match="Incorrect number of windows"):
+        p.setupParams()
+
+
+def test_check_range_strict_rule():
+    p = PyCuAmpcor()
+    p.referenceImageHeight = p.secondaryImageHeight = 200
+    p.referenceImageWidth = p.secondaryImageWidth = 200
+    p.numberWindowDown = p.numberWindowAcross = 2
+    p.referenceStartPixelDownStatic = p.referenceStartPixelAcrossStatic = 20
+    p.skipSampleDown = p.skipSampleAcross = 64
+    p.setupParams()
+    p.setConstantGrossOffset(0, 0)
+    p.checkPixelInImageRange()                      # 20+64+104-20 = 168 < 200
+    p.referenceImageWidth = 148                     # 84 + 64 = 148 >= 148 -> strict failure
+    with pytest.raises(IndexError, match=r"Reference Window end pixel out ot range in Across, window \(0,1\)"):
+        p.checkPixelInImageRange()
+
+
+def test_raster_readers_roundtrip(tmp_path):
+    a = (np.arange(12 * 10).reshape(12, 10) + 1j * np.arange(120).reshape(12, 10)[::-1]).astype(np.complex64)
+    fn = tmp_path / "img.slc"
+    a.tofile(fn)
+    raster.write_vrt(str(fn), 10, 12, 1, "CFloat32")
+    raster.write_isce_xml(str(fn), 10, 12, 1, "CFLOAT")
+    np.testing.assert_array_equal(raster.open_raster(str(fn) + ".vrt"), a)
+    np.testing.assert_array_equal(raster.open_raster(str(fn) + ".xml"), a)
+    np.testing.assert_array_equal(raster.open_raster(str(fn), 12, 10), a)
+    # a 3-band BIP float file read through its VRT gives band 1
+    b = np.random.default_rng(0).random((5, 7, 3)).astype(np.float32)
+    fb = tmp_path / "cov.bip"
+    b.tofile(fb)
+    raster.write_vrt(str(fb), 7, 5, 3, "Float32", "BIP")
+    np.testing.assert_array_equal(raster.open_raster(str(fb) + ".vrt"), b[:, :, 0])
+    with pytest.raises(raster.RasterError):
+        raster.open_raster(str(tmp_path / "missing.slc"))
