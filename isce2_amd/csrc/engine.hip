@@ -171,7 +171,7 @@ void Engine::run_staged(const RunArgs &a) {
     const hipStream_t st = a.stream;
     const int W_h = p.window_h, W_w = p.window_w;
     const int S_h = sz.search_h, S_w = sz.search_w;
-    const int chunk = a.chunk > 0 ? a.chunk : 1024;
+    const int chunk = std::min(a.chunk > 0 ? a.chunk : 1024, 65535);
     const int mode = p.image_complex ? (p.deramp == 0 ? 1 : 0) : 2;
     // correlation transform sizes: the search chip sizes (cuCorrFrequency.cu:16-43)
     const int L1h = S_h, L1w = S_w, L2h = sz.os_search_h, L2w = sz.os_search_w;
@@ -219,12 +219,12 @@ void Engine::run_staged(const RunArgs &a) {
                            int64_t(n) * S_h * S_w);
         // raw correlation (cuAmpcorChunk.cpp:48-52)
         hipLaunchKernelGGL(k_pack_corr<T>, grid2(L1h * L1w, n), dim3(256), 0, st, r_ref, W_h, W_w, r_sec, S_h, S_w,
-                           zc, L1h);
+                           zc, L1h, L1w);
         fft2<T, -1>(*this, zc, n, L1h, L1w, st);
-        hipLaunchKernelGGL(k_spec_product<T>, grid2(L1h * L1w, n), dim3(256), 0, st, zc, L1h,
+        hipLaunchKernelGGL(k_spec_product<T>, grid2(L1h * L1w, n), dim3(256), 0, st, zc, L1h, L1w,
                            T(1.0 / (double(L1h) * L1w)));
         fft2<T, +1>(*this, zc, n, L1h, L1w, st);
-        hipLaunchKernelGGL(k_crop_real<T>, grid2(sz.raw_corr_h * sz.raw_corr_w, n), dim3(256), 0, st, zc, L1h,
+        hipLaunchKernelGGL(k_crop_real<T>, grid2(sz.raw_corr_h * sz.raw_corr_w, n), dim3(256), 0, st, zc, L1h, L1w,
                            corr, sz.raw_corr_h, sz.raw_corr_w);
         // normalize (cuAmpcorChunk.cpp:60)
         hipLaunchKernelGGL(k_sat<T>, dim3(n), dim3(256), 0, st, r_sec, S_h, S_w, sat1, sat2);
@@ -257,12 +257,12 @@ void Engine::run_staged(const RunArgs &a) {
                            int64_t(n) * L2h * L2w);
         // oversampled correlation + normalize (cuAmpcorChunk.cpp:148-161)
         hipLaunchKernelGGL(k_pack_corr<T>, grid2(L2h * L2w, n), dim3(256), 0, st, r_ref_os, sz.os_win_h,
-                           sz.os_win_w, r_sec_os, L2h, L2w, zc, L2h);
+                           sz.os_win_w, r_sec_os, L2h, L2w, zc, L2h, L2w);
         fft2<T, -1>(*this, zc, n, L2h, L2w, st);
-        hipLaunchKernelGGL(k_spec_product<T>, grid2(L2h * L2w, n), dim3(256), 0, st, zc, L2h,
+        hipLaunchKernelGGL(k_spec_product<T>, grid2(L2h * L2w, n), dim3(256), 0, st, zc, L2h, L2w,
                            T(1.0 / (double(L2h) * L2w)));
         fft2<T, +1>(*this, zc, n, L2h, L2w, st);
-        hipLaunchKernelGGL(k_crop_real<T>, grid2(sz.os_corr_h * sz.os_corr_w, n), dim3(256), 0, st, zc, L2h,
+        hipLaunchKernelGGL(k_crop_real<T>, grid2(sz.os_corr_h * sz.os_corr_w, n), dim3(256), 0, st, zc, L2h, L2w,
                            corr_os, sz.os_corr_h, sz.os_corr_w);
         hipLaunchKernelGGL(k_sat<T>, dim3(n), dim3(256), 0, st, r_sec_os, L2h, L2w, sat1, sat2);
         hipLaunchKernelGGL(k_normalize<T>, dim3(n), dim3(256), 0, st, corr_os, sz.os_corr_h, sz.os_corr_w,

@@ -114,14 +114,14 @@ __global__ void __launch_bounds__(256) k_sub_mean(T *data, int size) {
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ void k_pack_corr(const T *tmpl, int th, int tw, const T *srch, int sh, int sw,
-                            cx<T> *z, int L) {
+                            cx<T> *z, int Lh, int Lw) {
     const int win = blockIdx.y;
-    const int tot = L * L;
+    const int tot = Lh * Lw;
     const T *t = tmpl + size_t(win) * th * tw;
     const T *s = srch + size_t(win) * sh * sw;
     cx<T> *o = z + size_t(win) * tot;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += gridDim.x * blockDim.x) {
-        const int r = i / L, c = i - r * L;
+        const int r = i / Lw, c = i - r * Lw;
         const T a = (r < th && c < tw) ? t[r * tw + c] : T(0);
         const T b = (r < sh && c < sw) ? s[r * sw + c] : T(0);
         o[i] = {a, b};
@@ -131,14 +131,14 @@ __global__ void k_pack_corr(const T *tmpl, int th, int tw, const T *srch, int sh
 // P[k] = conj(T[k]) S[k] / L^2 with T = (Z[k] + conj Z[-k])/2, S = (Z[k] - conj Z[-k])/(2i);
 // P[-k] = conj(P[k]). One thread per (k, -k) pair.
 template <class T>
-__global__ void k_spec_product(cx<T> *z, int L, T scale) {
+__global__ void k_spec_product(cx<T> *z, int Lh, int Lw, T scale) {
     const int win = blockIdx.y;
-    cx<T> *Z = z + size_t(win) * L * L;
-    const int tot = L * L;
+    cx<T> *Z = z + size_t(win) * Lh * Lw;
+    const int tot = Lh * Lw;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += gridDim.x * blockDim.x) {
-        const int r = i / L, c = i - r * L;
-        const int rn = r ? L - r : 0, cn = c ? L - c : 0;
-        const int j = rn * L + cn;
+        const int r = i / Lw, c = i - r * Lw;
+        const int rn = r ? Lh - r : 0, cn = c ? Lw - c : 0;
+        const int j = rn * Lw + cn;
         if (j < i) continue;   // pair handled by the smaller index
         const cx<T> a = Z[i], bc = conj(Z[j]);
         const cx<T> tk = (a + bc) * T(0.5);
@@ -152,12 +152,12 @@ __global__ void k_spec_product(cx<T> *z, int L, T scale) {
 
 // real part of the top-left oh x ow block
 template <class T>
-__global__ void k_crop_real(const cx<T> *z, int L, T *out, int oh, int ow) {
+__global__ void k_crop_real(const cx<T> *z, int Lh, int Lw, T *out, int oh, int ow) {
     const int win = blockIdx.y;
     const int tot = oh * ow;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < tot; i += gridDim.x * blockDim.x) {
         const int r = i / ow, c = i - r * ow;
-        out[size_t(win) * tot + i] = z[size_t(win) * L * L + r * L + c].x;
+        out[size_t(win) * tot + i] = z[size_t(win) * Lh * Lw + r * Lw + c].x;
     }
 }
 
@@ -470,7 +470,7 @@ __global__ void k_sinc(const T *in, int nin, T *out, int nout, const int32_t *sh
 }
 
 // final offset (cuOffset.cu:137-146)
-__global__ void k_subpixel(const int32_t *start, const int32_t *zidx, int n, float ratio, int hs_down,
+static __global__ void k_subpixel(const int32_t *start, const int32_t *zidx, int n, float ratio, int hs_down,
                            int hs_across, float *offset) {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
     if (w >= n) return;
