@@ -1,0 +1,165 @@
+#!/usr/bin/env python3
+"""Throughput benchmark: dense offset tracking on one MI355X.
+
+Workload (one "step" = one pass over the whole offset field): a complex64
+8192x4096 speckle pair whose secondary is warped by a smooth offset field
+(2nd-order polynomial, max 8 px, plus N(0, 0.05) px per 256x256 tile), window
+64x64, half search 20, skip 32 (252 x 124 = 31,248 windows), raw oversampling
+2, 16x16 zoom surface oversampled 16x, derampMethod 1. Images are resident in
+HBM before the timed region. Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# model FLOPs per window at the reference's transform sizes (5 n log2 n per
+# complex FFT, 2.5 n log2 n per real FFT, plus pointwise terms), split over the
+# engine's kernels; they sum to the 12.364 MFLOP/window of the whole pipeline
+MFLOP_PER_WINDOW = 12.364
+KERNEL_MFLOP = {"k_coarse": 1.235, "k_oversample_ref": 1.434, "k_oversample_search": 1.817, "k_fine": 7.878}
+FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--mode", default="auto", choices=["auto", "fused", "staged"])
+    ap.add_argument("--deramp", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-plain", action="store_true", help="skip the staged-pipeline comparison")
+    ap.add_argument("--cpu-windows", type=int, default=384)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(json.dumps({"error": "this engine runs on one GPU per process; bench.py measures N=1 only "
+                                   "(windows are independent, see the sharding notes in DESIGN.md)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    from isce2_amd.engine import MODE_AUTO, MODE_FUSED, MODE_STAGED, Engine, WindowParams
+    from isce2_amd.synth import speckle_pair_warp
+
+    dev = torch.device("cuda", 0)
+    H, Wd, W, HS, SKIP = 8192, 4096, 64, 20, 32
+    t0 = time.time()
+    ref_np, sec_np, dd, da = speckle_pair_warp(H, Wd, max_amp=8.0, jitter=0.05, tile=256, seed=0, device=dev)
+    gen_s = time.time() - t0
+    nd = (H - 2 * HS - W) // SKIP
+    na = (Wd - 2 * HS - W) // SKIP
+    rows, cols = np.meshgrid(np.arange(nd), np.arange(na), indexing="ij")
+    ref_d = (HS + rows * SKIP).ravel()
+    ref_a = (HS + cols * SKIP).ravel()
+    sec_d, sec_a = ref_d - HS, ref_a - HS
+    n = nd * na
+    ref = torch.from_numpy(ref_np).to(dev)
+    sec = torch.from_numpy(sec_np).to(dev)
+    starts = [torch.as_tensor(a.astype(np.int32)).to(dev) for a in (ref_d, ref_a, sec_d, sec_a)]
+
+    eng = Engine(0)
+    params = WindowParams(window_h=W, window_w=W, hs_down=HS, hs_across=HS, deramp=args.deramp)
+    sizes = eng.setup(params)
+    mode = {"auto": MODE_AUTO, "fused": MODE_FUSED, "staged": MODE_STAGED}[args.mode]
+    out = None
+    for _ in range(args.warmup):
+        out = eng.run_windows(ref, sec, *starts, mode=mode, out=out)
+    torch.cuda.synchronize(dev)
+    eng.profile(True)
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = eng.run_windows(ref, sec, *starts, mode=mode, out=out)
+    torch.cuda.synchronize(dev)
+    elapsed = time.perf_counter() - t0
+    prof = eng.profile_report()
+    eng.profile(False)
+    wps = n * args.steps / elapsed
+    ms_step = 1000 * elapsed / args.steps
+    gpu_out = {k: v.cpu().numpy() for k, v in out.items()}
+
+    # roofline of the dominant kernel (largest total time)
+    roof = None
+    kernels = {}
+    if prof:
+        for k, v in prof.items():
+            avg_ms = v["ms"] / max(1, v["launches"])
+            per_launch = v["windows"] / max(1, v["launches"])
+            kernels[k] = dict(avg_ms=round(avg_ms, 4), launches=v["launches"], total_ms=round(v["ms"], 3),
+                              model_tflops=round(KERNEL_MFLOP.get(k, 0) * 1e6 * per_launch / (avg_ms * 1e-3) / 1e12, 2))
+        dom = max(prof, key=lambda k: prof[k]["ms"])
+        v = prof[dom]
+        avg_s = v["ms"] / v["launches"] * 1e-3
+        achieved = KERNEL_MFLOP[dom] * 1e6 * (v["windows"] / v["launches"]) / avg_s / 1e12
+        roof = {"bound": "valu", "kernel": dom, "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
+                "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "per_unit": f"{KERNEL_MFLOP[dom]} MFLOP/window (reference-size FFT model)",
+                "windows_per_launch": v["windows"] // v["launches"], "avg_launch_ms": round(v["ms"] / v["launches"], 4)}
+
+    # plain (staged, one batched kernel per stage) for comparison
+    plain = None
+    if not args.no_plain and args.mode != "staged":
+        torch.cuda.synchronize(dev)
+        o2 = eng.run_windows(ref, sec, *starts, mode=MODE_STAGED, chunk=1024)
+        torch.cuda.synchronize(dev)
+        t1 = time.perf_counter()
+        o2 = eng.run_windows(ref, sec, *starts, mode=MODE_STAGED, chunk=1024)
+        torch.cuda.synchronize(dev)
+        t_plain = time.perf_counter() - t1
+        o2n = {k: v.cpu().numpy() for k, v in o2.items()}
+        plain = {"windows_per_s": round(n / t_plain, 1), "ms_per_step": round(1000 * t_plain, 2),
+                 "speedup_fused_over_staged": round(wps / (n / t_plain), 2),
+                 "max_abs_offset_diff_vs_fused": float(np.abs(o2n["offset"] - gpu_out["offset"]).max())}
+
+    # CPU baseline: the FP64 oracle (a port of the per-window pipeline) on a bounded sample
+    cpu = None
+    accuracy = None
+    if not args.no_cpu:
+        from oracle import ampcor_oracle as O
+        cfg = O.AmpcorConfig(window_h=W, window_w=W, hs_down=HS, hs_across=HS, deramp=args.deramp)
+        rng = np.random.default_rng(7)
+        sel = np.sort(rng.choice(n, size=min(args.cpu_windows, n), replace=False))
+        tc = time.perf_counter()
+        o = O.run_windows(ref_np, sec_np, ref_d[sel], ref_a[sel], sec_d[sel], sec_a[sel], cfg)
+        cpu_s = time.perf_counter() - tc
+        cpu = {"value": round(len(sel) / cpu_s, 2), "unit": "windows/s", "cores": 1, "kind": "port",
+               "sample": f"{len(sel)} random windows of the same 8192x4096 field, FP64 NumPy oracle "
+                         f"(single thread), {cpu_s:.1f} s"}
+        g_off = gpu_out["offset"][sel]
+        err = np.abs(g_off - o["offset"]).max(axis=1)
+        accuracy = {"windows_checked": int(len(sel)), "max_offset_err_px": float(err.max()),
+                    "p99_offset_err_px": float(np.percentile(err, 99)),
+                    "windows_over_0.01px": int((err > 0.01).sum())}
+        # the true field: mean |offset - d(window centre)|
+        cr = ref_d[sel] + W // 2
+        cc = ref_a[sel] + W // 2
+        truth = np.stack([dd[cr, cc], da[cr, cc]], axis=1)
+        accuracy["median_abs_err_vs_true_field_px"] = float(np.median(np.abs(g_off - truth)))
+
+    line = {
+        "metric": "windows/sec @64x64 ±20px, offset err<0.01px vs FP64; serve p50/p99 latency",
+        "value": round(wps, 1), "unit": "windows/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3), "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (FP64 window sums)", "data": "synthetic",
+        "config": {"workload": "plain single-GPU offset field: complex64 8192x4096 speckle pair, polynomial warp "
+                               "+-8 px, window 64x64, half search 20, skip 32, 31248 windows, raw ovs 2, "
+                               "surface ovs 16, derampMethod %d" % args.deramp,
+                   "windows": n, "mode": args.mode, "engine_path": "fused" if sizes["fused"] and args.mode != "staged" else "staged"},
+        "roofline": roof, "cpu_baseline": cpu, "kernels": kernels, "plain_staged": plain, "accuracy": accuracy,
+        "scene_gen_s": round(gen_s, 1),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

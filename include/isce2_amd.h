@@ -117,6 +117,15 @@ int ampcor_run_windows_debug(ampcor_ctx *ctx,
                        int32_t *out_zoom_idx, float *out_maxval,
                        int32_t chunk, int32_t mode, void *stream);
 
+/* Per-kernel timing with HIP events recorded on the launch stream around every
+ * kernel of ampcor_run_windows. No counterpart in the reference (its timing is
+ * host wall clock, cuDenseOffsets.py:465,484-485); for benchmarks and profiling.
+ * ampcor_profile(ctx, 1) resets and enables, (ctx, 0) disables.
+ * ampcor_profile_report synchronizes the recorded events and writes lines
+ * "kernel_name launches total_ms windows\n" into buf (NUL-terminated). */
+int ampcor_profile(ampcor_ctx *ctx, int32_t enable);
+int ampcor_profile_report(ampcor_ctx *ctx, char *buf, int64_t len);
+
 /* Text of the last error on this context (or of the last ampcor_create failure
  * when ctx is NULL). */
 const char *ampcor_last_error(const ampcor_ctx *ctx);

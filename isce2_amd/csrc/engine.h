@@ -54,6 +54,25 @@ struct Engine {
     DevBuf sinc_table;
     int sinc_dec = 4096, sinc_len = 8;
 
+    // per-kernel event timing (ampcor_profile)
+    struct ProfRec { std::string name; hipEvent_t a, b; int windows; };
+    bool profiling = false;
+    std::vector<ProfRec> prof_pending;
+    std::vector<hipEvent_t> event_pool;
+    struct ProfAcc { long launches = 0; double ms = 0.0; long windows = 0; };
+    std::map<std::string, ProfAcc> prof_acc;
+    hipEvent_t take_event();
+    template <class F> void timed(const char *name, int windows, hipStream_t st, F &&launch) {
+        if (!profiling) { launch(); return; }
+        ProfRec r{name, take_event(), take_event(), windows};
+        (void)hipEventRecord(r.a, st);
+        launch();
+        (void)hipEventRecord(r.b, st);
+        prof_pending.push_back(r);
+    }
+    void prof_collect();
+    ~Engine();
+
     const FftPlan &plan(int n);
     template <class U> U *buf(const std::string &name, size_t count) {
         auto &b = bufs[name];

@@ -27,6 +27,37 @@ DevBuf::~DevBuf() {
     if (ptr) (void)hipFree(ptr);
 }
 
+hipEvent_t Engine::take_event() {
+    if (!event_pool.empty()) {
+        hipEvent_t e = event_pool.back();
+        event_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e;
+    HIP_OK(hipEventCreate(&e));
+    return e;
+}
+
+void Engine::prof_collect() {
+    for (auto &r : prof_pending) {
+        HIP_OK(hipEventSynchronize(r.b));
+        float ms = 0.f;
+        HIP_OK(hipEventElapsedTime(&ms, r.a, r.b));
+        auto &acc = prof_acc[r.name];
+        acc.launches += 1;
+        acc.ms += ms;
+        acc.windows += r.windows;
+        event_pool.push_back(r.a);
+        event_pool.push_back(r.b);
+    }
+    prof_pending.clear();
+}
+
+Engine::~Engine() {
+    for (auto &r : prof_pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
+    for (auto e : event_pool) (void)hipEventDestroy(e);
+}
+
 FftDesc make_schedule(int n) {
     FftDesc d{};
     d.n = n;
@@ -412,6 +443,29 @@ int ampcor_run_windows(ampcor_ctx *ctx, const void *ref_img, int64_t ref_h, int6
     return ampcor_run_windows_debug(ctx, ref_img, ref_h, ref_w, ref_pitch, sec_img, sec_h, sec_w, sec_pitch, ref_d,
                                     ref_a, sec_d, sec_a, n, out_offset, out_snr, out_cov, nullptr, nullptr,
                                     nullptr, nullptr, nullptr, chunk, mode, stream);
+}
+
+int ampcor_profile(ampcor_ctx *ctx, int32_t enable) {
+    if (!ctx) return AMPCOR_E_ARG;
+    return guarded(ctx, [&] {
+        ctx->eng.prof_collect();
+        ctx->eng.prof_acc.clear();
+        ctx->eng.profiling = enable != 0;
+    });
+}
+
+int ampcor_profile_report(ampcor_ctx *ctx, char *buf, int64_t len) {
+    if (!ctx || !buf || len <= 0) return AMPCOR_E_ARG;
+    return guarded(ctx, [&] {
+        ctx->eng.prof_collect();
+        std::string out;
+        for (auto &kv : ctx->eng.prof_acc)
+            out += kv.first + " " + std::to_string(kv.second.launches) + " " + std::to_string(kv.second.ms) + " " +
+                   std::to_string(kv.second.windows) + "\n";
+        const size_t n = std::min<size_t>(out.size(), size_t(len - 1));
+        std::memcpy(buf, out.data(), n);
+        buf[n] = 0;
+    });
 }
 
 const char *ampcor_last_error(const ampcor_ctx *ctx) {

@@ -452,13 +452,15 @@ void Engine::run_fused(const RunArgs &a) {
 
     for (int w0 = 0; w0 < a.n; w0 += nb) {
         const int n = std::min(nb, a.n - w0);
-        if (W == 64) {
-            if (L1 == 128) launch_coarse<64, 128>(K1, ca, pl1.desc32, n, w0, lds_c, st);
-            else launch_coarse<64, 64>(K1, ca, pl1.desc32, n, w0, lds_c, st);
-        } else {
-            if (L1 == 128) launch_coarse<32, 128>(K1, ca, pl1.desc32, n, w0, lds_c, st);
-            else launch_coarse<32, 64>(K1, ca, pl1.desc32, n, w0, lds_c, st);
-        }
+        timed("k_coarse", n, st, [&] {
+            if (W == 64) {
+                if (L1 == 128) launch_coarse<64, 128>(K1, ca, pl1.desc32, n, w0, lds_c, st);
+                else launch_coarse<64, 64>(K1, ca, pl1.desc32, n, w0, lds_c, st);
+            } else {
+                if (L1 == 128) launch_coarse<32, 128>(K1, ca, pl1.desc32, n, w0, lds_c, st);
+                else launch_coarse<32, 64>(K1, ca, pl1.desc32, n, w0, lds_c, st);
+            }
+        });
         // chips are addressed with absolute window indices; outputs of the
         // oversampler are per chunk, hence the pointer shifts by w0
         OsArgs oref{a.ref, a.ref_pitch, mode, a.ref_d, a.ref_a, nullptr, p.deramp, h_ref,
@@ -469,15 +471,27 @@ void Engine::run_fused(const RunArgs &a) {
                     p.hs_across, 1.0f / float(p.surf_ovs * p.raw_ovs), static_cast<const cf *>(pl256.tw32.ptr),
                     zidx, a.offset};
         if (W == 64) {
-            hipLaunchKernelGGL((k_oversample<64>), dim3(n), dim3(NT), lds_or, st, oref, plW.desc32, w0);
-            hipLaunchKernelGGL((k_oversample<72>), dim3(n), dim3(NT), lds_oz, st, osec, plZ.desc32, w0);
-            hipLaunchKernelGGL((k_fine<128, 144, 144, 6>), dim3(n), dim3(NT), lds_f, st, fa, pl2.desc32,
-                               pl16.desc32, w0);
+            timed("k_oversample_ref", n, st, [&] {
+                hipLaunchKernelGGL((k_oversample<64>), dim3(n), dim3(NT), lds_or, st, oref, plW.desc32, w0);
+            });
+            timed("k_oversample_search", n, st, [&] {
+                hipLaunchKernelGGL((k_oversample<72>), dim3(n), dim3(NT), lds_oz, st, osec, plZ.desc32, w0);
+            });
+            timed("k_fine", n, st, [&] {
+                hipLaunchKernelGGL((k_fine<128, 144, 144, 6>), dim3(n), dim3(NT), lds_f, st, fa, pl2.desc32,
+                                   pl16.desc32, w0);
+            });
         } else {
-            hipLaunchKernelGGL((k_oversample<32>), dim3(n), dim3(NT), lds_or, st, oref, plW.desc32, w0);
-            hipLaunchKernelGGL((k_oversample<40>), dim3(n), dim3(NT), lds_oz, st, osec, plZ.desc32, w0);
-            hipLaunchKernelGGL((k_fine<64, 80, 80, 3>), dim3(n), dim3(NT), lds_f, st, fa, pl2.desc32,
-                               pl16.desc32, w0);
+            timed("k_oversample_ref", n, st, [&] {
+                hipLaunchKernelGGL((k_oversample<32>), dim3(n), dim3(NT), lds_or, st, oref, plW.desc32, w0);
+            });
+            timed("k_oversample_search", n, st, [&] {
+                hipLaunchKernelGGL((k_oversample<40>), dim3(n), dim3(NT), lds_oz, st, osec, plZ.desc32, w0);
+            });
+            timed("k_fine", n, st, [&] {
+                hipLaunchKernelGGL((k_fine<64, 80, 80, 3>), dim3(n), dim3(NT), lds_f, st, fa, pl2.desc32,
+                                   pl16.desc32, w0);
+            });
         }
         HIP_OK(hipGetLastError());
     }

@@ -119,6 +119,20 @@ class Engine:
         check(code, self._h)
         return res
 
+    def profile(self, enable: bool):
+        """Reset and enable (or disable) per-kernel HIP event timing."""
+        check(self.lib.ampcor_profile(self._h, int(bool(enable))), self._h)
+
+    def profile_report(self) -> dict:
+        """{kernel: {"launches", "ms", "windows"}} accumulated since profile(True)."""
+        buf = C.create_string_buffer(1 << 16)
+        check(self.lib.ampcor_profile_report(self._h, buf, len(buf)), self._h)
+        out = {}
+        for line in buf.value.decode().splitlines():
+            name, launches, ms, windows = line.split()
+            out[name] = dict(launches=int(launches), ms=float(ms), windows=int(windows))
+        return out
+
     @staticmethod
     def version() -> str:
         return _lib.load().ampcor_version().decode()

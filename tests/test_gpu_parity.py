@@ -107,6 +107,54 @@ def test_staged_edges_and_errors(eng, scene1):
     assert r["offset"].shape == (0, 2)
 
 
+@pytest.mark.parametrize("W,hs,deramp", [(64, 16, 1), (64, 20, 0), (64, 32, 1), (32, 8, 1), (32, 16, 0),
+                                         (32, 32, 1)])
+def test_fused_matches_oracle(eng, scene1, W, hs, deramp):
+    ref, sec = scene1
+    cfg = O.AmpcorConfig(window_h=W, window_w=W, hs_down=hs, hs_across=hs, deramp=deramp)
+    S = W + 2 * hs
+    nd = (512 - S) // 72
+    starts = _grid(nd, nd, hs, 72, hs)
+    g, o = run_both(eng, ref, sec, cfg, starts, MODE_FUSED, chunk=5)
+    c = compare(g, o)
+    _report(f"fused W={W} hs={hs} deramp={deramp}", c)
+    _assert_parity(c, zoom_allow=1)
+
+
+def test_fused_real_image_and_edges(eng, scene1):
+    ref, sec = scene1
+    ref_r = np.abs(ref).astype(np.float32)
+    sec_r = (np.abs(sec) + 0.1).astype(np.float32)
+    cfg = O.AmpcorConfig(hs_down=20, hs_across=20)
+    H, W = ref.shape
+    S = 104
+    rd = np.array([20, H - 64, 20, H - 84, 200])
+    ra = np.array([20, W - 64, W - 64, 60, 200])
+    sd = np.array([0, H - S, 0, H - S, 170])
+    sa = np.array([0, W - S, W - S, 20, 190])
+    g, o = run_both(eng, ref_r, sec_r, cfg, (rd, ra, sd, sa), MODE_FUSED)
+    c = compare(g, o)
+    _report("fused real/edges", c)
+    _assert_parity(c, zoom_allow=1)
+
+
+def test_fused_equals_staged_integers(eng, scene1):
+    ref, sec = scene1
+    cfg = O.AmpcorConfig(hs_down=20, hs_across=20)
+    starts = _grid(5, 5, 20, 80, 20)
+    from .gpu_helpers import params_from_cfg
+    eng.setup(params_from_cfg(cfg))
+    dev = torch.device("cuda", 0)
+    rt, stt = torch.from_numpy(ref).to(dev), torch.from_numpy(sec).to(dev)
+    st = [torch.as_tensor(np.asarray(s, np.int32)).to(dev) for s in starts]
+    a = eng.run_windows(rt, stt, *st, mode=MODE_FUSED, debug=True)
+    b = eng.run_windows(rt, stt, *st, mode=MODE_STAGED, debug=True)
+    torch.cuda.synchronize()
+    for k in ("maxloc", "start", "shift", "zoom_idx"):
+        assert torch.equal(a[k], b[k]), k
+    torch.testing.assert_close(a["snr"], b["snr"], rtol=1e-4, atol=0)
+
+
 def test_staged_chunk_invariance(eng, scene1):
     ref, sec = scene1
     cfg = O.AmpcorConfig(hs_down=16, hs_across=16)
