@@ -106,6 +106,7 @@ int ampcor_run_windows(ampcor_ctx *ctx,
  * before adjustment), zoom start (i_offsetInitAdjusted), zoom shift
  * (i_maxLocShift) and oversampled-surface argmax (i_offsetZoomIn), plus the raw
  * peak value (r_maxval). Names follow CUAMPCOR_DEBUG (cuAmpcorChunk.cpp:82-198).
+ * The fused path also reports each window's argmax margins (float[n]).
  * Any of the extra pointers may be NULL. */
 int ampcor_run_windows_debug(ampcor_ctx *ctx,
                        const void *ref_img, int64_t ref_h, int64_t ref_w, int64_t ref_pitch,
@@ -115,7 +116,20 @@ int ampcor_run_windows_debug(ampcor_ctx *ctx,
                        float *out_offset, float *out_snr, float *out_cov,
                        int32_t *out_maxloc, int32_t *out_start, int32_t *out_shift,
                        int32_t *out_zoom_idx, float *out_maxval,
+                       float *out_margin_coarse, float *out_margin_fine,
                        int32_t chunk, int32_t mode, void *stream);
+
+/* Precision control. No counterpart in the reference (it runs FP32 only).
+ *   mode 0: FP32 kernels only.
+ *   mode 1 (default): FP32 kernels; windows whose raw-surface or oversampled-
+ *     surface argmax margin (top-1 minus top-2 of the normalized surface) is
+ *     below tau_coarse / tau_fine are re-run through an FP64 pipeline, so the
+ *     integer peaks are those of an FP64 evaluation.
+ *   mode 2: FP64 pipeline for every window.
+ * Negative taus keep the current thresholds. ampcor_precision_stats returns
+ * the windows checked and re-run since the last call. */
+int ampcor_set_precision(ampcor_ctx *ctx, int32_t mode, float tau_coarse, float tau_fine);
+int ampcor_precision_stats(ampcor_ctx *ctx, int64_t *checked, int64_t *rerun);
 
 /* Per-kernel timing with HIP events recorded on the launch stream around every
  * kernel of ampcor_run_windows. No counterpart in the reference (its timing is

@@ -93,9 +93,17 @@ struct Engine {
         float *maxval;
         int32_t chunk, mode;
         hipStream_t stream;
+        float *margin_coarse = nullptr, *margin_fine = nullptr;   // debug: top-1 minus top-2
     };
+    // precision: 0 FP32 only; 1 FP32 + FP64 re-run of windows whose argmax
+    // margins fall below the thresholds; 2 FP64 staged pipeline for every window
+    int precision = 1;
+    float tau_coarse = 2e-5f, tau_fine = 2e-5f;
+    long long prec_checked = 0, prec_rerun = 0;
     void run(const RunArgs &a);
     void run_staged(const RunArgs &a);
+    template <class T> void run_staged_t(const RunArgs &a);
+    void rerun_fp64(const RunArgs &a, const float *mc, const float *mf);
     void run_fused(const RunArgs &a);
 };
 

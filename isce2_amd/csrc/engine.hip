@@ -197,8 +197,9 @@ inline dim3 grid2(int per_img, int nimg) { return dim3(std::min(idivup(per_img, 
 }  // namespace
 
 // cuAmpcorChunk::run (cuAmpcorChunk.cpp:12-217) over chunks of windows.
-void Engine::run_staged(const RunArgs &a) {
-    using T = float;
+template <class T>
+void Engine::run_staged_t(const RunArgs &a) {
+    const std::string tn = sizeof(T) == 8 ? "d_" : "f_";
     const hipStream_t st = a.stream;
     const int W_h = p.window_h, W_w = p.window_w;
     const int S_h = sz.search_h, S_w = sz.search_w;
@@ -209,31 +210,31 @@ void Engine::run_staged(const RunArgs &a) {
     const int Z = sz.zoom, So = sz.surf_out;
     const int nb = std::min(chunk, a.n);
 
-    cx<T> *c_ref = buf<cx<T>>("c_ref", size_t(nb) * W_h * W_w);
-    T *r_ref = buf<T>("r_ref", size_t(nb) * W_h * W_w);
-    cx<T> *c_sec = buf<cx<T>>("c_sec", size_t(nb) * S_h * S_w);
-    T *r_sec = buf<T>("r_sec", size_t(nb) * S_h * S_w);
+    cx<T> *c_ref = buf<cx<T>>(tn + "c_ref", size_t(nb) * W_h * W_w);
+    T *r_ref = buf<T>(tn + "r_ref", size_t(nb) * W_h * W_w);
+    cx<T> *c_sec = buf<cx<T>>(tn + "c_sec", size_t(nb) * S_h * S_w);
+    T *r_sec = buf<T>(tn + "r_sec", size_t(nb) * S_h * S_w);
     const size_t zmax = std::max(size_t(L1h) * L1w, size_t(L2h) * L2w);
-    cx<T> *zc = buf<cx<T>>("zc", size_t(nb) * zmax);
+    cx<T> *zc = buf<cx<T>>(tn + "zc", size_t(nb) * zmax);
     const size_t satmax = std::max(size_t(S_h + 1) * (S_w + 1), size_t(L2h + 1) * (L2w + 1));
-    double *sat1 = buf<double>("sat1", size_t(nb) * satmax);
-    double *sat2 = buf<double>("sat2", size_t(nb) * satmax);
-    T *corr = buf<T>("corr", size_t(nb) * sz.raw_corr_h * sz.raw_corr_w);
-    int32_t *loc = buf<int32_t>("loc", size_t(nb) * 2);
-    T *val = buf<T>("val", nb);
-    int32_t *start = buf<int32_t>("start", size_t(nb) * 2);
-    int32_t *shift = buf<int32_t>("shift", size_t(nb) * 2);
-    int32_t *zidx = buf<int32_t>("zidx", size_t(nb) * 2);
-    cx<T> *c_ref_os = buf<cx<T>>("c_ref_os", size_t(nb) * sz.os_win_h * sz.os_win_w);
-    T *r_ref_os = buf<T>("r_ref_os", size_t(nb) * sz.os_win_h * sz.os_win_w);
-    cx<T> *c_zoom = buf<cx<T>>("c_zoom", size_t(nb) * sz.zoom_in_h * sz.zoom_in_w);
-    cx<T> *c_sec_os = buf<cx<T>>("c_sec_os", size_t(nb) * L2h * L2w);
-    T *r_sec_os = buf<T>("r_sec_os", size_t(nb) * L2h * L2w);
-    T *corr_os = buf<T>("corr_os", size_t(nb) * sz.os_corr_h * sz.os_corr_w);
-    cx<T> *c_surf_in = buf<cx<T>>("c_surf_in", size_t(nb) * Z * Z);
-    cx<T> *c_surf = buf<cx<T>>("c_surf", size_t(nb) * So * So);
-    T *r_surf = buf<T>("r_surf", size_t(nb) * So * So);
-    T *r_adj = buf<T>("r_adj", size_t(nb) * Z * Z);
+    double *sat1 = buf<double>(tn + "sat1", size_t(nb) * satmax);
+    double *sat2 = buf<double>(tn + "sat2", size_t(nb) * satmax);
+    T *corr = buf<T>(tn + "corr", size_t(nb) * sz.raw_corr_h * sz.raw_corr_w);
+    int32_t *loc = buf<int32_t>(tn + "loc", size_t(nb) * 2);
+    T *val = buf<T>(tn + "val", nb);
+    int32_t *start = buf<int32_t>(tn + "start", size_t(nb) * 2);
+    int32_t *shift = buf<int32_t>(tn + "shift", size_t(nb) * 2);
+    int32_t *zidx = buf<int32_t>(tn + "zidx", size_t(nb) * 2);
+    cx<T> *c_ref_os = buf<cx<T>>(tn + "c_ref_os", size_t(nb) * sz.os_win_h * sz.os_win_w);
+    T *r_ref_os = buf<T>(tn + "r_ref_os", size_t(nb) * sz.os_win_h * sz.os_win_w);
+    cx<T> *c_zoom = buf<cx<T>>(tn + "c_zoom", size_t(nb) * sz.zoom_in_h * sz.zoom_in_w);
+    cx<T> *c_sec_os = buf<cx<T>>(tn + "c_sec_os", size_t(nb) * L2h * L2w);
+    T *r_sec_os = buf<T>(tn + "r_sec_os", size_t(nb) * L2h * L2w);
+    T *corr_os = buf<T>(tn + "corr_os", size_t(nb) * sz.os_corr_h * sz.os_corr_w);
+    cx<T> *c_surf_in = buf<cx<T>>(tn + "c_surf_in", size_t(nb) * Z * Z);
+    cx<T> *c_surf = buf<cx<T>>(tn + "c_surf", size_t(nb) * So * So);
+    T *r_surf = buf<T>(tn + "r_surf", size_t(nb) * So * So);
+    T *r_adj = buf<T>(tn + "r_adj", size_t(nb) * Z * Z);
     if (p.surf_method) ensure_sinc_table();
 
     for (int w0 = 0; w0 < a.n; w0 += nb) {
@@ -321,9 +322,74 @@ void Engine::run_staged(const RunArgs &a) {
         if (a.start) HIP_OK(hipMemcpyAsync(a.start + 2 * size_t(w0), start, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToDevice, st));
         if (a.shift) HIP_OK(hipMemcpyAsync(a.shift + 2 * size_t(w0), shift, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToDevice, st));
         if (a.zoom_idx) HIP_OK(hipMemcpyAsync(a.zoom_idx + 2 * size_t(w0), zidx, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToDevice, st));
-        if (a.maxval) HIP_OK(hipMemcpyAsync(a.maxval + w0, val, sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+        if (a.maxval) hipLaunchKernelGGL(k_to_float<T>, dim3(idivup(n, 256)), dim3(256), 0, st, val, a.maxval + w0, n);
         HIP_OK(hipGetLastError());
     }
+}
+
+void Engine::run_staged(const RunArgs &a) { run_staged_t<float>(a); }
+template void Engine::run_staged_t<double>(const RunArgs &a);
+
+namespace {
+__global__ void k_take_i32(const int32_t *src, const int32_t *idx, int32_t *dst, int n, int width) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n * width) dst[i] = src[size_t(idx[i / width]) * width + i % width];
+}
+template <class U>
+__global__ void k_put(const U *src, const int32_t *idx, U *dst, int n, int width) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n * width) dst[size_t(idx[i / width]) * width + i % width] = src[i];
+}
+}  // namespace
+
+// Re-run in FP64 (staged pipeline, T = double) every window whose coarse or
+// fine argmax margin (top-1 minus top-2 of the normalized surface) is below
+// its threshold, and overwrite that window's outputs. FP32 rounding can only
+// reorder near-equal candidates, so this makes the integer peaks those of an
+// FP64 evaluation.
+void Engine::rerun_fp64(const RunArgs &a, const float *mc, const float *mf) {
+    std::vector<float> hc(a.n), hf(a.n);
+    HIP_OK(hipMemcpyAsync(hc.data(), mc, sizeof(float) * a.n, hipMemcpyDeviceToHost, a.stream));
+    HIP_OK(hipMemcpyAsync(hf.data(), mf, sizeof(float) * a.n, hipMemcpyDeviceToHost, a.stream));
+    HIP_OK(hipStreamSynchronize(a.stream));
+    std::vector<int32_t> idx;
+    for (int i = 0; i < a.n; ++i)
+        if (!(hc[i] >= tau_coarse) || !(hf[i] >= tau_fine)) idx.push_back(i);
+    prec_checked += a.n;
+    prec_rerun += long(idx.size());
+    if (idx.empty()) return;
+    const int m = int(idx.size());
+    int32_t *d_idx = buf<int32_t>("p_idx", m);
+    HIP_OK(hipMemcpyAsync(d_idx, idx.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, a.stream));
+    int32_t *st4 = buf<int32_t>("p_starts", size_t(4) * m);
+    const int32_t *srcs[4] = {a.ref_d, a.ref_a, a.sec_d, a.sec_a};
+    for (int k = 0; k < 4; ++k)
+        hipLaunchKernelGGL(k_take_i32, dim3(idivup(m, 256)), dim3(256), 0, a.stream, srcs[k], d_idx, st4 + k * m, m, 1);
+    float *o_off = buf<float>("p_off", size_t(2) * m), *o_snr = buf<float>("p_snr", m);
+    float *o_cov = buf<float>("p_cov", size_t(3) * m), *o_val = buf<float>("p_val", m);
+    int32_t *o_i = buf<int32_t>("p_ints", size_t(8) * m);
+    RunArgs sub = a;
+    sub.ref_d = st4; sub.ref_a = st4 + m; sub.sec_d = st4 + 2 * m; sub.sec_a = st4 + 3 * m;
+    sub.n = m;
+    sub.offset = o_off; sub.snr = o_snr; sub.cov = o_cov; sub.maxval = o_val;
+    sub.maxloc = o_i; sub.start = o_i + 2 * m; sub.shift = o_i + 4 * m; sub.zoom_idx = o_i + 6 * m;
+    sub.chunk = 0;
+    run_staged_t<double>(sub);
+    auto put_f = [&](const float *src, float *dst, int width) {
+        if (dst) hipLaunchKernelGGL(k_put<float>, dim3(idivup(m * width, 256)), dim3(256), 0, a.stream, src, d_idx, dst, m, width);
+    };
+    auto put_i = [&](const int32_t *src, int32_t *dst) {
+        if (dst) hipLaunchKernelGGL(k_put<int32_t>, dim3(idivup(m * 2, 256)), dim3(256), 0, a.stream, src, d_idx, dst, m, 2);
+    };
+    put_f(o_off, a.offset, 2);
+    put_f(o_snr, a.snr, 1);
+    put_f(o_cov, a.cov, 3);
+    put_f(o_val, a.maxval, 1);
+    put_i(sub.maxloc, a.maxloc);
+    put_i(sub.start, a.start);
+    put_i(sub.shift, a.shift);
+    put_i(sub.zoom_idx, a.zoom_idx);
+    HIP_OK(hipGetLastError());
 }
 
 void Engine::run(const RunArgs &a) {
@@ -351,7 +417,8 @@ void Engine::run(const RunArgs &a) {
     const bool fused = (a.mode == AMPCOR_MODE_FUSED) || (a.mode == AMPCOR_MODE_AUTO && sz.fused);
     if (a.mode == AMPCOR_MODE_FUSED && !sz.fused)
         throw Error(AMPCOR_E_UNSUPPORTED, "the fused kernels do not cover this configuration");
-    if (fused) run_fused(a);
+    if (precision == 2) run_staged_t<double>(a);
+    else if (fused) run_fused(a);
     else run_staged(a);
     HIP_OK(hipGetLastError());
 }
@@ -424,15 +491,33 @@ int ampcor_run_windows_debug(ampcor_ctx *ctx, const void *ref_img, int64_t ref_h
                              const int32_t *ref_d, const int32_t *ref_a, const int32_t *sec_d,
                              const int32_t *sec_a, int32_t n, float *out_offset, float *out_snr, float *out_cov,
                              int32_t *out_maxloc, int32_t *out_start, int32_t *out_shift, int32_t *out_zoom_idx,
-                             float *out_maxval, int32_t chunk, int32_t mode, void *stream) {
+                             float *out_maxval, float *out_margin_coarse, float *out_margin_fine, int32_t chunk,
+                             int32_t mode, void *stream) {
     if (!ctx) return AMPCOR_E_ARG;
     return guarded(ctx, [&] {
         ampcor::Engine::RunArgs a{ref_img, ref_h, ref_w, ref_pitch, sec_img, sec_h, sec_w, sec_pitch,
                                   ref_d, ref_a, sec_d, sec_a, n, out_offset, out_snr, out_cov,
                                   out_maxloc, out_start, out_shift, out_zoom_idx, out_maxval,
-                                  chunk, mode, static_cast<hipStream_t>(stream)};
+                                  chunk, mode, static_cast<hipStream_t>(stream),
+                                  out_margin_coarse, out_margin_fine};
         ctx->eng.run(a);
     });
+}
+
+int ampcor_set_precision(ampcor_ctx *ctx, int32_t mode, float tau_coarse, float tau_fine) {
+    if (!ctx || mode < 0 || mode > 2) return AMPCOR_E_ARG;
+    ctx->eng.precision = mode;
+    if (tau_coarse >= 0.f) ctx->eng.tau_coarse = tau_coarse;
+    if (tau_fine >= 0.f) ctx->eng.tau_fine = tau_fine;
+    return AMPCOR_OK;
+}
+
+int ampcor_precision_stats(ampcor_ctx *ctx, int64_t *checked, int64_t *rerun) {
+    if (!ctx) return AMPCOR_E_ARG;
+    if (checked) *checked = ctx->eng.prec_checked;
+    if (rerun) *rerun = ctx->eng.prec_rerun;
+    ctx->eng.prec_checked = ctx->eng.prec_rerun = 0;
+    return AMPCOR_OK;
 }
 
 int ampcor_run_windows(ampcor_ctx *ctx, const void *ref_img, int64_t ref_h, int64_t ref_w, int64_t ref_pitch,
@@ -442,7 +527,7 @@ int ampcor_run_windows(ampcor_ctx *ctx, const void *ref_img, int64_t ref_h, int6
                        void *stream) {
     return ampcor_run_windows_debug(ctx, ref_img, ref_h, ref_w, ref_pitch, sec_img, sec_h, sec_w, sec_pitch, ref_d,
                                     ref_a, sec_d, sec_a, n, out_offset, out_snr, out_cov, nullptr, nullptr,
-                                    nullptr, nullptr, nullptr, chunk, mode, stream);
+                                    nullptr, nullptr, nullptr, nullptr, nullptr, chunk, mode, stream);
 }
 
 int ampcor_profile(ampcor_ctx *ctx, int32_t enable) {

@@ -1,0 +1,142 @@
+// In-place two-factor FFTs (n = R1*R2) over many signals held in LDS.
+//
+// fft_nat2dr: natural-order input, output X[k1 + R1*k2] at q = k2 + R2*k1
+//   pass 1: DFT_R1 over n1 (stride R2), twiddle w_n^{n2*k1}
+//   pass 2: DFT_R2 over n2 (contiguous blocks of R2)
+// fft_dr2nat: input Z[k1 + R1*k2] at q = k2 + R2*k1, natural-order output
+//   pass A: DFT_R2 over k2 (contiguous), twiddle w_n^{m2*k1}
+//   pass B: DFT_R1 over k1 (stride R2)
+// Signal s, storage index q lives at buf[s*ss + q*es]. Consecutive threads
+// take consecutive signals, so rows (es=1) with an odd pitch and columns
+// (ss=1) are both free of LDS bank-conflict hot spots. Each pass ends with a
+// workgroup barrier. DIR is the exponent sign; no scaling.
+#pragma once
+#include "cx.h"
+#include "fft_lds.h"
+
+namespace ampcor {
+
+template <class T, int DIR>
+__device__ __forceinline__ void dft5(cx<T> *x) {
+    const T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);
+    const T s1 = T(DIR) * T(0.95105651629515357212), s2 = T(DIR) * T(0.58778525229247312917);
+    const cx<T> a = x[1] + x[4], b = x[1] - x[4], c = x[2] + x[3], d = x[2] - x[3];
+    const cx<T> m1 = {x[0].x + c1 * a.x + c2 * c.x, x[0].y + c1 * a.y + c2 * c.y};
+    const cx<T> m2 = {x[0].x + c2 * a.x + c1 * c.x, x[0].y + c2 * a.y + c1 * c.y};
+    // i * (s1 b + s2 d) and i * (s2 b - s1 d)
+    const cx<T> r1 = {-(s1 * b.y + s2 * d.y), s1 * b.x + s2 * d.x};
+    const cx<T> r2 = {-(s2 * b.y - s1 * d.y), s2 * b.x - s1 * d.x};
+    x[0] = x[0] + a + c;
+    x[1] = m1 + r1;
+    x[4] = m1 - r1;
+    x[2] = m2 + r2;
+    x[3] = m2 - r2;
+}
+
+template <class T, int DIR>
+__device__ __forceinline__ void dft9(cx<T> *x) {
+    // 3 x 3: n = n2 + 3 n1, k = k1 + 3 k2
+    const T c1 = T(0.76604444311897803520), s1 = T(0.64278760968653932632);
+    const T c2 = T(0.17364817766693034885), s2 = T(0.98480775301220805936);
+    const T c4 = T(-0.93969262078590838405), s4 = T(0.34202014332566873304);
+    cx<T> y[9];
+#pragma unroll
+    for (int n2 = 0; n2 < 3; ++n2) {
+        cx<T> v[3] = {x[n2], x[n2 + 3], x[n2 + 6]};
+        dft3<T, DIR>(v);
+        y[n2 * 3 + 0] = v[0]; y[n2 * 3 + 1] = v[1]; y[n2 * 3 + 2] = v[2];   // y[n2][k1]
+    }
+    // twiddles w9^{n2 k1}: (n2,k1) = (1,1):1 (1,2):2 (2,1):2 (2,2):4
+    y[4] = cmul(y[4], cx<T>{c1, T(DIR) * s1});
+    y[5] = cmul(y[5], cx<T>{c2, T(DIR) * s2});
+    y[7] = cmul(y[7], cx<T>{c2, T(DIR) * s2});
+    y[8] = cmul(y[8], cx<T>{c4, T(DIR) * s4});
+#pragma unroll
+    for (int k1 = 0; k1 < 3; ++k1) {
+        cx<T> v[3] = {y[k1], y[3 + k1], y[6 + k1]};
+        dft3<T, DIR>(v);
+        x[k1] = v[0]; x[k1 + 3] = v[1]; x[k1 + 6] = v[2];
+    }
+}
+
+template <class T, int DIR, int R>
+__device__ __forceinline__ void dft_r(cx<T> *x) {
+    if constexpr (R == 2) dft2<T, DIR>(x[0], x[1]);
+    else if constexpr (R == 3) dft3<T, DIR>(x);
+    else if constexpr (R == 4) dft4<T, DIR>(x[0], x[1], x[2], x[3]);
+    else if constexpr (R == 5) dft5<T, DIR>(x);
+    else if constexpr (R == 8) dft8<T, DIR>(x);
+    else if constexpr (R == 9) dft9<T, DIR>(x);
+    else if constexpr (R == 16) dft16<T, DIR>(x);
+    else static_assert(R == 2, "unsupported radix");
+}
+
+// digit-reversed storage index of bin k
+template <int R1, int R2>
+__host__ __device__ __forceinline__ int dr_pos(int k) { return k / R1 + R2 * (k % R1); }
+
+template <class T, int DIR, int R1, int R2, int NTH>
+__device__ void fft_nat2dr(cx<T> *buf, int nsig, int ss, int es, const cx<T> *tw) {
+    constexpr int n = R1 * R2;
+    #pragma unroll 1
+    for (int t = threadIdx.x; t < nsig * R2; t += NTH) {
+        const int s = t % nsig, n2 = t / nsig;
+        cx<T> *b = buf + s * ss + n2 * es;
+        cx<T> x[R1];
+#pragma unroll
+        for (int n1 = 0; n1 < R1; ++n1) x[n1] = b[(R2 * n1) * es];
+        dft_r<T, DIR, R1>(x);
+#pragma unroll
+        for (int k1 = 1; k1 < R1; ++k1)
+            if (n2) x[k1] = cmul(x[k1], twid<T, DIR>(tw, n2 * k1));
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) b[(R2 * k1) * es] = x[k1];
+    }
+    __syncthreads();
+    #pragma unroll 1
+    for (int t = threadIdx.x; t < nsig * R1; t += NTH) {
+        const int s = t % nsig, k1 = t / nsig;
+        cx<T> *b = buf + s * ss + (R2 * k1) * es;
+        cx<T> x[R2];
+#pragma unroll
+        for (int n2 = 0; n2 < R2; ++n2) x[n2] = b[n2 * es];
+        dft_r<T, DIR, R2>(x);
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) b[k2 * es] = x[k2];
+    }
+    __syncthreads();
+    (void)n;
+}
+
+template <class T, int DIR, int R1, int R2, int NTH>
+__device__ void fft_dr2nat(cx<T> *buf, int nsig, int ss, int es, const cx<T> *tw) {
+    #pragma unroll 1
+    for (int t = threadIdx.x; t < nsig * R1; t += NTH) {
+        const int s = t % nsig, k1 = t / nsig;
+        cx<T> *b = buf + s * ss + (R2 * k1) * es;
+        cx<T> x[R2];
+#pragma unroll
+        for (int k2 = 0; k2 < R2; ++k2) x[k2] = b[k2 * es];
+        dft_r<T, DIR, R2>(x);
+#pragma unroll
+        for (int m2 = 1; m2 < R2; ++m2)
+            if (k1) x[m2] = cmul(x[m2], twid<T, DIR>(tw, m2 * k1));
+#pragma unroll
+        for (int m2 = 0; m2 < R2; ++m2) b[m2 * es] = x[m2];
+    }
+    __syncthreads();
+    #pragma unroll 1
+    for (int t = threadIdx.x; t < nsig * R2; t += NTH) {
+        const int s = t % nsig, m2 = t / nsig;
+        cx<T> *b = buf + s * ss + m2 * es;
+        cx<T> x[R1];
+#pragma unroll
+        for (int k1 = 0; k1 < R1; ++k1) x[k1] = b[(R2 * k1) * es];
+        dft_r<T, DIR, R1>(x);
+#pragma unroll
+        for (int m1 = 0; m1 < R1; ++m1) b[(R2 * m1) * es] = x[m1];
+    }
+    __syncthreads();
+}
+
+}  // namespace ampcor

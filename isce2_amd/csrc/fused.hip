@@ -1,7 +1,7 @@
 // Fused per-window pipeline: the fast path for square windows of 32 or 64
 // pixels with 2x raw oversampling, a 16x16 zoom surface and 16x FFT surface
 // oversampling (the reference's defaults). Three kernels per chunk of windows:
-// k_coarse -> k_oversample (reference chip, search zoom chip) -> k_fine.
+// k_coarse -> k_oversample (reference chip; search zoom chip) -> k_fine.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -12,19 +12,25 @@
 namespace ampcor {
 namespace fused {
 
-// byte offsets of LDS regions, shared by host (sizing) and device (carving)
+__host__ __device__ inline int al16(int x) { return (x + 15) & ~15; }
+
+// ---------------------------------------------------------------------------
+// coarse stage (cuAmpcorChunk.cpp:18-106), one workgroup per window
+// ---------------------------------------------------------------------------
 struct CoarseLds {
-    int A, work, tb, tst, nrm, corr, red, total;
-    __host__ __device__ CoarseLds(int W, int L, int SH, int NL2) {
-        auto al = [](int x) { return (x + 15) & ~15; };
-        A = 0;
-        work = al(A + SH * L * 4);
-        tb = al(work + 8 * L * 8);
-        tst = al(tb + 4 * L * 8);
-        nrm = al(tst + 8 * W * 4);
-        corr = al(nrm + NL2 * 4);
-        red = al(corr + NL2 * 4);
-        total = red + 16 * 8;
+    int spec, nrm, corr, red, total, y_off;   // bytes; y_off in cf units from spec
+    __host__ __device__ CoarseLds(int W, int L, int SH, int NLh, int NLw) {
+        const int P = L + 1, H = L / 2;
+        const int nS = (SH + 1) / 2, nT = W / 2;
+        y_off = (NLh * H + 1) & ~1;
+        const int npair = (NLh + 1) / 2;
+        int cf_units = (nS + nT) * P;
+        if (y_off + npair * P > cf_units) cf_units = y_off + npair * P;
+        spec = 0;
+        nrm = al16(spec + cf_units * 8);
+        corr = al16(nrm + NLh * NLw * 4);
+        red = al16(corr + NLh * NLw * 4);
+        total = red + 32 * 8;
     }
 };
 
@@ -35,80 +41,95 @@ struct CoarseArgs {
     const int32_t *ref_d, *ref_a, *sec_d, *sec_a;
     int hs_d, hs_a, hz, patch_h, patch_w;
     int32_t *maxloc; float *maxval; int32_t *start, *shift; float *snr, *cov;
+    float *margin;                 // [n] top-1 minus top-2 of the normalized surface, or null
 };
 
-// cuAmpcorChunk.cpp:18-106 for one window per workgroup
-template <int W, int L, int K>
-__global__ void __launch_bounds__(NT) k_coarse(CoarseArgs a, FftDesc d, int w0) {
+template <int W, int R1, int R2, int K>
+__global__ void __launch_bounds__(NT) k_coarse(CoarseArgs a, const cf *tw, int w0) {
     extern __shared__ __align__(16) unsigned char smem[];
-    constexpr int TPT = (W * W + NT - 1) / NT;
+    constexpr int L = R1 * R2, H = L / 2, P = L + 1, nT = W / 2;
     const int SH = W + 2 * a.hs_d, SW = W + 2 * a.hs_a;
     const int NLh = SH - W + 1, NLw = SW - W + 1;
-    const CoarseLds lay(W, L, SH, NLh * NLw);
-    float *A = reinterpret_cast<float *>(smem + lay.A);
-    cf *work = reinterpret_cast<cf *>(smem + lay.work);
-    cf *tb = reinterpret_cast<cf *>(smem + lay.tb);
-    float *tst = reinterpret_cast<float *>(smem + lay.tst);
+    const int nS = (SH + 1) / 2;
+    const CoarseLds lay(W, L, SH, NLh, NLw);
+    cf *S = reinterpret_cast<cf *>(smem + lay.spec);
+    cf *Tb = S + nS * P;
     float *nrm = reinterpret_cast<float *>(smem + lay.nrm);
     float *corr = reinterpret_cast<float *>(smem + lay.corr);
     double *red = reinterpret_cast<double *>(smem + lay.red);
     const int win = w0 + blockIdx.x;
 
-    // template amplitudes in registers; FP64 mean and sum of squares (cuCorrNormalization.cu:210-235)
+    // template amplitudes, row pairs packed as (row 2j, row 2j+1); FP64 mean and sum of squares
     const int64_t rd = a.ref_d[win], ra = a.ref_a[win];
-    float t[TPT];
-    double s = 0.0;
-#pragma unroll
-    for (int k = 0; k < TPT; ++k) {
-        const int e = threadIdx.x + NT * k;
-        t[k] = 0.f;
-        if (e < W * W) {
-            const int r = e / W, c = e - r * W;
-            t[k] = pixel_amp(a.ref, a.mode, (rd + r) * a.ref_pitch + ra + c);
-            s += t[k];
+    double sum = 0.0;
+    for (int i = threadIdx.x; i < nT * L; i += NT) {
+        const int j = i / L, x = i - j * L;
+        cf v = {0.f, 0.f};
+        if (x < W) {
+            v.x = pixel_amp(a.ref, a.mode, (rd + 2 * j) * a.ref_pitch + ra + x);
+            v.y = pixel_amp(a.ref, a.mode, (rd + 2 * j + 1) * a.ref_pitch + ra + x);
+            sum += double(v.x) + double(v.y);
         }
+        Tb[j * P + x] = v;
     }
-    const double mean = block_sum(s, red) / double(W * W);
+    const double mean = block_sum(sum, red) / double(W * W);
     double t2 = 0.0;
-#pragma unroll
-    for (int k = 0; k < TPT; ++k) {
-        const int e = threadIdx.x + NT * k;
-        if (e < W * W) {
-            const double dv = double(t[k]) - mean;
-            t2 += dv * dv;
-            t[k] = float(dv);
+    for (int i = threadIdx.x; i < nT * L; i += NT) {
+        const int j = i / L, x = i - j * L;
+        if (x < W) {
+            const cf v = Tb[j * P + x];
+            const double d0 = double(v.x) - mean, d1 = double(v.y) - mean;
+            t2 += d0 * d0 + d1 * d1;
+            Tb[j * P + x] = {float(d0), float(d1)};
         }
     }
     t2 = block_sum(t2, red);
 
-    // search amplitudes -> A[SH][SW]
+    // search amplitudes packed the same way
     const int64_t sd = a.sec_d[win], sa = a.sec_a[win];
-    for (int i = threadIdx.x; i < SH * SW; i += NT) {
-        const int r = i / SW, c = i - r * SW;
-        A[i] = pixel_amp(a.sec, a.mode, (sd + r) * a.sec_pitch + sa + c);
+    for (int i = threadIdx.x; i < nS * L; i += NT) {
+        const int j = i / L, x = i - j * L;
+        cf v = {0.f, 0.f};
+        if (x < SW) {
+            v.x = pixel_amp(a.sec, a.mode, (sd + 2 * j) * a.sec_pitch + sa + x);
+            if (2 * j + 1 < SH) v.y = pixel_amp(a.sec, a.mode, (sd + 2 * j + 1) * a.sec_pitch + sa + x);
+        }
+        S[j * P + x] = v;
     }
     __syncthreads();
-    window_norms<(L + 63) / 64>(A, SW, SW, W, W, NLh, NLw, t2, nrm);
-    __syncthreads();
-    search_spectra_inplace<L>(A, SH, SW, work, d);
-
-    correlate_rows<L, K>(
-        [&](int i0, int nr) {
-#pragma unroll
-            for (int k = 0; k < TPT; ++k) {
-                const int e = threadIdx.x + NT * k - i0 * W;
-                if (e >= 0 && e < nr * W) tst[e] = t[k];
-            }
+    window_norms<(L + 63) / 64>(
+        [&](int r, int c) -> double {
+            const cf v = S[(r >> 1) * P + c];
+            return (r & 1) ? v.y : v.x;
         },
-        [&](int i, int x) -> float { return x < W ? tst[(i & 7) * W + x] : 0.f; },
-        W, SH, NLh, NLw, reinterpret_cast<cf *>(A), work, tb, d, corr);
-
-    for (int i = threadIdx.x; i < NLh * NLw; i += NT) corr[i] *= nrm[i];
+        SW, W, W, NLh, NLw, t2, nrm);
     __syncthreads();
-    int idx;
-    float val;
-    block_argmax(corr, NLh * NLw, idx, val, reinterpret_cast<float *>(red + 8), reinterpret_cast<uint64_t *>(red + 12));
+
+    // row spectra of search and template rows (one in-place pass over both)
+    fft_nat2dr<float, -1, R1, R2, NT>(S, nS + nT, P, 1, tw);
+    split_pairs_inplace<R1, R2>(S, nS + nT, P);
+
+    const int f = threadIdx.x % H, blk = threadIdx.x / H, u0 = blk * K;
+    const bool active = (blk < NT / H) && (u0 < NLh);
+    cf q[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) q[k] = {0.f, 0.f};
+    if (active) mac_rows<H, K>(q, S, P, Tb, P, 0, 0, W, u0, SH, f);
+    __syncthreads();
+    cf *Qs = S;
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (u0 + k < NLh) Qs[(u0 + k) * H + f] = q[k];
+    }
+    __syncthreads();
+    inverse_lags<R1, R2>(Qs, NLh, NLw, S + lay.y_off, P, tw, nrm, corr);
+
+    const Top2 top = block_argmax(corr, NLh * NLw, red);
+    const int idx = top.index();
+    const float val = top.v1;
     if (threadIdx.x == 0) {
+        if (a.margin) a.margin[win] = top.margin();
         const int px = idx / NLw, py = idx - px * NLw;
         window_stats(corr, NLh, NLw, px, py, double(val), W * W, a.patch_h, a.patch_w, a.cov + 3 * win,
                      a.snr + win);
@@ -123,13 +144,11 @@ __global__ void __launch_bounds__(NT) k_coarse(CoarseArgs a, FftDesc d, int w0) 
 }
 
 // ---------------------------------------------------------------------------
-// x2 chip oversampling: deramp, then columns and rows through the half-sample
-// shift  odd[m] = (1/N) FFT_-(FFT_+(x) * e^{-i pi k_s/N})[m], even[m] = x[m]
-// (the quadrant-padded FFT interpolator of cuOverSampler.cpp:72-78 and
-// cuArraysPadding.cu:223-266, separable per axis), then |.| -> HBM.
+// x2 chip oversampling (cuOverSampler.cpp:72-78, cuArraysPadding.cu:223-266,
+// cuDeramp.cu:343-390): per axis, odd samples = (1/N) FFT_-(FFT_+(x) e^{-i pi k_s/N}),
+// even samples = x. Rows of the chip X go to Y (odd columns); then every
+// column of [X | Y] is interpolated the same way. |.| goes to HBM.
 // ---------------------------------------------------------------------------
-constexpr int OG = 8;   // columns / rows per FFT batch
-
 struct OsArgs {
     const void *img; int64_t pitch; int mode;
     const int32_t *sd, *sa, *start;   // start may be null
@@ -139,34 +158,38 @@ struct OsArgs {
     double *mean;                     // [n] or null
 };
 
-template <int N>
-__global__ void __launch_bounds__(NT) k_oversample(OsArgs a, FftDesc d, int w0) {
+__host__ __device__ inline int os_lds_bytes(int N) { return al16(N * (2 * N + 1) * 8 + N * 8) + 32 * 8; }
+
+template <int R1, int R2>
+__global__ void __launch_bounds__(NT) k_oversample(OsArgs a, const cf *tw, int w0) {
     extern __shared__ __align__(16) unsigned char smem[];
-    cf *X = reinterpret_cast<cf *>(smem);
-    cf *Y = X + N * N;
-    cf *work = Y + N * N;
-    double *red = reinterpret_cast<double *>(work + 2 * OG * N);
+    constexpr int N = R1 * R2, PC = 2 * N + 1;
+    cf *C = reinterpret_cast<cf *>(smem);
+    cf *hsh = C + N * PC;
+    double *red = reinterpret_cast<double *>(smem + al16(N * PC * 8 + N * 8));
     const int win = w0 + blockIdx.x;
     int64_t r0 = a.sd[win], c0 = a.sa[win];
     if (a.start) { r0 += a.start[2 * win]; c0 += a.start[2 * win + 1]; }
     for (int i = threadIdx.x; i < N * N; i += NT) {
         const int r = i / N, c = i - r * N;
-        X[i] = pixel_cx(a.img, a.mode, (r0 + r) * a.pitch + c0 + c);
+        C[r * PC + c] = pixel_cx(a.img, a.mode, (r0 + r) * a.pitch + c0 + c);
     }
+    for (int i = threadIdx.x; i < N; i += NT) hsh[i] = a.hsh[i];
     __syncthreads();
     if (a.deramp == 1) {
-        // cuDeramp.cu:343-390 with float2's operator*= (imaginary part uses the new real part)
+        // float2's operator*= (float2.h:71-75): the imaginary part uses the new real part
         double ayr = 0, ayi = 0, axr = 0, axi = 0;
         for (int i = threadIdx.x; i < N * N; i += NT) {
             const int r = i / N, c = i - r * N;
-            const double zr = X[i].x, zi = X[i].y;
+            const cf z = C[r * PC + c];
+            const double zr = z.x, zi = z.y;
             if (c < N - 1) {
-                const double br = X[i + 1].x, bi = X[i + 1].y;
-                ayr += zr * br + zi * bi; ayi += zi * br - zr * bi;
+                const cf b = C[r * PC + c + 1];
+                ayr += zr * b.x + zi * b.y; ayi += zi * b.x - zr * b.y;
             }
             if (r < N - 1) {
-                const double br = X[i + N].x, bi = X[i + N].y;
-                axr += zr * br + zi * bi; axi += zi * br - zr * bi;
+                const cf b = C[(r + 1) * PC + c];
+                axr += zr * b.x + zi * b.y; axi += zi * b.x - zr * b.y;
             }
         }
         ayr = block_sum(ayr, red); ayi = block_sum(ayi, red);
@@ -176,52 +199,52 @@ __global__ void __launch_bounds__(NT) k_oversample(OsArgs a, FftDesc d, int w0) 
             const int r = i / N, c = i - r * N;
             double sn, cs;
             sincos(double(c) * phx + double(r) * phy, &sn, &cs);
-            const double x = X[i].x, y = X[i].y;
-            const double nx = x * cs - y * sn;
-            X[i] = {float(nx), float(y * cs + nx * sn)};
+            const cf z = C[r * PC + c];
+            const double nx = double(z.x) * cs - double(z.y) * sn;
+            C[r * PC + c] = {float(nx), float(double(z.y) * cs + nx * sn)};
         }
         __syncthreads();
     }
-    // column pass: odd rows -> Y
-    for (int cb = 0; cb < N; cb += OG) {
-        for (int i = threadIdx.x; i < N * OG; i += NT) {
-            const int e = i / OG, j = i - e * OG;
-            work[i] = X[e * N + cb + j];
-        }
-        __syncthreads();
-        cf *Z = fft_lds<float, +1>(work, work + OG * N, d, OG, OG, 1);
-        for (int i = threadIdx.x; i < N * OG; i += NT) Z[i] = cmul(Z[i], a.hsh[i / OG]);
-        __syncthreads();
-        cf *Z2 = fft_lds<float, -1>(Z, Z == work ? work + OG * N : work, d, OG, OG, 1);
-        for (int i = threadIdx.x; i < N * OG; i += NT) {
-            const int e = i / OG, j = i - e * OG;
-            Y[e * N + cb + j] = Z2[i];
-        }
-        __syncthreads();
+    // Y = X, then interpolate along rows
+    for (int i = threadIdx.x; i < N * N; i += NT) {
+        const int r = i / N, c = i - r * N;
+        C[r * PC + N + c] = C[r * PC + c];
     }
-    // row pass over the 2N rows (even: X, odd: Y), amplitudes to HBM
-    double sum = 0.0;
+    __syncthreads();
+    fft_nat2dr<float, +1, R1, R2, NT>(C + N, N, PC, 1, tw);
+    for (int i = threadIdx.x; i < N * N; i += NT) {
+        const int r = i / N, k = i - r * N;
+        cf &z = C[r * PC + N + dr_pos<R1, R2>(k)];
+        z = cmul(z, hsh[k]);
+    }
+    __syncthreads();
+    fft_dr2nat<float, -1, R1, R2, NT>(C + N, N, PC, 1, tw);
+    // even output rows 2r: interleave X (even columns) and Y (odd columns)
     float *out = a.out + size_t(win) * 4 * N * N;
-    for (int R0 = 0; R0 < 2 * N; R0 += OG) {
-        for (int i = threadIdx.x; i < OG * N; i += NT) {
-            const int j = i / N, x = i - j * N;
-            const int R = R0 + j;
-            work[i] = ((R & 1) ? Y : X)[(R >> 1) * N + x];
-        }
-        __syncthreads();
-        cf *Z = fft_lds<float, +1>(work, work + OG * N, d, OG, 1, N);
-        for (int i = threadIdx.x; i < OG * N; i += NT) Z[i] = cmul(Z[i], a.hsh[i % N]);
-        __syncthreads();
-        cf *Z2 = fft_lds<float, -1>(Z, Z == work ? work + OG * N : work, d, OG, 1, N);
-        for (int i = threadIdx.x; i < OG * 2 * N; i += NT) {
-            const int j = i / (2 * N), x = i - j * 2 * N;
-            const int R = R0 + j;
-            const cf v = (x & 1) ? Z2[j * N + (x >> 1)] : ((R & 1) ? Y : X)[(R >> 1) * N + (x >> 1)];
-            const float amp = sqrtf(v.x * v.x + v.y * v.y);
-            out[size_t(R) * 2 * N + x] = amp;
-            sum += amp;
-        }
-        __syncthreads();
+    double sum = 0.0;
+    for (int i = threadIdx.x; i < N * 2 * N; i += NT) {
+        const int r = i / (2 * N), x = i - r * 2 * N;
+        const cf v = C[r * PC + ((x & 1) ? N : 0) + (x >> 1)];
+        const float amp = sqrtf(v.x * v.x + v.y * v.y);
+        out[size_t(2 * r) * 2 * N + x] = amp;
+        sum += amp;
+    }
+    __syncthreads();
+    // interpolate all 2N columns of [X | Y] -> odd output rows
+    fft_nat2dr<float, +1, R1, R2, NT>(C, 2 * N, 1, PC, tw);
+    for (int i = threadIdx.x; i < N * 2 * N; i += NT) {
+        const int k = i / (2 * N), c = i - k * 2 * N;
+        cf &z = C[dr_pos<R1, R2>(k) * PC + c];
+        z = cmul(z, hsh[k]);
+    }
+    __syncthreads();
+    fft_dr2nat<float, -1, R1, R2, NT>(C, 2 * N, 1, PC, tw);
+    for (int i = threadIdx.x; i < N * 2 * N; i += NT) {
+        const int r = i / (2 * N), x = i - r * 2 * N;
+        const cf v = C[r * PC + ((x & 1) ? N : 0) + (x >> 1)];
+        const float amp = sqrtf(v.x * v.x + v.y * v.y);
+        out[size_t(2 * r + 1) * 2 * N + x] = amp;
+        sum += amp;
     }
     if (a.mean) {
         sum = block_sum(sum, red);
@@ -230,23 +253,30 @@ __global__ void __launch_bounds__(NT) k_oversample(OsArgs a, FftDesc d, int w0) 
 }
 
 // ---------------------------------------------------------------------------
-// fine stage: oversampled correlation, normalization, 16x16 trim, 16x surface
-// interpolation and argmax, sub-pixel offset (cuAmpcorChunk.cpp:148-207)
+// fine stage (cuAmpcorChunk.cpp:148-207): oversampled correlation, normalization,
+// trim to 16x16, 16x surface interpolation (16-sparse spectrum evaluated with
+// 16-point DFTs), argmax with the reference tie rule, sub-pixel offset
 // ---------------------------------------------------------------------------
+constexpr int TCH = 64;   // template rows per spectrum chunk
+
 struct FineLds {
-    int A, work, tb, nrm, corr, tw, red, total;
-    __host__ __device__ FineLds(int P, int L, int NL2) {
-        auto al = [](int x) { return (x + 15) & ~15; };
-        A = 0;
-        int a_bytes = P * L * 4;
-        if (a_bytes < 16 * 256 * 8) a_bytes = 16 * 256 * 8;   // surface G[16][256] reuses A
-        work = al(A + a_bytes);
-        tb = al(work + 8 * L * 8);
-        nrm = al(tb + 4 * L * 8);
-        corr = al(nrm + NL2 * 4);
-        tw = al(corr + NL2 * 4);
-        red = al(tw + 256 * 8);
-        total = red + 16 * 8;
+    int spec, tchunk, nrm, corr, red, total, y_off;
+    __host__ __device__ FineLds(int P, int NL) {
+        const int L = P, PP = L + 1, H = L / 2;
+        const int nS = P / 2;
+        y_off = (NL * H + 1) & ~1;
+        int s_units = nS * PP;
+        const int npair = (NL + 1) / 2;
+        if (y_off + npair * PP > s_units) s_units = y_off + npair * PP;
+        if (s_units < 16 * 256) s_units = 16 * 256;            // surface G[16][256]
+        int t_units = (TCH / 2) * PP;
+        if (t_units < 2 * 256) t_units = 2 * 256;             // 16x16 surface + twiddles
+        spec = 0;
+        tchunk = al16(s_units * 8);
+        nrm = al16(tchunk + t_units * 8);
+        corr = al16(nrm + NL * NL * 4);
+        red = al16(corr + NL * NL * 4);
+        total = red + 32 * 8;
     }
 };
 
@@ -258,131 +288,170 @@ struct FineArgs {
     float ratio;                               // 1/(surf_ovs*raw_ovs)
     const cf *tw256;                           // e^{-2 pi i k/256}
     int32_t *zoom_idx; float *offset;
+    float *margin;                             // [n] top-1 minus top-2 of the surface, or null
 };
 
-template <int T, int P, int L, int K>
-__global__ void __launch_bounds__(NT) k_fine(FineArgs a, FftDesc d, FftDesc d16, int w0) {
+template <int T, int R1, int R2, int K>
+__global__ void __launch_bounds__(NT) k_fine(FineArgs a, const cf *tw, int w0) {
     extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int P = R1 * R2, L = P, PP = L + 1, H = L / 2, nS = P / 2;
     constexpr int NL = P - T + 1;      // 17
     constexpr int Z = NL - 1;          // 16 (trimmed surface)
     constexpr int SO = 256;            // oversampled surface side
-    const FineLds lay(P, L, NL * NL);
-    float *A = reinterpret_cast<float *>(smem + lay.A);
-    cf *work = reinterpret_cast<cf *>(smem + lay.work);
-    cf *tb = reinterpret_cast<cf *>(smem + lay.tb);
+    static_assert(Z == 16, "fused surface stage assumes a 16x16 zoom surface");
+    const FineLds lay(P, NL);
+    cf *S = reinterpret_cast<cf *>(smem + lay.spec);
+    cf *Tb = reinterpret_cast<cf *>(smem + lay.tchunk);
     float *nrm = reinterpret_cast<float *>(smem + lay.nrm);
     float *corr = reinterpret_cast<float *>(smem + lay.corr);
-    cf *tw = reinterpret_cast<cf *>(smem + lay.tw);
     double *red = reinterpret_cast<double *>(smem + lay.red);
     const int win = w0 + blockIdx.x;
     const float *tp = a.tamp + size_t(win) * T * T;
-    const float *sp_g = a.samp + size_t(win) * P * P;
+    const float *sg = a.samp + size_t(win) * P * P;
     const double mean = a.tmean[win];
-    const float meanf = float(mean);
 
-    for (int i = threadIdx.x; i < SO; i += NT) tw[i] = a.tw256[i];
     double t2 = 0.0;
     for (int i = threadIdx.x; i < T * T; i += NT) {
-        const double dv = double(tp[i]) - mean;
-        t2 += dv * dv;
+        const double d = double(tp[i]) - mean;
+        t2 += d * d;
     }
     t2 = block_sum(t2, red);
-    for (int i = threadIdx.x; i < P * P; i += NT) A[i] = sp_g[i];
-    __syncthreads();
-    window_norms<(P + 63) / 64>(A, P, P, T, T, NL, NL, t2, nrm);
-    __syncthreads();
-    search_spectra_inplace<L>(A, P, P, work, d);
-    correlate_rows<L, K>([](int, int) {},
-                         [&](int i, int x) -> float { return x < T ? tp[i * T + x] - meanf : 0.f; },
-                         T, P, NL, NL, reinterpret_cast<cf *>(A), work, tb, d, corr);
-    // normalize, trim to Z x Z, 2-D +i transform / (Z*Z) (cuOverSampler.cpp:141-145)
-    for (int i = threadIdx.x; i < Z * Z; i += NT) {
-        const int r = i / Z, c = i - r * Z;
-        work[i] = {corr[r * NL + c] * nrm[r * NL + c], 0.f};
+    for (int i = threadIdx.x; i < nS * L; i += NT) {
+        const int j = i / L, x = i - j * L;
+        S[j * PP + x] = {sg[(2 * j) * P + x], sg[(2 * j + 1) * P + x]};
     }
     __syncthreads();
-    cf *S1 = fft_lds<float, +1>(work, work + Z * Z, d16, Z, 1, Z);
-    cf *S2 = fft_lds<float, +1>(S1, S1 == work ? work + Z * Z : work, d16, Z, Z, 1);
-    // G[k1][16c+dd] = DFT16_c( S2[k1][k2] e^{-2 pi i dd ks2/256} ) / 256, one (k1, dd) per thread
-    cf *G = reinterpret_cast<cf *>(A);
-    {
+    window_norms<(P + 63) / 64>(
+        [&](int r, int c) -> double {
+            const cf v = S[(r >> 1) * PP + c];
+            return (r & 1) ? v.y : v.x;
+        },
+        P, T, T, NL, NL, t2, nrm);
+    __syncthreads();
+    fft_nat2dr<float, -1, R1, R2, NT>(S, nS, PP, 1, tw);
+    split_pairs_inplace<R1, R2>(S, nS, PP);
+
+    const int f = threadIdx.x % H, blk = threadIdx.x / H, u0 = blk * K;
+    const bool active = (blk < NT / H) && (u0 < NL);
+    cf q[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) q[k] = {0.f, 0.f};
+    for (int c0 = 0; c0 < T; c0 += TCH) {
+        const int nr = min(TCH, T - c0);
+        for (int i = threadIdx.x; i < (TCH / 2) * L; i += NT) {
+            const int j = i / L, x = i - j * L;
+            const int r = c0 + 2 * j;
+            cf v = {0.f, 0.f};
+            if (x < T && 2 * j < nr) {
+                v.x = float(double(tp[r * T + x]) - mean);
+                if (2 * j + 1 < nr) v.y = float(double(tp[(r + 1) * T + x]) - mean);
+            }
+            Tb[j * PP + x] = v;
+        }
+        __syncthreads();
+        fft_nat2dr<float, -1, R1, R2, NT>(Tb, TCH / 2, PP, 1, tw);
+        split_pairs_inplace<R1, R2>(Tb, TCH / 2, PP);
+        if (active) mac_rows<H, K>(q, S, PP, Tb, PP, c0, c0, c0 + nr, u0, P, f);
+        __syncthreads();
+    }
+    cf *Qs = S;
+    if (active) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+            if (u0 + k < NL) Qs[(u0 + k) * H + f] = q[k];
+    }
+    __syncthreads();
+    inverse_lags<R1, R2>(Qs, NL, NL, S + lay.y_off, PP, tw, nrm, corr);
+
+    // trimmed 16x16 surface -> 2-D +i DFT / 256 (cuOverSampler.cpp:141-145)
+    cf *a16 = Tb;            // [16][16]
+    cf *tws = Tb + 256;      // e^{-2 pi i k/256}
+    for (int i = threadIdx.x; i < Z * Z; i += NT) {
+        const int r = i / Z, c = i - r * Z;
+        a16[i] = {corr[r * NL + c] * (1.0f / 256.0f), 0.f};
+    }
+    for (int i = threadIdx.x; i < SO; i += NT) tws[i] = a.tw256[i];
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        cf x[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) x[c] = a16[threadIdx.x * 16 + c];
+        dft16<float, +1>(x);
+#pragma unroll
+        for (int c = 0; c < 16; ++c) a16[threadIdx.x * 16 + c] = x[c];
+    }
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        cf x[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) x[r] = a16[r * 16 + threadIdx.x];
+        dft16<float, +1>(x);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) a16[r * 16 + threadIdx.x] = x[r];
+    }
+    __syncthreads();
+    // G[k1][16c+dd] = DFT16_c( A[k1][k2] e^{-2 pi i dd ks2/256} ), ks2 = signed k2
+    cf *G = S;
+    if (threadIdx.x < 256) {
         const int k1 = threadIdx.x >> 4, dd = threadIdx.x & 15;
         cf x[16];
 #pragma unroll
         for (int k2 = 0; k2 < 16; ++k2) {
             const int ks = k2 < 8 ? k2 : k2 - 16;
-            const cf v = S2[k1 * Z + k2] * (1.0f / 256.0f);
-            x[k2] = cmul(v, tw[(dd * ks + 4096) & 255]);
+            x[k2] = cmul(a16[k1 * 16 + k2], tws[(dd * ks + 4096) & 255]);
         }
         dft16<float, -1>(x);
 #pragma unroll
         for (int c = 0; c < 16; ++c) G[k1 * SO + 16 * c + dd] = x[c];
     }
     __syncthreads();
-    // out[16a+b][q] = Re DFT16_a( G[k1][q] e^{-2 pi i b ks1/256} ); running argmax
-    float bv = -FLT_MAX;
-    uint64_t bk = ~uint64_t(0);
+    // out[16aa+b][q] = Re DFT16_aa( G[k1][q] e^{-2 pi i b ks1/256} ); running argmax
+    Top2 top;
     for (int task = threadIdx.x; task < SO * 16; task += NT) {
-        const int q = task & (SO - 1), b = task >> 8;
+        const int qq = task & (SO - 1), b = task >> 8;
         cf x[16];
 #pragma unroll
         for (int k1 = 0; k1 < 16; ++k1) {
             const int ks = k1 < 8 ? k1 : k1 - 16;
-            x[k1] = cmul(G[k1 * SO + q], tw[(b * ks + 4096) & 255]);
+            x[k1] = cmul(G[k1 * SO + qq], tws[(b * ks + 4096) & 255]);
         }
         dft16<float, -1>(x);
 #pragma unroll
-        for (int aa = 0; aa < 16; ++aa) {
-            const float v = x[aa].x;
-            const uint32_t idx = uint32_t((16 * aa + b) * SO + q);
-            const uint64_t key = maxloc_key(idx);
-            if (v > -FLT_MAX && better(v, key, bv, bk)) { bv = v; bk = key; }
-        }
+        for (int aa = 0; aa < 16; ++aa) top.add(x[aa].x, maxloc_key(uint32_t((16 * aa + b) * SO + qq)));
     }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const float ov = __shfl_xor(bv, o);
-        const uint64_t ok = __shfl_xor(bk, o);
-        if (better(ov, ok, bv, bk)) { bv = ov; bk = ok; }
-    }
-    float *shv = reinterpret_cast<float *>(red + 8);
-    uint64_t *shk = reinterpret_cast<uint64_t *>(red + 12);
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) { shv[threadIdx.x >> 6] = bv; shk[threadIdx.x >> 6] = bk; }
-    __syncthreads();
+    top.reduce(reinterpret_cast<float *>(red + 8), reinterpret_cast<float *>(red + 8) + 8,
+               reinterpret_cast<uint64_t *>(red + 16));
     if (threadIdx.x == 0) {
-        bv = shv[0]; bk = shk[0];
-        for (int i = 1; i < NT / 64; ++i)
-            if (better(shv[i], shk[i], bv, bk)) { bv = shv[i]; bk = shk[i]; }
-        const int idx = (bk == ~uint64_t(0)) ? 0 : int(bk & 0xffffffffu);
+        const int idx = top.index();
         const int px = idx / SO, py = idx % SO;
         a.zoom_idx[2 * win] = px;
         a.zoom_idx[2 * win + 1] = py;
         a.offset[2 * win] = a.ratio * float(px) + float(a.start[2 * win]) - float(a.hs_d);
         a.offset[2 * win + 1] = a.ratio * float(py) + float(a.start[2 * win + 1]) - float(a.hs_a);
+        if (a.margin) a.margin[win] = top.margin();
     }
 }
 
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-int coarse_L(int W, int SW) { return SW <= 64 ? 64 : (SW <= 128 ? 128 : 0); }
-const int kCoarseK[] = {3, 5, 9, 11, 17};
+int coarse_L(int SW) { return SW <= 64 ? 64 : (SW <= 128 ? 128 : 0); }
+const int kK[] = {2, 3, 5, 6, 9};
 
 int pick_k(int nl, int blocks) {
-    for (int k : kCoarseK)
+    for (int k : kK)
         if (k * blocks >= nl) return k;
     return 0;
 }
 
-template <int W, int L>
-void launch_coarse(int K, const CoarseArgs &a, const FftDesc &d, int n, int w0, size_t lds, hipStream_t st) {
+template <int W, int R1, int R2>
+void launch_coarse(int K, const CoarseArgs &a, const cf *tw, int n, int w0, size_t lds, hipStream_t st) {
     switch (K) {
-        case 3: hipLaunchKernelGGL((k_coarse<W, L, 3>), dim3(n), dim3(NT), lds, st, a, d, w0); break;
-        case 5: hipLaunchKernelGGL((k_coarse<W, L, 5>), dim3(n), dim3(NT), lds, st, a, d, w0); break;
-        case 9: hipLaunchKernelGGL((k_coarse<W, L, 9>), dim3(n), dim3(NT), lds, st, a, d, w0); break;
-        case 11: hipLaunchKernelGGL((k_coarse<W, L, 11>), dim3(n), dim3(NT), lds, st, a, d, w0); break;
-        default: hipLaunchKernelGGL((k_coarse<W, L, 17>), dim3(n), dim3(NT), lds, st, a, d, w0); break;
+        case 2: hipLaunchKernelGGL((k_coarse<W, R1, R2, 2>), dim3(n), dim3(NT), lds, st, a, tw, w0); break;
+        case 3: hipLaunchKernelGGL((k_coarse<W, R1, R2, 3>), dim3(n), dim3(NT), lds, st, a, tw, w0); break;
+        case 5: hipLaunchKernelGGL((k_coarse<W, R1, R2, 5>), dim3(n), dim3(NT), lds, st, a, tw, w0); break;
+        case 6: hipLaunchKernelGGL((k_coarse<W, R1, R2, 6>), dim3(n), dim3(NT), lds, st, a, tw, w0); break;
+        default: hipLaunchKernelGGL((k_coarse<W, R1, R2, 9>), dim3(n), dim3(NT), lds, st, a, tw, w0); break;
     }
 }
 
@@ -392,10 +461,10 @@ bool fused_supported(const ampcor_params &p, const ampcor_sizes &s) {
     using namespace fused;
     if (p.window_h != p.window_w || (p.window_h != 32 && p.window_h != 64)) return false;
     if (p.raw_ovs != 2 || s.zoom != 16 || p.surf_ovs != 16 || p.surf_method != 0) return false;
-    const int L = coarse_L(p.window_w, s.search_w);
-    if (L == 0 || s.search_w > L) return false;
-    if (pick_k(s.raw_corr_h, 256 / (L / 2)) == 0) return false;
-    if (CoarseLds(p.window_w, L, s.search_h, s.raw_corr_h * s.raw_corr_w).total > 160 * 1024) return false;
+    const int L = coarse_L(s.search_w);
+    if (L == 0) return false;
+    if (pick_k(s.raw_corr_h, NT / (L / 2)) == 0) return false;
+    if (CoarseLds(p.window_w, L, s.search_h, s.raw_corr_h, s.raw_corr_w).total > 160 * 1024) return false;
     return true;
 }
 
@@ -404,9 +473,8 @@ void Engine::run_fused(const RunArgs &a) {
     const hipStream_t st = a.stream;
     const int W = p.window_w, Zc = sz.zoom_in_w;          // 64 -> 72, 32 -> 40
     const int T2 = 2 * W, P = 2 * Zc;
-    const int L1 = coarse_L(W, sz.search_w);
+    const int L1 = coarse_L(sz.search_w);
     const int K1 = pick_k(sz.raw_corr_h, NT / (L1 / 2));
-    const int L2 = P;                                      // 144 or 80
     const int mode = p.image_complex ? (p.deramp == 0 ? 1 : 0) : 2;
     const int chunk = std::min(a.chunk > 0 ? a.chunk : 2048, 65535);
     const int nb = std::min(chunk, a.n);
@@ -419,6 +487,9 @@ void Engine::run_fused(const RunArgs &a) {
     float *tamp = buf<float>("f_tamp", size_t(nb) * T2 * T2);
     double *tmean = buf<double>("f_tmean", nb);
     float *samp = buf<float>("f_samp", size_t(nb) * P * P);
+    const bool want_margin = precision == 1 || a.margin_coarse || a.margin_fine;
+    float *mc = a.margin_coarse ? a.margin_coarse : (want_margin ? buf<float>("f_mc", a.n) : nullptr);
+    float *mf = a.margin_fine ? a.margin_fine : (want_margin ? buf<float>("f_mf", a.n) : nullptr);
 
     // half-sample shift tables h[k] = e^{-i pi k_s/N}/N for the chip sizes
     auto hshift = [&](int N) -> const cf * {
@@ -439,62 +510,54 @@ void Engine::run_fused(const RunArgs &a) {
         return b->as<cf>();
     };
     const cf *h_ref = hshift(W), *h_zoom = hshift(Zc);
-    const FftPlan &pl1 = plan(L1), &plW = plan(W), &plZ = plan(Zc), &pl2 = plan(L2), &pl16 = plan(16),
-                  &pl256 = plan(256);
+    auto tw = [&](int n) { return static_cast<const cf *>(plan(n).tw32.ptr); };
+    const cf *tw1 = tw(L1), *twW = tw(W), *twZ = tw(Zc), *tw2 = tw(P), *tw256 = tw(256);
 
     CoarseArgs ca{a.ref, a.ref_pitch, a.sec, a.sec_pitch, mode, a.ref_d, a.ref_a, a.sec_d, a.sec_a,
                   p.hs_down, p.hs_across, sz.half_zoom, sz.corr_raw_zoom_h, sz.corr_raw_zoom_w,
-                  loc, val, start, shift, a.snr, a.cov};
-    const size_t lds_c = CoarseLds(W, L1, sz.search_h, sz.raw_corr_h * sz.raw_corr_w).total;
-    const size_t lds_or = (2 * size_t(W) * W + 2 * OG * W) * sizeof(cf) + 16 * 8;
-    const size_t lds_oz = (2 * size_t(Zc) * Zc + 2 * OG * Zc) * sizeof(cf) + 16 * 8;
-    const size_t lds_f = FineLds(P, L2, (P - T2 + 1) * (P - T2 + 1)).total;
+                  loc, val, start, shift, a.snr, a.cov, mc};
+    const size_t lds_c = CoarseLds(W, L1, sz.search_h, sz.raw_corr_h, sz.raw_corr_w).total;
+    const size_t lds_or = os_lds_bytes(W), lds_oz = os_lds_bytes(Zc);
+    const size_t lds_f = FineLds(P, P - T2 + 1).total;
 
     for (int w0 = 0; w0 < a.n; w0 += nb) {
         const int n = std::min(nb, a.n - w0);
         timed("k_coarse", n, st, [&] {
-            if (W == 64) {
-                if (L1 == 128) launch_coarse<64, 128>(K1, ca, pl1.desc32, n, w0, lds_c, st);
-                else launch_coarse<64, 64>(K1, ca, pl1.desc32, n, w0, lds_c, st);
-            } else {
-                if (L1 == 128) launch_coarse<32, 128>(K1, ca, pl1.desc32, n, w0, lds_c, st);
-                else launch_coarse<32, 64>(K1, ca, pl1.desc32, n, w0, lds_c, st);
-            }
+            if (W == 64) launch_coarse<64, 16, 8>(K1, ca, tw1, n, w0, lds_c, st);
+            else if (L1 == 128) launch_coarse<32, 16, 8>(K1, ca, tw1, n, w0, lds_c, st);
+            else launch_coarse<32, 8, 8>(K1, ca, tw1, n, w0, lds_c, st);
         });
-        // chips are addressed with absolute window indices; outputs of the
-        // oversampler are per chunk, hence the pointer shifts by w0
+        // chips use absolute window indices; oversampler outputs are per chunk
         OsArgs oref{a.ref, a.ref_pitch, mode, a.ref_d, a.ref_a, nullptr, p.deramp, h_ref,
                     tamp - size_t(w0) * T2 * T2, tmean - w0};
         OsArgs osec{a.sec, a.sec_pitch, mode, a.sec_d, a.sec_a, start, p.deramp, h_zoom,
                     samp - size_t(w0) * P * P, nullptr};
         FineArgs fa{tamp - size_t(w0) * T2 * T2, tmean - w0, samp - size_t(w0) * P * P, start, p.hs_down,
-                    p.hs_across, 1.0f / float(p.surf_ovs * p.raw_ovs), static_cast<const cf *>(pl256.tw32.ptr),
-                    zidx, a.offset};
+                    p.hs_across, 1.0f / float(p.surf_ovs * p.raw_ovs), tw256, zidx, a.offset, mf};
         if (W == 64) {
             timed("k_oversample_ref", n, st, [&] {
-                hipLaunchKernelGGL((k_oversample<64>), dim3(n), dim3(NT), lds_or, st, oref, plW.desc32, w0);
+                hipLaunchKernelGGL((k_oversample<8, 8>), dim3(n), dim3(NT), lds_or, st, oref, twW, w0);
             });
             timed("k_oversample_search", n, st, [&] {
-                hipLaunchKernelGGL((k_oversample<72>), dim3(n), dim3(NT), lds_oz, st, osec, plZ.desc32, w0);
+                hipLaunchKernelGGL((k_oversample<8, 9>), dim3(n), dim3(NT), lds_oz, st, osec, twZ, w0);
             });
             timed("k_fine", n, st, [&] {
-                hipLaunchKernelGGL((k_fine<128, 144, 144, 6>), dim3(n), dim3(NT), lds_f, st, fa, pl2.desc32,
-                                   pl16.desc32, w0);
+                hipLaunchKernelGGL((k_fine<128, 16, 9, 3>), dim3(n), dim3(NT), lds_f, st, fa, tw2, w0);
             });
         } else {
             timed("k_oversample_ref", n, st, [&] {
-                hipLaunchKernelGGL((k_oversample<32>), dim3(n), dim3(NT), lds_or, st, oref, plW.desc32, w0);
+                hipLaunchKernelGGL((k_oversample<8, 4>), dim3(n), dim3(NT), lds_or, st, oref, twW, w0);
             });
             timed("k_oversample_search", n, st, [&] {
-                hipLaunchKernelGGL((k_oversample<40>), dim3(n), dim3(NT), lds_oz, st, osec, plZ.desc32, w0);
+                hipLaunchKernelGGL((k_oversample<8, 5>), dim3(n), dim3(NT), lds_oz, st, osec, twZ, w0);
             });
             timed("k_fine", n, st, [&] {
-                hipLaunchKernelGGL((k_fine<64, 80, 80, 3>), dim3(n), dim3(NT), lds_f, st, fa, pl2.desc32,
-                                   pl16.desc32, w0);
+                hipLaunchKernelGGL((k_fine<64, 16, 5, 2>), dim3(n), dim3(NT), lds_f, st, fa, tw2, w0);
             });
         }
         HIP_OK(hipGetLastError());
     }
+    if (precision == 1) rerun_fp64(a, mc, mf);
 }
 
 }  // namespace ampcor

@@ -1,29 +1,21 @@
-// Fused per-window kernels: one workgroup per window keeps each stage's
-// intermediates in LDS/registers instead of HBM.
+// Building blocks of the fused per-window kernels (fused.hip): one workgroup
+// of NT threads per window keeps the window's chips, spectra and surfaces in
+// LDS; every FFT transforms all rows (or columns) of a chip at once in place.
 //
-//   k_coarse     chip gather + |z| + template mean + FP64 window sums + raw
-//                correlation + normalization + argmax + covariance + SNR +
-//                zoom start/shift            (cuAmpcorChunk.cpp:18-106)
-//   k_oversample chip gather + deramp + x2 band-limited interpolation + |z|
-//                (+ FP64 mean)               (cuAmpcorChunk.cpp:115-137)
-//   k_fine       FP64 window sums + oversampled correlation + normalization +
-//                trim + 16x surface interpolation + argmax + sub-pixel offset
-//                                            (cuAmpcorChunk.cpp:148-207)
-//
-// Correlations use row FFTs of length L >= search width plus a direct sum
-// over template rows per frequency bin:
-//   c[u,v] = sum_i (1/L) IFFT_L( conj(T_i) S_{i+u} )[v]
-// which equals the reference's padded 2-D FFT correlation (cuCorrFrequency.cu:63-79)
-// for every valid lag, with only S x L/2 complex values resident in LDS.
+// Correlations use row FFTs of length L >= search width plus a direct sum over
+// template rows per frequency bin:
+//   c[u,v] = (1/L) IFFT_L( sum_i conj(T_i) S_{i+u} )[v]
+// which equals the reference's zero-padded 2-D FFT correlation
+// (cuCorrFrequency.cu:63-79) at every valid lag.
 #pragma once
 #include <cfloat>
 #include "cx.h"
-#include "fft_lds.h"
+#include "fft_inplace.h"
 
 namespace ampcor {
 namespace fused {
 
-constexpr int NT = 256;   // threads per workgroup
+constexpr int NT = 512;   // threads per workgroup (8 wave64)
 
 using cf = cx<float>;
 
@@ -40,6 +32,7 @@ __device__ __forceinline__ cf pixel_cx(const void *img, int mode, int64_t off) {
     return {z.x, z.y};
 }
 
+// FP64 workgroup sum; `sh` holds NT/64 doubles
 __device__ __forceinline__ double block_sum(double v, double *sh) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -54,16 +47,17 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
 }
 
 // ---------------------------------------------------------------------------
-// FP64 window sums -> normalization factors nrm[u][v] = 1/sqrt((s2 - s1^2/n) t2 + FLT_EPSILON)
-// (cuCorrNormalization.cu:402-423). Waves own contiguous lag rows; lanes own
-// columns (CPL per lane); vertical sums slide down, horizontal sums come from
-// wave prefix scans. No barriers inside.
+// FP64 window sums -> nrm[u][v] = 1/sqrt((s2 - s1^2/n) t2 + FLT_EPSILON)
+// (cuCorrNormalization.cu:402-423, cuCorrNormalizationSAT.cu:237-240).
+// Waves own contiguous lag rows, lanes own columns (CPL per lane); vertical
+// sums slide down the rows, horizontal window sums come from wave prefix
+// scans (differences of FP64 prefix sums). No barriers inside.
 // ---------------------------------------------------------------------------
-template <int CPL>
-__device__ void window_norms(const float *amp, int pitch, int SW, int TH, int TW, int NLh, int NLw,
-                             double t2, float *nrm) {
+template <int CPL, class Amp>
+__device__ void window_norms(Amp amp, int SW, int TH, int TW, int NLh, int NLw, double t2, float *nrm) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int per = (NLh + NT / 64 - 1) / (NT / 64);
+    constexpr int NW = NT / 64;
+    const int per = (NLh + NW - 1) / NW;
     const int u_beg = wave * per, u_end = min(NLh, u_beg + per);
     if (u_beg >= u_end) return;
     double g1[CPL], g2[CPL];
@@ -72,13 +66,15 @@ __device__ void window_norms(const float *amp, int pitch, int SW, int TH, int TW
         g1[m] = 0.0; g2[m] = 0.0;
         const int c = lane + 64 * m;
         if (c < SW)
+            #pragma unroll 1
             for (int i = 0; i < TH; ++i) {
-                const double v = amp[(u_beg + i) * pitch + c];
+                const double v = amp(u_beg + i, c);
                 g1[m] += v;
                 g2[m] += v * v;
             }
     }
     const double inv_n = 1.0 / double(TH * TW);
+    #pragma unroll 1
     for (int u = u_beg; u < u_end; ++u) {
         double p1[CPL], p2[CPL];
         double c1 = 0.0, c2 = 0.0;
@@ -95,7 +91,7 @@ __device__ void window_norms(const float *amp, int pitch, int SW, int TH, int TW
             c1 = __shfl(x1, 63);
             c2 = __shfl(x2, 63);
         }
-        // window sums for v = lane + 64*k: P(v+TW-1) - P(v-1)
+        #pragma unroll 1
         for (int k = 0; k * 64 < NLw; ++k) {
             const int v = lane + 64 * k;
             const int hi = min(v + TW - 1, SW - 1), lo = v - 1;
@@ -118,7 +114,7 @@ __device__ void window_norms(const float *amp, int pitch, int SW, int TH, int TW
             for (int m = 0; m < CPL; ++m) {
                 const int c = lane + 64 * m;
                 if (c < SW) {
-                    const double vn = amp[(u + TH) * pitch + c], vo = amp[u * pitch + c];
+                    const double vn = amp(u + TH, c), vo = amp(u, c);
                     g1[m] += vn - vo;
                     g2[m] += vn * vn - vo * vo;
                 }
@@ -128,180 +124,183 @@ __device__ void window_norms(const float *amp, int pitch, int SW, int TH, int TW
 }
 
 // ---------------------------------------------------------------------------
-// real row FFTs: up to 8 real rows (two per complex transform) -> packed half
-// spectra rows of L/2 complex, slot 0 = (DC, Nyquist). rowval(r, x) gives the
-// real input (x < L). work holds 2*4*L complex.
+// In-place split of packed real-pair spectra. Signal j (pitch P) holds
+// Z = FFT(a + i b) in digit-reversed order (fft_nat2dr); afterwards slots
+// [0, H) hold A_k (row 2j) and [H, 2H) hold B_k (row 2j+1), k < H = L/2, with
+// slot 0 = (DC, Nyquist) of that row.
 // ---------------------------------------------------------------------------
-template <int L, class RowFn>
-__device__ void rfft_rows8(RowFn rowval, int nr, cf *work, const FftDesc &d, cf *dst) {
-    constexpr int H = L / 2;
-    for (int i = threadIdx.x; i < 4 * L; i += NT) {
-        const int j = i / L, x = i - j * L;
-        const float a = (2 * j < nr) ? rowval(2 * j, x) : 0.f;
-        const float b = (2 * j + 1 < nr) ? rowval(2 * j + 1, x) : 0.f;
-        work[i] = {a, b};
-    }
-    __syncthreads();
-    const cf *Z = fft_lds<float, -1>(work, work + 4 * L, d, 4, 1, L);
-    for (int i = threadIdx.x; i < 4 * H; i += NT) {
-        const int j = i / H, k = i - j * H;
-        const cf *z = Z + j * L;
-        cf A, B;
-        if (k == 0) {
-            A = {z[0].x, z[H].x};
-            B = {z[0].y, z[H].y};
-        } else {
-            const cf zk = z[k], zc = conj(z[L - k]);
-            A = (zk + zc) * 0.5f;
-            const cf D = (zk - zc) * 0.5f;
-            B = {D.y, -D.x};
+template <int R1, int R2>
+__device__ void split_pairs_inplace(cf *buf, int nsig, int P) {
+    constexpr int L = R1 * R2, H = L / 2, U = 4;
+    const int total = nsig * H;
+    #pragma unroll 1
+    for (int base = 0; base < total; base += NT * U) {
+        cf av[U], bv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int it = base + u * NT + threadIdx.x;
+            if (it < total) {
+                const int j = it / H, k = it - j * H;
+                const cf *z = buf + j * P;
+                if (k == 0) {
+                    const cf z0 = z[0], zh = z[dr_pos<R1, R2>(H)];
+                    av[u] = {z0.x, zh.x};
+                    bv[u] = {z0.y, zh.y};
+                } else {
+                    const cf zk = z[dr_pos<R1, R2>(k)], zc = conj(z[dr_pos<R1, R2>(L - k)]);
+                    av[u] = (zk + zc) * 0.5f;
+                    const cf d = (zk - zc) * 0.5f;
+                    bv[u] = {d.y, -d.x};
+                }
+            }
         }
-        if (2 * j < nr) dst[(2 * j) * H + k] = A;
-        if (2 * j + 1 < nr) dst[(2 * j + 1) * H + k] = B;
+        __syncthreads();
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int it = base + u * NT + threadIdx.x;
+            if (it < total) {
+                const int j = it / H, k = it - j * H;
+                buf[j * P + k] = av[u];
+                buf[j * P + H + k] = bv[u];
+            }
+        }
+        __syncthreads();
     }
-    __syncthreads();
 }
 
-// ---------------------------------------------------------------------------
-// Correlation of template rows (staged in batches of 8 by fill(i0, nr), then read
-// by trow(i, x) from any thread) with
-// the search spectra resident in `sp` ([SH][L/2] packed). Lags u < NLh,
-// v < NLw. Thread (f, b): bin slot f = tid % H, lag block b = tid / H holding
-// K consecutive lags. Result (unnormalized) -> corr[NLh][NLw]. `sp` is
-// reused as scratch for the inverse transforms.
-// ---------------------------------------------------------------------------
-template <int L, int K, class FillFn, class TRowFn>
-__device__ void correlate_rows(FillFn fill, TRowFn trow, int TH, int SH, int NLh, int NLw, cf *sp, cf *work,
-                               cf *tb, const FftDesc &d, float *corr) {
-    constexpr int H = L / 2;
-    const int f = threadIdx.x % H;
-    const int blk = threadIdx.x / H;
-    const int u0 = blk * K;
-    const bool active = (blk < NT / H) && (u0 < NLh);
-    cf q[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) q[k] = {0.f, 0.f};
+// address of row r's half spectrum after split_pairs_inplace
+template <int H>
+__device__ __forceinline__ int row_slot(int r, int P) { return (r >> 1) * P + (r & 1) * H; }
 
-    for (int i0 = 0; i0 < TH; i0 += 8) {
-        const int nr = min(8, TH - i0);
-        fill(i0, nr);          // stage template rows i0..i0+nr-1 where trow can read them
-        __syncthreads();
-        rfft_rows8<L>([&](int r, int x) { return trow(i0 + r, x); }, nr, work, d, tb);
-        if (active) {
-            cf sw[K + 7];
+// ---------------------------------------------------------------------------
+// MAC over template rows [i_beg, i_end): q[k] += conj(T_i[f]) S_{i+u0+k}[f]
+// (slot 0 packs two real bins: (DC, Nyquist) multiply component-wise)
+// ---------------------------------------------------------------------------
+template <int H, int K>
+__device__ __forceinline__ void mac_rows(cf (&q)[K], const cf *S, int Sp, const cf *Tsp, int Tp, int t_off,
+                                         int i_beg, int i_end, int u0, int SH, int f) {
+    #pragma unroll 1
+    for (int i0 = i_beg; i0 < i_end; i0 += 8) {
+        cf sw[K + 7];
 #pragma unroll
-            for (int m = 0; m < K + 7; ++m) {
-                const int row = i0 + u0 + m;
-                sw[m] = (row < SH) ? sp[row * H + f] : cf{0.f, 0.f};
-            }
+        for (int m = 0; m < K + 7; ++m) {
+            const int row = i0 + u0 + m;
+            sw[m] = (row < SH) ? S[row_slot<H>(row, Sp) + f] : cf{0.f, 0.f};
+        }
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                if (i < nr) {
-                    const cf tv = tb[i * H + f];
-                    if (f == 0) {
+        for (int i = 0; i < 8; ++i) {
+            if (i0 + i < i_end) {
+                const cf tv = Tsp[row_slot<H>(i0 + i - t_off, Tp) + f];
+                if (f == 0) {
 #pragma unroll
-                        for (int k = 0; k < K; ++k) {
-                            q[k].x += tv.x * sw[i + k].x;
-                            q[k].y += tv.y * sw[i + k].y;
-                        }
-                    } else {
+                    for (int k = 0; k < K; ++k) {
+                        q[k].x += tv.x * sw[i + k].x;
+                        q[k].y += tv.y * sw[i + k].y;
+                    }
+                } else {
 #pragma unroll
-                        for (int k = 0; k < K; ++k) {
-                            const cf s = sw[i + k];
-                            q[k].x += tv.x * s.x + tv.y * s.y;
-                            q[k].y += tv.x * s.y - tv.y * s.x;
-                        }
+                    for (int k = 0; k < K; ++k) {
+                        const cf s = sw[i + k];
+                        q[k].x += tv.x * s.x + tv.y * s.y;
+                        q[k].y += tv.x * s.y - tv.y * s.x;
                     }
                 }
             }
         }
-        __syncthreads();   // tb is rewritten by the next batch
     }
-    // spectra no longer needed: store Q[u][f] in sp
-    if (active) {
-#pragma unroll
-        for (int k = 0; k < K; ++k)
-            if (u0 + k < NLh) sp[(u0 + k) * H + f] = q[k];
-    }
-    __syncthreads();
-    // inverse: pack lag pairs (u, u+1) into one complex transform, 4 at a time
+}
+
+// Inverse of the accumulated Q[u][f] (Qs: [NLh][H]) into corr[u][v] * nrm[u][v]
+// for v < NLw: lag pairs (2p, 2p+1) share one complex transform placed in Y
+// (npair signals, pitch P, digit-reversed input order).
+template <int R1, int R2>
+__device__ void inverse_lags(const cf *Qs, int NLh, int NLw, cf *Y, int P, const cx<float> *tw, const float *nrm,
+                             float *corr) {
+    constexpr int L = R1 * R2, H = L / 2;
     const int npair = (NLh + 1) / 2;
+    for (int i = threadIdx.x; i < npair * L; i += NT) {
+        const int p = i / L, k = i - p * L;
+        const int ua = 2 * p, ub = ua + 1;
+        const cf *qa = Qs + ua * H;
+        const bool hb = ub < NLh;
+        const cf *qb = Qs + (hb ? ub : ua) * H;
+        cf y;
+        if (k == 0) y = {qa[0].x, hb ? qb[0].x : 0.f};
+        else if (k == H) y = {qa[0].y, hb ? qb[0].y : 0.f};
+        else if (k < H) {
+            const cf a = qa[k], b = hb ? qb[k] : cf{0.f, 0.f};
+            y = {a.x - b.y, a.y + b.x};
+        } else {
+            const cf a = qa[L - k], b = hb ? qb[L - k] : cf{0.f, 0.f};
+            y = {a.x + b.y, b.x - a.y};
+        }
+        Y[p * P + dr_pos<R1, R2>(k)] = y;
+    }
+    __syncthreads();
+    fft_dr2nat<float, +1, R1, R2, NT>(Y, npair, P, 1, tw);
     const float invL = 1.0f / float(L);
-    for (int p0 = 0; p0 < npair; p0 += 4) {
-        const int np = min(4, npair - p0);
-        for (int i = threadIdx.x; i < 4 * L; i += NT) {
-            const int j = i / L, k = i - j * L;
-            const int ua = 2 * (p0 + j), ub = ua + 1;
-            cf y = {0.f, 0.f};
-            if (j < np) {
-                const cf *qa = sp + ua * H;
-                const cf *qb = (ub < NLh) ? sp + ub * H : nullptr;
-                if (k == 0) y = {qa[0].x, qb ? qb[0].x : 0.f};
-                else if (k == H) y = {qa[0].y, qb ? qb[0].y : 0.f};
-                else if (k < H) {
-                    const cf a = qa[k], b = qb ? qb[k] : cf{0.f, 0.f};
-                    y = {a.x - b.y, a.y + b.x};
-                } else {
-                    const cf a = qa[L - k], b = qb ? qb[L - k] : cf{0.f, 0.f};
-                    y = {a.x + b.y, b.x - a.y};
-                }
-            }
-            work[i] = y;
-        }
-        __syncthreads();
-        const cf *Y = fft_lds<float, +1>(work, work + 4 * L, d, 4, 1, L);
-        for (int i = threadIdx.x; i < 4 * NLw; i += NT) {
-            const int j = i / NLw, v = i - j * NLw;
-            const int ua = 2 * (p0 + j);
-            if (j < np) {
-                const cf y = Y[j * L + v];
-                corr[ua * NLw + v] = y.x * invL;
-                if (ua + 1 < NLh) corr[(ua + 1) * NLw + v] = y.y * invL;
-            }
-        }
-        __syncthreads();
+    for (int i = threadIdx.x; i < NLh * NLw; i += NT) {
+        const int u = i / NLw, v = i - u * NLw;
+        const cf y = Y[(u >> 1) * P + v];
+        corr[i] = ((u & 1) ? y.y : y.x) * invL * nrm[i];
     }
+    __syncthreads();
 }
 
-// in-place (bottom-up) conversion of float amplitudes amp[SH][SW] (pitch SW)
-// into packed half spectra rows [SH][L/2] complex occupying the same memory
-template <int L>
-__device__ void search_spectra_inplace(float *amp, int SH, int SW, cf *work, const FftDesc &d) {
-    cf *sp = reinterpret_cast<cf *>(amp);
-    const int nb = (SH + 7) / 8;
-    for (int b = nb - 1; b >= 0; --b) {
-        const int r0 = b * 8, nr = min(8, SH - r0);
-        rfft_rows8<L>([&](int r, int x) { return x < SW ? amp[(r0 + r) * SW + x] : 0.f; }, nr, work, d,
-                      sp + r0 * (L / 2));
+// running top-2 for argmax with the reference tie rule: best (value, key) and
+// the largest value among all other candidates
+struct Top2 {
+    float v1 = -FLT_MAX, v2 = -FLT_MAX;
+    uint64_t k1 = ~uint64_t(0);
+    __device__ __forceinline__ void add(float v, uint64_t k) {
+        if (!(v > -FLT_MAX)) return;
+        if (better(v, k, v1, k1)) {
+            v2 = fmaxf(v2, v1);
+            v1 = v; k1 = k;
+        } else {
+            v2 = fmaxf(v2, v);
+        }
     }
-}
+    __device__ __forceinline__ void merge(float ov1, uint64_t ok1, float ov2) {
+        if (better(ov1, ok1, v1, k1)) {
+            v2 = fmaxf(fmaxf(v2, ov2), v1);
+            v1 = ov1; k1 = ok1;
+        } else {
+            v2 = fmaxf(v2, fmaxf(ov1, ov2));
+        }
+    }
+    // workgroup reduction; every thread returns with the block result
+    __device__ void reduce(float *shv, float *shv2, uint64_t *shk) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const float a = __shfl_xor(v1, o), b = __shfl_xor(v2, o);
+            const uint64_t k = __shfl_xor(k1, o);
+            merge(a, k, b);
+        }
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) {
+            shv[threadIdx.x >> 6] = v1; shv2[threadIdx.x >> 6] = v2; shk[threadIdx.x >> 6] = k1;
+        }
+        __syncthreads();
+        v1 = shv[0]; v2 = shv2[0]; k1 = shk[0];
+#pragma unroll
+        for (int i = 1; i < NT / 64; ++i) merge(shv[i], shk[i], shv2[i]);
+        __syncthreads();
+    }
+    __device__ __forceinline__ int index() const { return k1 == ~uint64_t(0) ? 0 : int(k1 & 0xffffffffu); }
+    // top-1 minus top-2 (0 for exact ties; +inf when there is no runner-up)
+    __device__ __forceinline__ float margin() const { return v2 > -FLT_MAX ? v1 - v2 : INFINITY; }
+};
 
-// argmax with the reference tie rule over n values in LDS; returns (idx, val) to all threads
-__device__ __forceinline__ void block_argmax(const float *v, int n, int &idx, float &val, float *shv,
-                                             uint64_t *shk) {
-    float bv = -FLT_MAX;
-    uint64_t bk = ~uint64_t(0);
-    for (int i = threadIdx.x; i < n; i += NT) {
-        const float x = v[i];
-        const uint64_t k = maxloc_key(uint32_t(i));
-        if (x > -FLT_MAX && better(x, k, bv, bk)) { bv = x; bk = k; }
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const float ov = __shfl_xor(bv, o);
-        const uint64_t ok = __shfl_xor(bk, o);
-        if (better(ov, ok, bv, bk)) { bv = ov; bk = ok; }
-    }
-    __syncthreads();
-    if ((threadIdx.x & 63) == 0) { shv[threadIdx.x >> 6] = bv; shk[threadIdx.x >> 6] = bk; }
-    __syncthreads();
-    bv = shv[0]; bk = shk[0];
-#pragma unroll
-    for (int i = 1; i < NT / 64; ++i)
-        if (better(shv[i], shk[i], bv, bk)) { bv = shv[i]; bk = shk[i]; }
-    __syncthreads();
-    idx = (bk == ~uint64_t(0)) ? 0 : int(bk & 0xffffffffu);
-    val = bv;
+// argmax over n values in LDS with the reference tie rule
+__device__ __forceinline__ Top2 block_argmax(const float *v, int n, double *red) {
+    Top2 t;
+    for (int i = threadIdx.x; i < n; i += NT) t.add(v[i], maxloc_key(uint32_t(i)));
+    float *shv = reinterpret_cast<float *>(red + 8);
+    float *shv2 = shv + 8;
+    uint64_t *shk = reinterpret_cast<uint64_t *>(red + 16);
+    t.reduce(shv, shv2, shk);
+    return t;
 }
 
 }  // namespace fused

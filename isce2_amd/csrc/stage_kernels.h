@@ -469,6 +469,12 @@ __global__ void k_sinc(const T *in, int nin, T *out, int nout, const int32_t *sh
     }
 }
 
+template <class T>
+__global__ void k_to_float(const T *in, float *out, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = float(in[i]);
+}
+
 // final offset (cuOffset.cu:137-146)
 static __global__ void k_subpixel(const int32_t *start, const int32_t *zidx, int n, float ratio, int hs_down,
                            int hs_across, float *offset) {

@@ -110,14 +110,26 @@ class Engine:
         if debug:
             for k in ("maxloc", "start", "shift", "zoom_idx"):
                 res[k] = torch.empty((n, 2), dtype=torch.int32, device=dev)
-            res["maxval"] = torch.empty((n,), dtype=torch.float32, device=dev)
+            for k in ("maxval", "margin_coarse", "margin_fine"):
+                res[k] = torch.full((n,), float("nan"), dtype=torch.float32, device=dev)
             code = self.lib.ampcor_run_windows_debug(
                 *common, _ptr(res["maxloc"]), _ptr(res["start"]), _ptr(res["shift"]),
-                _ptr(res["zoom_idx"]), _ptr(res["maxval"]), int(chunk), int(mode), st)
+                _ptr(res["zoom_idx"]), _ptr(res["maxval"]), _ptr(res["margin_coarse"]),
+                _ptr(res["margin_fine"]), int(chunk), int(mode), st)
         else:
             code = self.lib.ampcor_run_windows(*common, int(chunk), int(mode), st)
         check(code, self._h)
         return res
+
+    def set_precision(self, mode: int, tau_coarse: float = -1.0, tau_fine: float = -1.0):
+        """0: FP32 only; 1: FP32 + FP64 re-run of near-tie windows (default); 2: FP64 for all."""
+        check(self.lib.ampcor_set_precision(self._h, int(mode), float(tau_coarse), float(tau_fine)), self._h)
+
+    def precision_stats(self) -> tuple:
+        """(windows checked, windows re-run in FP64) since the previous call."""
+        a, b = C.c_int64(), C.c_int64()
+        check(self.lib.ampcor_precision_stats(self._h, C.byref(a), C.byref(b)), self._h)
+        return a.value, b.value
 
     def profile(self, enable: bool):
         """Reset and enable (or disable) per-kernel HIP event timing."""
