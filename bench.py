@@ -77,6 +77,7 @@ def main():
         out = eng.run_windows(ref, sec, *starts, mode=mode, out=out)
     torch.cuda.synchronize(dev)
     eng.profile(True)
+    eng.precision_stats()
     torch.cuda.synchronize(dev)
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -85,6 +86,7 @@ def main():
     elapsed = time.perf_counter() - t0
     prof = eng.profile_report()
     eng.profile(False)
+    checked, rerun = eng.precision_stats()
     wps = n * args.steps / elapsed
     ms_step = 1000 * elapsed / args.steps
     gpu_out = {k: v.cpu().numpy() for k, v in out.items()}
@@ -165,6 +167,9 @@ def main():
                                "+-8 px, window 64x64, half search 20, skip 32, 31248 windows, raw ovs 2, "
                                "surface ovs 16, derampMethod %d" % args.deramp,
                    "windows": n, "mode": args.mode, "engine_path": "fused" if sizes["fused"] and args.mode != "staged" else "staged"},
+        "precision": {"mode": "FP32 kernels, FP64 window sums; FP64 re-run of windows whose argmax margin < 2e-6",
+                      "windows_rerun_fp64_per_step": rerun // max(1, args.steps),
+                      "rerun_fraction": round(rerun / max(1, checked), 5)},
         "roofline": roof, "cpu_baseline": cpu, "kernels": kernels, "plain_staged": plain, "accuracy": accuracy,
         "scene_gen_s": round(gen_s, 1),
     }

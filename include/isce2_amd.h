@@ -126,7 +126,9 @@ int ampcor_run_windows_debug(ampcor_ctx *ctx,
  *     below tau_coarse / tau_fine are re-run through an FP64 pipeline, so the
  *     integer peaks are those of an FP64 evaluation.
  *   mode 2: FP64 pipeline for every window.
- * Negative taus keep the current thresholds. ampcor_precision_stats returns
+ * Default thresholds: 2e-6 for both (surface values are of order 1 and FP32
+ * rounding moves them by a few 1e-7). Negative taus keep the current
+ * thresholds. ampcor_precision_stats returns
  * the windows checked and re-run since the last call. */
 int ampcor_set_precision(ampcor_ctx *ctx, int32_t mode, float tau_coarse, float tau_fine);
 int ampcor_precision_stats(ampcor_ctx *ctx, int64_t *checked, int64_t *rerun);

@@ -1,6 +1,7 @@
 // Complex arithmetic and small helpers shared by the engine's kernels.
 #pragma once
 #include <hip/hip_runtime.h>
+#include <cfloat>
 #include <stdint.h>
 
 namespace ampcor {
@@ -45,5 +46,58 @@ template <class T>
 __device__ __forceinline__ bool better(T va, uint64_t ka, T vb, uint64_t kb) {
     return va > vb || (va == vb && ka < kb);
 }
+
+// Running top-2 of an argmax with the tie rule above: best (value, key) and
+// the largest value among the other candidates. margin() = top-1 minus top-2
+// tells how far the winner is from a rounding-level tie.
+template <class T>
+struct Top2 {
+    T v1 = T(-FLT_MAX), v2 = T(-FLT_MAX);
+    uint64_t k1 = ~uint64_t(0);
+    __device__ __forceinline__ static T mx(T a, T b) { return a > b ? a : b; }
+    __device__ __forceinline__ void add(T v, uint64_t k) {
+        if (!(v > T(-FLT_MAX))) return;   // -FLT_MAX and NaN are never selected
+        if (better(v, k, v1, k1)) {
+            v2 = mx(v2, v1);
+            v1 = v; k1 = k;
+        } else {
+            v2 = mx(v2, v);
+        }
+    }
+    __device__ __forceinline__ void merge(T ov1, uint64_t ok1, T ov2) {
+        if (better(ov1, ok1, v1, k1)) {
+            v2 = mx(mx(v2, ov2), v1);
+            v1 = ov1; k1 = ok1;
+        } else {
+            v2 = mx(v2, mx(ov1, ov2));
+        }
+    }
+    // workgroup reduction over NTH threads (wave64); shv, shv2, shk hold NTH/64
+    // entries each; every thread returns with the workgroup result
+    template <int NTH>
+    __device__ void reduce(T *shv, T *shv2, uint64_t *shk) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const T a = __shfl_xor(v1, o), b = __shfl_xor(v2, o);
+            const uint64_t k = __shfl_xor(k1, o);
+            merge(a, k, b);
+        }
+        __syncthreads();
+        if ((threadIdx.x & 63) == 0) {
+            shv[threadIdx.x >> 6] = v1; shv2[threadIdx.x >> 6] = v2; shk[threadIdx.x >> 6] = k1;
+        }
+        __syncthreads();
+        v1 = shv[0]; v2 = shv2[0]; k1 = shk[0];
+#pragma unroll
+        for (int i = 1; i < NTH / 64; ++i) merge(shv[i], shk[i], shv2[i]);
+        __syncthreads();
+    }
+    // index of the winner (0 when no value was eligible)
+    __device__ __forceinline__ int index() const { return k1 == ~uint64_t(0) ? 0 : int(k1 & 0xffffffffu); }
+    // 0 for exact ties; +inf when there is no runner-up
+    __device__ __forceinline__ float margin() const {
+        return v2 > T(-FLT_MAX) ? float(v1 - v2) : INFINITY;
+    }
+};
 
 }  // namespace ampcor

@@ -97,8 +97,10 @@ struct Engine {
     };
     // precision: 0 FP32 only; 1 FP32 + FP64 re-run of windows whose argmax
     // margins fall below the thresholds; 2 FP64 staged pipeline for every window
+    // FP32 surfaces (values of order 1) carry rounding errors of a few 1e-7;
+    // on a 31k-window speckle field every FP32/FP64 flip had a margin <= 6e-8
     int precision = 1;
-    float tau_coarse = 2e-5f, tau_fine = 2e-5f;
+    float tau_coarse = 2e-6f, tau_fine = 2e-6f;
     long long prec_checked = 0, prec_rerun = 0;
     void run(const RunArgs &a);
     void run_staged(const RunArgs &a);

@@ -263,7 +263,8 @@ void Engine::run_staged_t(const RunArgs &a) {
         hipLaunchKernelGGL(k_normalize<T>, dim3(n), dim3(256), 0, st, corr, sz.raw_corr_h, sz.raw_corr_w, r_ref,
                            W_h, W_w, sat1, sat2, S_h, S_w);
         // argmax, covariance, SNR, zoom start/shift (cuAmpcorChunk.cpp:69-106)
-        hipLaunchKernelGGL(k_maxloc<T>, dim3(n), dim3(256), 0, st, corr, sz.raw_corr_h, sz.raw_corr_w, loc, val);
+        hipLaunchKernelGGL(k_maxloc<T>, dim3(n), dim3(256), 0, st, corr, sz.raw_corr_h, sz.raw_corr_w, loc, val,
+                           a.margin_coarse ? a.margin_coarse + w0 : nullptr);
         hipLaunchKernelGGL(k_stats<T>, dim3(idivup(n, 64)), dim3(64), 0, st, corr, sz.raw_corr_h, sz.raw_corr_w, n,
                            loc, val, W_h * W_w, sz.corr_raw_zoom_h, sz.corr_raw_zoom_w, p.hs_down, p.hs_across,
                            sz.half_zoom, a.cov + 3 * size_t(w0), a.snr + w0, start, shift);
@@ -315,7 +316,8 @@ void Engine::run_staged_t(const RunArgs &a) {
                                int64_t(n) * So * So);
         }
         // sub-pixel argmax and final offset (cuAmpcorChunk.cpp:194-207)
-        hipLaunchKernelGGL(k_maxloc<T>, dim3(n), dim3(256), 0, st, r_surf, So, So, zidx, (T *)nullptr);
+        hipLaunchKernelGGL(k_maxloc<T>, dim3(n), dim3(256), 0, st, r_surf, So, So, zidx, (T *)nullptr,
+                           a.margin_fine ? a.margin_fine + w0 : nullptr);
         hipLaunchKernelGGL(k_subpixel, dim3(idivup(n, 64)), dim3(64), 0, st, start, zidx, n,
                            1.0f / float(p.surf_ovs * p.raw_ovs), p.hs_down, p.hs_across, a.offset + 2 * size_t(w0));
         if (a.maxloc) HIP_OK(hipMemcpyAsync(a.maxloc + 2 * size_t(w0), loc, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToDevice, st));
@@ -327,7 +329,17 @@ void Engine::run_staged_t(const RunArgs &a) {
     }
 }
 
-void Engine::run_staged(const RunArgs &a) { run_staged_t<float>(a); }
+// FP32 staged pipeline; under precision 1 the windows whose argmax margins
+// fall below the thresholds are re-run in FP64
+void Engine::run_staged(const RunArgs &a) {
+    RunArgs b = a;
+    if (precision == 1) {
+        if (!b.margin_coarse) b.margin_coarse = buf<float>("f_mc", a.n);
+        if (!b.margin_fine) b.margin_fine = buf<float>("f_mf", a.n);
+    }
+    run_staged_t<float>(b);
+    if (precision == 1) rerun_fp64(b, b.margin_coarse, b.margin_fine);
+}
 template void Engine::run_staged_t<double>(const RunArgs &a);
 
 namespace {
@@ -371,6 +383,7 @@ void Engine::rerun_fp64(const RunArgs &a, const float *mc, const float *mf) {
     RunArgs sub = a;
     sub.ref_d = st4; sub.ref_a = st4 + m; sub.sec_d = st4 + 2 * m; sub.sec_a = st4 + 3 * m;
     sub.n = m;
+    sub.margin_coarse = sub.margin_fine = nullptr;
     sub.offset = o_off; sub.snr = o_snr; sub.cov = o_cov; sub.maxval = o_val;
     sub.maxloc = o_i; sub.start = o_i + 2 * m; sub.shift = o_i + 4 * m; sub.zoom_idx = o_i + 6 * m;
     sub.chunk = 0;

@@ -125,7 +125,7 @@ __global__ void __launch_bounds__(NT) k_coarse(CoarseArgs a, const cf *tw, int w
     __syncthreads();
     inverse_lags<R1, R2>(Qs, NLh, NLw, S + lay.y_off, P, tw, nrm, corr);
 
-    const Top2 top = block_argmax(corr, NLh * NLw, red);
+    const Top2<float> top = block_argmax(corr, NLh * NLw, red);
     const int idx = top.index();
     const float val = top.v1;
     if (threadIdx.x == 0) {
@@ -406,7 +406,7 @@ __global__ void __launch_bounds__(NT) k_fine(FineArgs a, const cf *tw, int w0) {
     }
     __syncthreads();
     // out[16aa+b][q] = Re DFT16_aa( G[k1][q] e^{-2 pi i b ks1/256} ); running argmax
-    Top2 top;
+    Top2<float> top;
     for (int task = threadIdx.x; task < SO * 16; task += NT) {
         const int qq = task & (SO - 1), b = task >> 8;
         cf x[16];
@@ -419,7 +419,7 @@ __global__ void __launch_bounds__(NT) k_fine(FineArgs a, const cf *tw, int w0) {
 #pragma unroll
         for (int aa = 0; aa < 16; ++aa) top.add(x[aa].x, maxloc_key(uint32_t((16 * aa + b) * SO + qq)));
     }
-    top.reduce(reinterpret_cast<float *>(red + 8), reinterpret_cast<float *>(red + 8) + 8,
+    top.reduce<NT>(reinterpret_cast<float *>(red + 8), reinterpret_cast<float *>(red + 8) + 8,
                reinterpret_cast<uint64_t *>(red + 16));
     if (threadIdx.x == 0) {
         const int idx = top.index();

@@ -127,19 +127,24 @@ __device__ void window_norms(Amp amp, int SW, int TH, int TW, int NLh, int NLw, 
 // In-place split of packed real-pair spectra. Signal j (pitch P) holds
 // Z = FFT(a + i b) in digit-reversed order (fft_nat2dr); afterwards slots
 // [0, H) hold A_k (row 2j) and [H, 2H) hold B_k (row 2j+1), k < H = L/2, with
-// slot 0 = (DC, Nyquist) of that row.
+// slot 0 = (DC, Nyquist) of that row. A round covers whole signals only: the
+// reads of signal j (bins k and L-k) must all happen before any write to j.
 // ---------------------------------------------------------------------------
 template <int R1, int R2>
 __device__ void split_pairs_inplace(cf *buf, int nsig, int P) {
     constexpr int L = R1 * R2, H = L / 2, U = 4;
+    constexpr int SIG_PER_ROUND = (NT * U) / H;
+    static_assert(SIG_PER_ROUND >= 1, "a round must hold at least one signal");
     const int total = nsig * H;
     #pragma unroll 1
-    for (int base = 0; base < total; base += NT * U) {
+    for (int j0 = 0; j0 < nsig; j0 += SIG_PER_ROUND) {
+        const int base = j0 * H;
+        const int end = min(total, base + SIG_PER_ROUND * H);
         cf av[U], bv[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int it = base + u * NT + threadIdx.x;
-            if (it < total) {
+            if (it < end) {
                 const int j = it / H, k = it - j * H;
                 const cf *z = buf + j * P;
                 if (k == 0) {
@@ -158,7 +163,7 @@ __device__ void split_pairs_inplace(cf *buf, int nsig, int P) {
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int it = base + u * NT + threadIdx.x;
-            if (it < total) {
+            if (it < end) {
                 const int j = it / H, k = it - j * H;
                 buf[j * P + k] = av[u];
                 buf[j * P + H + k] = bv[u];
@@ -247,59 +252,14 @@ __device__ void inverse_lags(const cf *Qs, int NLh, int NLw, cf *Y, int P, const
     __syncthreads();
 }
 
-// running top-2 for argmax with the reference tie rule: best (value, key) and
-// the largest value among all other candidates
-struct Top2 {
-    float v1 = -FLT_MAX, v2 = -FLT_MAX;
-    uint64_t k1 = ~uint64_t(0);
-    __device__ __forceinline__ void add(float v, uint64_t k) {
-        if (!(v > -FLT_MAX)) return;
-        if (better(v, k, v1, k1)) {
-            v2 = fmaxf(v2, v1);
-            v1 = v; k1 = k;
-        } else {
-            v2 = fmaxf(v2, v);
-        }
-    }
-    __device__ __forceinline__ void merge(float ov1, uint64_t ok1, float ov2) {
-        if (better(ov1, ok1, v1, k1)) {
-            v2 = fmaxf(fmaxf(v2, ov2), v1);
-            v1 = ov1; k1 = ok1;
-        } else {
-            v2 = fmaxf(v2, fmaxf(ov1, ov2));
-        }
-    }
-    // workgroup reduction; every thread returns with the block result
-    __device__ void reduce(float *shv, float *shv2, uint64_t *shk) {
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            const float a = __shfl_xor(v1, o), b = __shfl_xor(v2, o);
-            const uint64_t k = __shfl_xor(k1, o);
-            merge(a, k, b);
-        }
-        __syncthreads();
-        if ((threadIdx.x & 63) == 0) {
-            shv[threadIdx.x >> 6] = v1; shv2[threadIdx.x >> 6] = v2; shk[threadIdx.x >> 6] = k1;
-        }
-        __syncthreads();
-        v1 = shv[0]; v2 = shv2[0]; k1 = shk[0];
-#pragma unroll
-        for (int i = 1; i < NT / 64; ++i) merge(shv[i], shk[i], shv2[i]);
-        __syncthreads();
-    }
-    __device__ __forceinline__ int index() const { return k1 == ~uint64_t(0) ? 0 : int(k1 & 0xffffffffu); }
-    // top-1 minus top-2 (0 for exact ties; +inf when there is no runner-up)
-    __device__ __forceinline__ float margin() const { return v2 > -FLT_MAX ? v1 - v2 : INFINITY; }
-};
-
 // argmax over n values in LDS with the reference tie rule
-__device__ __forceinline__ Top2 block_argmax(const float *v, int n, double *red) {
-    Top2 t;
+__device__ __forceinline__ Top2<float> block_argmax(const float *v, int n, double *red) {
+    Top2<float> t;
     for (int i = threadIdx.x; i < n; i += NT) t.add(v[i], maxloc_key(uint32_t(i)));
     float *shv = reinterpret_cast<float *>(red + 8);
     float *shv2 = shv + 8;
     uint64_t *shk = reinterpret_cast<uint64_t *>(red + 16);
-    t.reduce(shv, shv2, shk);
+    t.reduce<NT>(shv, shv2, shk);
     return t;
 }
 

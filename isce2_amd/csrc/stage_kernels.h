@@ -227,42 +227,19 @@ __global__ void __launch_bounds__(256) k_normalize(T *corr, int ch, int cw, cons
 // argmax with the reference tie rule (cuOffset.cu:23-84), one block per surface
 // ---------------------------------------------------------------------------
 template <class T>
-__device__ __forceinline__ void argmax_reduce256(T &bv, uint64_t &bk, T *shv, uint64_t *shk) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const T ov = __shfl_xor(bv, o);
-        const uint64_t ok = __shfl_xor(bk, o);
-        if (better(ov, ok, bv, bk)) { bv = ov; bk = ok; }
-    }
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    __syncthreads();
-    if (lane == 0) { shv[wv] = bv; shk[wv] = bk; }
-    __syncthreads();
-    const int nw = (blockDim.x + 63) >> 6;
-    bv = shv[0]; bk = shk[0];
-    for (int i = 1; i < nw; ++i)
-        if (better(shv[i], shk[i], bv, bk)) { bv = shv[i]; bk = shk[i]; }
-    __syncthreads();
-}
-
-template <class T>
-__global__ void __launch_bounds__(256) k_maxloc(const T *surf, int h, int w, int32_t *loc, T *val) {
-    __shared__ T shv[4];
+__global__ void __launch_bounds__(256) k_maxloc(const T *surf, int h, int w, int32_t *loc, T *val, float *margin) {
+    __shared__ T shv[4], shv2[4];
     __shared__ uint64_t shk[4];
     const T *s = surf + size_t(blockIdx.x) * h * w;
-    T bv = T(-FLT_MAX);
-    uint64_t bk = ~uint64_t(0);
-    for (int i = threadIdx.x; i < h * w; i += blockDim.x) {
-        const T v = s[i];
-        const uint64_t k = maxloc_key(uint32_t(i));
-        if (v > T(-FLT_MAX) && better(v, k, bv, bk)) { bv = v; bk = k; }
-    }
-    argmax_reduce256(bv, bk, shv, shk);
+    Top2<T> top;
+    for (int i = threadIdx.x; i < h * w; i += blockDim.x) top.add(s[i], maxloc_key(uint32_t(i)));
+    top.template reduce<256>(shv, shv2, shk);
     if (threadIdx.x == 0) {
-        const int idx = (bk == ~uint64_t(0)) ? 0 : int(bk & 0xffffffffu);
+        const int idx = top.index();
         loc[2 * blockIdx.x] = idx / w;
         loc[2 * blockIdx.x + 1] = idx % w;
-        if (val) val[blockIdx.x] = bv;
+        if (val) val[blockIdx.x] = top.v1;
+        if (margin) margin[blockIdx.x] = top.margin();
     }
 }
 

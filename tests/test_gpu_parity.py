@@ -56,7 +56,7 @@ def test_staged_default_config(eng, scene1, deramp):
     g, o = run_both(eng, ref, sec, cfg, starts, MODE_STAGED, chunk=7)
     c = compare(g, o)
     _report(f"staged deramp={deramp}", c)
-    _assert_parity(c, zoom_allow=1)
+    _assert_parity(c)
 
 
 def test_staged_hs20_sinc_and_time_domain(eng, scene1):
@@ -66,7 +66,7 @@ def test_staged_hs20_sinc_and_time_domain(eng, scene1):
     g, o = run_both(eng, ref, sec, cfg, starts, MODE_STAGED)
     c = compare(g, o)
     _report("staged sinc", c)
-    _assert_parity(c, zoom_allow=1)
+    _assert_parity(c)
 
 
 def test_staged_real_image_and_small_window(eng, scene1):
@@ -78,7 +78,7 @@ def test_staged_real_image_and_small_window(eng, scene1):
     g, o = run_both(eng, ref_r, sec_r, cfg, starts, MODE_STAGED)
     c = compare(g, o)
     _report("staged real", c)
-    _assert_parity(c, zoom_allow=1)
+    _assert_parity(c)
 
 
 def test_staged_edges_and_errors(eng, scene1):
@@ -94,7 +94,7 @@ def test_staged_edges_and_errors(eng, scene1):
     g, o = run_both(eng, ref, sec, cfg, (rd, ra, sd, sa), MODE_STAGED)
     c = compare(g, o)
     _report("staged edges", c)
-    _assert_parity(c, zoom_allow=1)
+    _assert_parity(c)
     # a chip one pixel past the edge is rejected before any kernel runs
     dev = torch.device("cuda", 0)
     t = torch.from_numpy(ref).to(dev)
@@ -118,7 +118,7 @@ def test_fused_matches_oracle(eng, scene1, W, hs, deramp):
     g, o = run_both(eng, ref, sec, cfg, starts, MODE_FUSED, chunk=5)
     c = compare(g, o)
     _report(f"fused W={W} hs={hs} deramp={deramp}", c)
-    _assert_parity(c, zoom_allow=1)
+    _assert_parity(c)
 
 
 def test_fused_real_image_and_edges(eng, scene1):
@@ -135,7 +135,7 @@ def test_fused_real_image_and_edges(eng, scene1):
     g, o = run_both(eng, ref_r, sec_r, cfg, (rd, ra, sd, sa), MODE_FUSED)
     c = compare(g, o)
     _report("fused real/edges", c)
-    _assert_parity(c, zoom_allow=1)
+    _assert_parity(c)
 
 
 def test_fused_equals_staged_integers(eng, scene1):
