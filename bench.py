@@ -41,6 +41,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the staged-pipeline comparison")
     ap.add_argument("--cpu-windows", type=int, default=4096)
     ap.add_argument("--oracle-windows", type=int, default=384)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy (scale reference for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this engine runs on one GPU per process; bench.py measures N=1 only "
@@ -53,6 +55,15 @@ def main():
     from isce2_amd.synth import speckle_pair_warp
 
     dev = torch.device("cuda", 0)
+    if args.copy_calibration:
+        n = 1 << 28
+        x = torch.ones(n, dtype=torch.float32, device=dev)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize(dev)
+        print(json.dumps({"copy_bytes_read_per_dispatch": 4 * n, "copy_bytes_written_per_dispatch": 4 * n}))
+        return
     H, Wd, W, HS, SKIP = 8192, 4096, 64, 20, 32
     t0 = time.time()
     ref_np, sec_np, dd, da = speckle_pair_warp(H, Wd, max_amp=8.0, jitter=0.05, tile=256, seed=0, device=dev)
