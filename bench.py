@@ -41,6 +41,11 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the staged-pipeline comparison")
     ap.add_argument("--cpu-windows", type=int, default=4096)
     ap.add_argument("--oracle-windows", type=int, default=384)
+    ap.add_argument("--no-uneven", action="store_true", help="skip the uneven chip-pair batch leg")
+    ap.add_argument("--uneven-pairs", type=int, default=8192)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--serve-jobs", type=int, default=60)
+    ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate, jobs/s")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (scale reference for HBM byte counters)")
     args = ap.parse_args()
@@ -134,6 +139,17 @@ def main():
         plain = {"windows_per_s": round(n / t_plain, 1), "ms_per_step": round(1000 * t_plain, 2),
                  "speedup_fused_over_staged": round(wps / (n / t_plain), 2),
                  "max_abs_offset_diff_vs_fused": float(np.abs(o2n["offset"] - gpu_out["offset"]).max())}
+        # the reference controller's schedule: staged stages over chunks of 64 windows
+        # (1 x 64, TopsProc/runDenseOffsets.py:234-235) with a stream sync per chunk
+        torch.cuda.synchronize(dev)
+        t1 = time.perf_counter()
+        for c0 in range(0, n, 64):
+            sl = [t[c0:c0 + 64] for t in starts]
+            eng.run_windows(ref, sec, *sl, mode=MODE_STAGED, chunk=64)
+            torch.cuda.current_stream(dev).synchronize()
+        t_ref = time.perf_counter() - t1
+        plain["reference_chunking_windows_per_s"] = round(n / t_ref, 1)
+        plain["speedup_fused_over_reference_chunking"] = round(wps / (n / t_ref), 2)
 
     # CPU baseline: the C++/OpenMP restatement of mroipac/ampcor (ampcor.F) on a bounded
     # random sample of the same windows; the FP64 oracle checks the GPU offsets on a subset
@@ -169,6 +185,9 @@ def main():
         truth = np.stack([dd[cr, cc], da[cr, cc]], axis=1)
         accuracy["median_abs_err_vs_true_field_px"] = float(np.median(np.abs(g_off - truth)))
 
+    uneven = None if args.no_uneven else bench_uneven(dev, args.uneven_pairs, max(1, args.steps // 2))
+    serving = None if args.no_serve else bench_serving(dev, ref_np, sec_np, args.serve_jobs, args.serve_rate)
+
     line = {
         "metric": "windows/sec @64x64 ±20px, offset err<0.01px vs FP64; serve p50/p99 latency",
         "value": round(wps, 1), "unit": "windows/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -182,9 +201,108 @@ def main():
                       "windows_rerun_fp64_per_step": rerun // max(1, args.steps),
                       "rerun_fraction": round(rerun / max(1, checked), 5)},
         "roofline": roof, "cpu_baseline": cpu, "kernels": kernels, "plain_staged": plain, "accuracy": accuracy,
+        "uneven_batch": uneven, "serving": serving,
         "scene_gen_s": round(gen_s, 1),
     }
     print(json.dumps(line), flush=True)
+
+
+# model MFLOP per chip pair by (window, half search) at the reference's transform sizes
+PAIR_MFLOP = {(32, 8): 7.05, (32, 16): 7.25, (32, 32): 7.88, (64, 8): 11.82, (64, 16): 12.16, (64, 32): 13.08,
+              (128, 8): 31.90, (128, 16): 32.54, (128, 32): 34.08, (256, 8): 118.23, (256, 16): 119.52,
+              (256, 32): 122.38}
+
+
+def bench_uneven(dev, n_pairs, steps):
+    """Uneven chip pairs (window 32/64/128/256, half search 8/16/32, sub-pixel shifts up to 0.8 of
+    the search): one autotuned, graph-captured multi-class step per pass."""
+    import torch
+    from isce2_amd.autotune import Autotuner
+    from isce2_amd.batch import ChipPairBatch
+    from isce2_amd.synth import chip_pairs_uneven_grouped
+    t0 = time.time()
+    groups = chip_pairs_uneven_grouped(n=n_pairs, seed=0, device=dev)
+    gen_s = time.time() - t0
+    t0 = time.time()
+    tuner = Autotuner(device=dev.index or 0, reps=2, sample=1024, radix_sample=64)
+    b = ChipPairBatch(groups, device=dev.index or 0, graph=True, tuner=tuner)
+    b.run()                                   # capture + first replay
+    tune_s = time.time() - t0
+    torch.cuda.synchronize(dev)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.run()
+    el = (time.perf_counter() - t0) / steps
+    r = b.results()
+    truth = np.empty((b.n, 2))
+    mflop = 0.0
+    for k, g in groups.items():
+        truth[g["idx"]] = g["shift"]
+        mflop += PAIR_MFLOP.get(k, 0.0) * len(g["idx"])
+    err = np.abs(r["offset"] - truth).max(axis=1)
+    res = {"pairs": b.n, "classes": len(groups), "pairs_per_s": round(b.n / el, 1), "ms_per_step": round(el * 1e3, 2),
+           "model_tflops": round(mflop * 1e6 / el / 1e12, 2), "fp64_rerun_pairs": b.rerun_count,
+           "schedules": {k: [v.mode, v.chunk, {int(n): r for n, r in v.radices.items()}] for k, v in tuner.cache.items()},
+           "median_err_vs_true_shift_px": round(float(np.median(err)), 4), "setup_tune_capture_s": round(tune_s, 1),
+           "generate_s": round(gen_s, 1)}
+    b.close()
+    return res
+
+
+def bench_serving(dev, ref_np, sec_np, n_jobs, rate, seed=5):
+    """Poisson arrivals of offset-field jobs over crops of the 8192x4096 warped pair: log-uniform
+    sizes 512x512..8192x4096, window/half search from the uneven grid, skip = window/2."""
+    import torch
+    from isce2_amd.autotune import Autotuner
+    from isce2_amd.serving import OffsetJob, OffsetServer
+    H, Wd = ref_np.shape
+    pool_ref = torch.from_numpy(ref_np).pin_memory()
+    pool_sec = torch.from_numpy(sec_np).pin_memory()
+    rng = np.random.default_rng(seed)
+
+    def make(i):
+        s = rng.uniform()
+        h, w = int(round(512 * 16 ** s)), int(round(512 * 8 ** s))
+        W = int(rng.choice([32, 64, 128, 256]))
+        hs = int(rng.choice([8, 16, 32]))
+        if W + 2 * hs + 2 * hs > min(h, w):
+            W = 64
+        r0 = int(rng.integers(0, H - h + 1))
+        return OffsetJob(ref=pool_ref[r0:r0 + h], sec=pool_sec[r0:r0 + h], height=h, width=w, window=W,
+                         half_search=hs, skip_down=W // 2, skip_across=W // 2, job_id=i)
+
+    srv = OffsetServer(device=dev.index or 0, hbm_budget=200e9, pitch=Wd,
+                       tuner=Autotuner(device=dev.index or 0, reps=2, sample=1024, radix_sample=64))
+    srv.start()
+    # warm-up: one small job per window class (engine contexts, plans, tuning)
+    warm = [srv.submit(OffsetJob(ref=pool_ref[:600], sec=pool_sec[:600], height=600, width=600, window=W,
+                                 half_search=hs, skip_down=W // 2, skip_across=W // 2))
+            for W in (32, 64, 128, 256) for hs in (8, 16, 32) if W + 4 * hs <= 600]
+    for f in warm:
+        f.result()
+    gaps = rng.exponential(1.0 / rate, size=n_jobs)
+    jobs = [make(i) for i in range(n_jobs)]
+    t_start = time.perf_counter()
+    futs, t_next = [], t_start
+    for j, gap in zip(jobs, gaps):
+        t_next += gap
+        dt = t_next - time.perf_counter()
+        if dt > 0:
+            time.sleep(dt)
+        futs.append(srv.submit(j))
+    res = [f.result() for f in futs]
+    t_end = time.perf_counter()
+    srv.stop()
+    lat = np.array([r["latency_s"] for r in res])
+    wins = sum(r["windows"] for r in res)
+    return {"jobs": n_jobs, "arrival_rate_per_s": rate, "windows": int(wins),
+            "windows_per_s": round(wins / (t_end - t_start), 1),
+            "p50_latency_ms": round(float(np.percentile(lat, 50)) * 1e3, 1),
+            "p99_latency_ms": round(float(np.percentile(lat, 99)) * 1e3, 1),
+            "max_latency_ms": round(float(lat.max()) * 1e3, 1),
+            "mean_queue_ms": round(float(np.mean([r["queue_s"] for r in res])) * 1e3, 2),
+            "makespan_s": round(t_end - t_start, 2), "batches": srv.stats["batches"],
+            "hbm_budget_gb": 200}
 
 
 if __name__ == "__main__":

@@ -69,6 +69,13 @@ typedef struct ampcor_sizes {
 #define AMPCOR_MODE_AUTO    0   /* fused when supported, else staged */
 #define AMPCOR_MODE_STAGED  1   /* one batched kernel per pipeline stage (any size) */
 #define AMPCOR_MODE_FUSED   2   /* per-window fused kernels (window <= 64 class) */
+/* Flag OR-ed into the mode: no host synchronization inside the call. Chip
+ * bounds are then checked on the device (a window whose chips leave their image
+ * gets NaN offset/SNR/covariance instead of AMPCOR_E_RANGE) and the precision-1
+ * FP64 re-run is skipped (pass margin outputs to ampcor_run_windows_debug and
+ * re-run from them). Required on a stream under HIP graph capture; a captured
+ * call must not allocate, so run the identical call once before capturing it. */
+#define AMPCOR_FLAG_NOSYNC  0x100
 
 /* Create / destroy an engine context on HIP device `device`.
  * Replaces cuAmpcorController's ctor/dtor and gpuDeviceInit
@@ -132,6 +139,11 @@ int ampcor_run_windows_debug(ampcor_ctx *ctx,
  * the windows checked and re-run since the last call. */
 int ampcor_set_precision(ampcor_ctx *ctx, int32_t mode, float tau_coarse, float tau_fine);
 int ampcor_precision_stats(ampcor_ctx *ctx, int64_t *checked, int64_t *rerun);
+
+/* Radix schedule of the staged pipeline's length-n FFTs (radices from
+ * {2,3,4,5,7,8,11,13,16,17}, product n, at most 14 passes), for autotuning.
+ * No counterpart in the reference (cuFFT plans internally). */
+int ampcor_set_fft_radices(ampcor_ctx *ctx, int32_t n, const int32_t *radices, int32_t count);
 
 /* Per-kernel timing with HIP events recorded on the launch stream around every
  * kernel of ampcor_run_windows. No counterpart in the reference (its timing is

@@ -1,0 +1,164 @@
+"""Batched chip pairs of uneven window size and search range.
+
+Each pair is a reference chip ``[W, W]`` and a search chip ``[W + 2hs]^2``
+(the search chip's origin is ``hs`` pixels up-left of the reference chip, as
+for one window of an offset field). Pairs are grouped into size classes
+``(W, hs)``; a class's chips are stacked into two device arenas (reference
+``[m*W, W]``, search ``[m*S, S]``) so the engine runs them as ordinary windows
+with start pixels ``(i*W, 0)`` and ``(i*S, 0)``. Each class has its own engine
+context (resident plans and buffers) and a schedule chosen by the autotuner
+(fused kernels or staged pipeline, chunk, FFT radix orders).
+
+One step = every class's launches on one stream in no-sync mode; with
+``graph=True`` the step is captured once into a HIP graph
+(``torch.cuda.CUDAGraph``) and replayed. Under precision 1 the windows whose
+argmax margins fall below the thresholds are re-run through the FP64 pipeline
+after the step (a separate engine context, so the graph's buffers are never
+touched by the re-run).
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from .autotune import Autotuner, Schedule
+from .engine import MODE_FUSED, MODE_STAGED, Engine, WindowParams
+
+TAU_DEFAULT = 2e-6
+
+
+class _Class:
+    def __init__(self, key, ref, sec, idx, device, deramp):
+        self.W, self.hs = key
+        self.S = self.W + 2 * self.hs
+        self.idx = np.asarray(idx)
+        m = ref.shape[0]
+        self.m = m
+        dev = torch.device("cuda", device)
+        self.ref = ref.reshape(m * self.W, self.W).contiguous()
+        self.sec = sec.reshape(m * self.S, self.S).contiguous()
+        i = torch.arange(m, dtype=torch.int32, device=dev)
+        z = torch.zeros(m, dtype=torch.int32, device=dev)
+        self.starts = [(i * self.W).contiguous(), z, (i * self.S).contiguous(), z.clone()]
+        self.params = WindowParams(window_h=self.W, window_w=self.W, hs_down=self.hs, hs_across=self.hs,
+                                   deramp=deramp)
+        self.eng = Engine(device)
+        self.sizes = self.eng.setup(self.params)
+        self.out = dict(offset=torch.empty((m, 2), dtype=torch.float32, device=dev),
+                        snr=torch.empty(m, dtype=torch.float32, device=dev),
+                        cov=torch.empty((m, 3), dtype=torch.float32, device=dev))
+        for k in ("maxloc", "start", "shift", "zoom_idx"):
+            self.out[k] = torch.empty((m, 2), dtype=torch.int32, device=dev)
+        for k in ("maxval", "margin_coarse", "margin_fine"):
+            self.out[k] = torch.full((m,), float("nan"), dtype=torch.float32, device=dev)
+        self.sched = Schedule(mode=MODE_FUSED if self.sizes["fused"] else MODE_STAGED,
+                              chunk=2048 if self.sizes["fused"] else 256)
+
+    def args(self):
+        return (self.ref, self.sec, *self.starts)
+
+    def launch(self, nosync):
+        self.eng.run_windows(*self.args(), mode=self.sched.mode, chunk=self.sched.chunk, debug=True,
+                             out=self.out, nosync=nosync)
+
+
+class ChipPairBatch:
+    """Uneven chip pairs run as one graph-captured step (see module docstring).
+
+    ``groups`` maps ``(W, hs)`` to ``{"ref": complex64[m, W, W], "sec":
+    complex64[m, S, S], "idx": int[m]}`` device tensors (``idx`` = position of
+    each pair in the caller's order); :func:`ChipPairBatch.from_pairs` builds it
+    from a list of ``{"W", "hs", "ref", "sec"}`` host chips.
+    """
+
+    def __init__(self, groups, device=0, deramp=1, precision=1, tau=TAU_DEFAULT, graph=True,
+                 tuner: Autotuner | None = None):
+        self.device = device
+        self.dev = torch.device("cuda", device)
+        self.precision = precision
+        self.tau = tau
+        self.use_graph = graph
+        self.classes = [_Class(k, g["ref"], g["sec"], g["idx"], device, deramp) for k, g in sorted(groups.items())]
+        self.n = sum(c.m for c in self.classes)
+        for c in self.classes:
+            c.eng.set_precision(2 if precision == 2 else 0)
+            if tuner is not None:
+                c.sched = tuner.tune(c.eng, c.params, c.args(), c.m)
+                c.sched.apply(c.eng)
+                c.eng.setup(c.params)
+            c.eng.set_precision(2 if precision == 2 else 0)
+        self._rerun_eng = Engine(device) if precision == 1 else None
+        self.graph = None
+        self.stream = torch.cuda.Stream(device=self.dev)
+        self.rerun_count = 0
+
+    @classmethod
+    def from_pairs(cls, pairs, device=0, **kw):
+        dev = torch.device("cuda", device)
+        groups = {}
+        for i, p in enumerate(pairs):
+            groups.setdefault((int(p["W"]), int(p["hs"])), []).append(i)
+        g2 = {}
+        for k, idx in groups.items():
+            ref = torch.from_numpy(np.stack([np.asarray(pairs[i]["ref"], np.complex64) for i in idx]))
+            sec = torch.from_numpy(np.stack([np.asarray(pairs[i]["sec"], np.complex64) for i in idx]))
+            g2[k] = dict(ref=ref.to(dev), sec=sec.to(dev), idx=np.asarray(idx))
+        return cls(g2, device=device, **kw)
+
+    def _step(self, nosync):
+        for c in self.classes:
+            c.launch(nosync)
+
+    def run(self):
+        """One pass over every pair; results stay on the device until :meth:`results`."""
+        torch.cuda.current_stream(self.dev).synchronize()
+        with torch.cuda.stream(self.stream):
+            if self.use_graph:
+                if self.graph is None:
+                    self._step(True)                     # allocations, plans, tables
+                    self.stream.synchronize()
+                    g = torch.cuda.CUDAGraph()
+                    with torch.cuda.graph(g, stream=self.stream):
+                        self._step(True)
+                    self.graph = g
+                self.graph.replay()
+            else:
+                self._step(True)
+            if self.precision == 1:
+                self._rerun_near_ties()
+        self.stream.synchronize()
+
+    def _rerun_near_ties(self):
+        self.rerun_count = 0
+        for c in self.classes:
+            mc = c.out["margin_coarse"].cpu().numpy()
+            mf = c.out["margin_fine"].cpu().numpy()
+            sel = np.nonzero(~(mc >= self.tau) | ~(mf >= self.tau))[0]
+            if len(sel) == 0:
+                continue
+            self.rerun_count += len(sel)
+            e = self._rerun_eng
+            e.setup(c.params)
+            e.set_precision(2)
+            t = torch.as_tensor(sel, dtype=torch.int64, device=self.dev)
+            st = [s.index_select(0, t).contiguous() for s in c.starts]
+            o = e.run_windows(c.ref, c.sec, *st, mode=MODE_STAGED, debug=True)
+            for k in ("offset", "snr", "cov", "maxloc", "start", "shift", "zoom_idx", "maxval"):
+                c.out[k].index_copy_(0, t, o[k])
+
+    def results(self):
+        """Host arrays in the caller's pair order: offset [n,2], snr [n], cov [n,3] (+ zoom_idx, maxloc)."""
+        res = dict(offset=np.empty((self.n, 2), np.float32), snr=np.empty(self.n, np.float32),
+                   cov=np.empty((self.n, 3), np.float32), maxloc=np.empty((self.n, 2), np.int32),
+                   zoom_idx=np.empty((self.n, 2), np.int32))
+        for c in self.classes:
+            for k in res:
+                res[k][c.idx] = c.out[k].cpu().numpy()
+        return res
+
+    def close(self):
+        self.graph = None
+        for c in self.classes:
+            c.eng.close()
+        if self._rerun_eng is not None:
+            self._rerun_eng.close()

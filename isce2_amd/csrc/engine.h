@@ -73,10 +73,20 @@ struct Engine {
     void prof_collect();
     ~Engine();
 
+    // set while a call runs on a stream under HIP graph capture: anything that
+    // would allocate device memory or copy synchronously throws instead
+    bool no_alloc = false;
+    void forbid_in_nosync(const char *what) const {
+        if (no_alloc)
+            throw Error(AMPCOR_E_ARG, std::string("a call under stream capture would ") + what +
+                                          "; run the same call once before capturing it");
+    }
     const FftPlan &plan(int n);
+    void set_fft_radices(int n, const int32_t *radices, int count);
     template <class U> U *buf(const std::string &name, size_t count) {
         auto &b = bufs[name];
-        if (!b) b.reset(new DevBuf());
+        if (!b) { forbid_in_nosync("allocate a device buffer"); b.reset(new DevBuf()); }
+        if (b->bytes < count * sizeof(U)) forbid_in_nosync("grow a device buffer");
         b->reserve(count * sizeof(U));
         return b->as<U>();
     }
@@ -94,6 +104,7 @@ struct Engine {
         int32_t chunk, mode;
         hipStream_t stream;
         float *margin_coarse = nullptr, *margin_fine = nullptr;   // debug: top-1 minus top-2
+        bool nosync = false;                                       // AMPCOR_FLAG_NOSYNC
     };
     // precision: 0 FP32 only; 1 FP32 + FP64 re-run of windows whose argmax
     // margins fall below the thresholds; 2 FP64 staged pipeline for every window

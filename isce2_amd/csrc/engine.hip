@@ -78,6 +78,7 @@ FftDesc make_schedule(int n) {
 const FftPlan &Engine::plan(int n) {
     auto it = plans.find(n);
     if (it != plans.end()) return *it->second;
+    forbid_in_nosync("build an FFT plan");
     std::unique_ptr<FftPlan> p(new FftPlan());
     p->n = n;
     p->desc32 = p->desc64 = make_schedule(n);
@@ -139,8 +140,29 @@ void Engine::setup(const ampcor_params &q) {
     configured = true;
 }
 
+// Replace the radix schedule of the length-n staged FFTs (autotuning hook).
+// Radices must be supported by the Stockham passes and multiply to n.
+void Engine::set_fft_radices(int n, const int32_t *radices, int count) {
+    if (n < 2 || count < 1 || count > 14 || !radices) throw Error(AMPCOR_E_ARG, "bad FFT schedule");
+    long prod = 1;
+    for (int i = 0; i < count; ++i) {
+        const int r = radices[i];
+        if (r != 2 && r != 3 && r != 4 && r != 5 && r != 7 && r != 8 && r != 11 && r != 13 && r != 16 && r != 17)
+            throw Error(AMPCOR_E_ARG, "unsupported FFT radix " + std::to_string(r));
+        prod *= r;
+    }
+    if (prod != n) throw Error(AMPCOR_E_ARG, "FFT radices do not multiply to " + std::to_string(n));
+    FftPlan &p = const_cast<FftPlan &>(plan(n));
+    for (FftDesc *d : {&p.desc32, &p.desc64}) {
+        d->direct = 0;
+        d->npass = count;
+        for (int i = 0; i < count; ++i) d->radix[i] = radices[i];
+    }
+}
+
 void Engine::ensure_sinc_table() {
     if (sinc_table.ptr) return;
+    forbid_in_nosync("build the sinc table");
     // cuSincOverSamplerR2R table (cuSincOverSampler.cu:21-80, constants cuSincOverSampler.h:27-57)
     const double beta = 0.75, relfiltlen = 6.0, pedestal = 0.0, PI = 3.14159265359;
     sinc_len = int(relfiltlen / beta + 0.5);
@@ -338,11 +360,37 @@ void Engine::run_staged(const RunArgs &a) {
         if (!b.margin_fine) b.margin_fine = buf<float>("f_mf", a.n);
     }
     run_staged_t<float>(b);
-    if (precision == 1) rerun_fp64(b, b.margin_coarse, b.margin_fine);
+    if (precision == 1 && !a.nosync) rerun_fp64(b, b.margin_coarse, b.margin_fine);
 }
 template void Engine::run_staged_t<double>(const RunArgs &a);
 
 namespace {
+// no-sync bounds check: copy the start arrays, replacing every window whose
+// chips leave their image by window (0, 0) and flagging it
+__global__ void k_sanitize_starts(const int32_t *rd, const int32_t *ra, const int32_t *sd, const int32_t *sa, int n,
+                                  int64_t ref_h, int64_t ref_w, int64_t sec_h, int64_t sec_w, int wh, int ww,
+                                  int sh, int sw, int32_t *out, int32_t *bad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t r0 = rd[i], a0 = ra[i], s0 = sd[i], t0 = sa[i];
+    const bool ok = r0 >= 0 && a0 >= 0 && r0 + wh <= ref_h && a0 + ww <= ref_w && s0 >= 0 && t0 >= 0 &&
+                    s0 + sh <= sec_h && t0 + sw <= sec_w && ref_h >= wh && ref_w >= ww && sec_h >= sh && sec_w >= sw;
+    out[i] = ok ? int32_t(r0) : 0;
+    out[n + i] = ok ? int32_t(a0) : 0;
+    out[2 * n + i] = ok ? int32_t(s0) : 0;
+    out[3 * n + i] = ok ? int32_t(t0) : 0;
+    bad[i] = ok ? 0 : 1;
+}
+
+__global__ void k_mark_bad(const int32_t *bad, int n, float *offset, float *snr, float *cov) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !bad[i]) return;
+    const float q = __builtin_nanf("");
+    offset[2 * i] = q; offset[2 * i + 1] = q;
+    snr[i] = q;
+    cov[3 * i] = q; cov[3 * i + 1] = q; cov[3 * i + 2] = q;
+}
+
 __global__ void k_take_i32(const int32_t *src, const int32_t *idx, int32_t *dst, int n, int width) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n * width) dst[i] = src[size_t(idx[i / width]) * width + i % width];
@@ -405,34 +453,63 @@ void Engine::rerun_fp64(const RunArgs &a, const float *mc, const float *mf) {
     HIP_OK(hipGetLastError());
 }
 
-void Engine::run(const RunArgs &a) {
+void Engine::run(const RunArgs &in) {
     if (!configured) throw Error(AMPCOR_E_ARG, "ampcor_setup has not been called");
-    if (a.n < 0) throw Error(AMPCOR_E_ARG, "negative window count");
-    if (a.n == 0) return;
-    if (!a.ref || !a.sec || !a.ref_d || !a.ref_a || !a.sec_d || !a.sec_a || !a.offset || !a.snr || !a.cov)
+    if (in.n < 0) throw Error(AMPCOR_E_ARG, "negative window count");
+    if (in.n == 0) return;
+    if (!in.ref || !in.sec || !in.ref_d || !in.ref_a || !in.sec_d || !in.sec_a || !in.offset || !in.snr || !in.cov)
         throw Error(AMPCOR_E_ARG, "null pointer argument");
-    if (a.ref_pitch < a.ref_w || a.sec_pitch < a.sec_w) throw Error(AMPCOR_E_ARG, "pitch smaller than width");
+    if (in.ref_pitch < in.ref_w || in.sec_pitch < in.sec_w) throw Error(AMPCOR_E_ARG, "pitch smaller than width");
+    if ((in.mode & ~(0xff | AMPCOR_FLAG_NOSYNC)) != 0 || (in.mode & 0xff) > AMPCOR_MODE_FUSED)
+        throw Error(AMPCOR_E_ARG, "unknown mode");
     HIP_OK(hipSetDevice(device));
-    // bounds: every chip must lie inside its image (device-side start arrays are
-    // checked here with one small copy; the caller's checkPixelInImageRange is stricter)
-    std::vector<int32_t> h(4 * size_t(a.n));
-    HIP_OK(hipMemcpyAsync(h.data(), a.ref_d, sizeof(int32_t) * a.n, hipMemcpyDeviceToHost, a.stream));
-    HIP_OK(hipMemcpyAsync(h.data() + a.n, a.ref_a, sizeof(int32_t) * a.n, hipMemcpyDeviceToHost, a.stream));
-    HIP_OK(hipMemcpyAsync(h.data() + 2 * size_t(a.n), a.sec_d, sizeof(int32_t) * a.n, hipMemcpyDeviceToHost, a.stream));
-    HIP_OK(hipMemcpyAsync(h.data() + 3 * size_t(a.n), a.sec_a, sizeof(int32_t) * a.n, hipMemcpyDeviceToHost, a.stream));
-    HIP_OK(hipStreamSynchronize(a.stream));
-    for (int i = 0; i < a.n; ++i) {
-        const int64_t rd = h[i], ra = h[a.n + i], sd = h[2 * size_t(a.n) + i], sa = h[3 * size_t(a.n) + i];
-        if (rd < 0 || ra < 0 || rd + p.window_h > a.ref_h || ra + p.window_w > a.ref_w ||
-            sd < 0 || sa < 0 || sd + sz.search_h > a.sec_h || sa + sz.search_w > a.sec_w)
-            throw Error(AMPCOR_E_RANGE, "window " + std::to_string(i) + " reads outside its image");
-    }
+    RunArgs a = in;
+    a.mode = in.mode & 0xff;
+    a.nosync = (in.mode & AMPCOR_FLAG_NOSYNC) != 0;
+    hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+    HIP_OK(hipStreamIsCapturing(a.stream, &cs));
+    const bool capturing = cs != hipStreamCaptureStatusNone;
+    if (capturing && !a.nosync) throw Error(AMPCOR_E_ARG, "a call under stream capture needs AMPCOR_FLAG_NOSYNC");
+    if (capturing && profiling) throw Error(AMPCOR_E_ARG, "disable ampcor_profile before capturing");
+    struct Restore { bool &f; bool v; ~Restore() { f = v; } } restore{no_alloc, no_alloc};
+    no_alloc = capturing;
+
     const bool fused = (a.mode == AMPCOR_MODE_FUSED) || (a.mode == AMPCOR_MODE_AUTO && sz.fused);
     if (a.mode == AMPCOR_MODE_FUSED && !sz.fused)
         throw Error(AMPCOR_E_UNSUPPORTED, "the fused kernels do not cover this configuration");
+    if (a.ref_h < p.window_h || a.ref_w < p.window_w || a.sec_h < sz.search_h || a.sec_w < sz.search_w)
+        throw Error(AMPCOR_E_RANGE, "image smaller than one chip");
+    int32_t *bad = nullptr;
+    if (a.nosync) {
+        // device-side bounds check: out-of-image windows read window (0, 0)
+        // instead and report NaN outputs
+        int32_t *st4 = buf<int32_t>("ns_starts", 4 * size_t(a.n));
+        bad = buf<int32_t>("ns_bad", a.n);
+        hipLaunchKernelGGL(k_sanitize_starts, dim3(idivup(a.n, 256)), dim3(256), 0, a.stream, a.ref_d, a.ref_a,
+                           a.sec_d, a.sec_a, a.n, a.ref_h, a.ref_w, a.sec_h, a.sec_w, p.window_h, p.window_w,
+                           sz.search_h, sz.search_w, st4, bad);
+        a.ref_d = st4; a.ref_a = st4 + a.n; a.sec_d = st4 + 2 * size_t(a.n); a.sec_a = st4 + 3 * size_t(a.n);
+    } else {
+        // bounds: every chip must lie inside its image (device-side start arrays are
+        // checked here with one small copy; the caller's checkPixelInImageRange is stricter)
+        std::vector<int32_t> h(4 * size_t(a.n));
+        HIP_OK(hipMemcpyAsync(h.data(), a.ref_d, sizeof(int32_t) * a.n, hipMemcpyDeviceToHost, a.stream));
+        HIP_OK(hipMemcpyAsync(h.data() + a.n, a.ref_a, sizeof(int32_t) * a.n, hipMemcpyDeviceToHost, a.stream));
+        HIP_OK(hipMemcpyAsync(h.data() + 2 * size_t(a.n), a.sec_d, sizeof(int32_t) * a.n, hipMemcpyDeviceToHost, a.stream));
+        HIP_OK(hipMemcpyAsync(h.data() + 3 * size_t(a.n), a.sec_a, sizeof(int32_t) * a.n, hipMemcpyDeviceToHost, a.stream));
+        HIP_OK(hipStreamSynchronize(a.stream));
+        for (int i = 0; i < a.n; ++i) {
+            const int64_t rd = h[i], ra = h[a.n + i], sd = h[2 * size_t(a.n) + i], sa = h[3 * size_t(a.n) + i];
+            if (rd < 0 || ra < 0 || rd + p.window_h > a.ref_h || ra + p.window_w > a.ref_w ||
+                sd < 0 || sa < 0 || sd + sz.search_h > a.sec_h || sa + sz.search_w > a.sec_w)
+                throw Error(AMPCOR_E_RANGE, "window " + std::to_string(i) + " reads outside its image");
+        }
+    }
     if (precision == 2) run_staged_t<double>(a);
     else if (fused) run_fused(a);
     else run_staged(a);
+    if (bad)
+        hipLaunchKernelGGL(k_mark_bad, dim3(idivup(a.n, 256)), dim3(256), 0, a.stream, bad, a.n, a.offset, a.snr, a.cov);
     HIP_OK(hipGetLastError());
 }
 
@@ -523,6 +600,14 @@ int ampcor_set_precision(ampcor_ctx *ctx, int32_t mode, float tau_coarse, float 
     if (tau_coarse >= 0.f) ctx->eng.tau_coarse = tau_coarse;
     if (tau_fine >= 0.f) ctx->eng.tau_fine = tau_fine;
     return AMPCOR_OK;
+}
+
+int ampcor_set_fft_radices(ampcor_ctx *ctx, int32_t n, const int32_t *radices, int32_t count) {
+    if (!ctx) return AMPCOR_E_ARG;
+    return guarded(ctx, [&] {
+        HIP_OK(hipSetDevice(ctx->eng.device));
+        ctx->eng.set_fft_radices(n, radices, count);
+    });
 }
 
 int ampcor_precision_stats(ampcor_ctx *ctx, int64_t *checked, int64_t *rerun) {

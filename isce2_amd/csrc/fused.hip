@@ -496,6 +496,7 @@ void Engine::run_fused(const RunArgs &a) {
         const std::string key = "f_hsh" + std::to_string(N);
         auto &b = bufs[key];
         if (!b) {
+            forbid_in_nosync("build a shift table");
             std::vector<float> h(2 * N);
             for (int k = 0; k < N; ++k) {
                 const int ks = k < N / 2 ? k : k - N;
@@ -557,7 +558,7 @@ void Engine::run_fused(const RunArgs &a) {
         }
         HIP_OK(hipGetLastError());
     }
-    if (precision == 1) rerun_fp64(a, mc, mf);
+    if (precision == 1 && !a.nosync) rerun_fp64(a, mc, mf);
 }
 
 }  // namespace ampcor

@@ -14,7 +14,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import MODE_AUTO, MODE_FUSED, MODE_STAGED, Params, Sizes, check
+from ._lib import FLAG_NOSYNC, MODE_AUTO, MODE_FUSED, MODE_STAGED, Params, Sizes, check
 
 __all__ = ["Engine", "WindowParams", "MODE_AUTO", "MODE_STAGED", "MODE_FUSED"]
 
@@ -55,6 +55,7 @@ class Engine:
         check(self.lib.ampcor_create(self.device, C.byref(h)), None)
         self._h = h
         self.params: WindowParams | None = None
+        self.precision = 1
         self.sizes: dict | None = None
 
     def close(self):
@@ -77,12 +78,15 @@ class Engine:
         return self.sizes
 
     def run_windows(self, ref, sec, ref_d, ref_a, sec_d, sec_a, *, mode=MODE_AUTO, chunk=0,
-                    debug=False, stream=None, out=None):
+                    debug=False, stream=None, out=None, nosync=False):
         """Run the per-window pipeline; every tensor argument lives on this engine's device.
 
         ``ref``/``sec`` are complex64 (or float32) 2-D tensors; start arrays are
         int32 1-D tensors of length n. Returns a dict of device tensors
         ``offset [n,2]``, ``snr [n]``, ``cov [n,3]`` (+ debug stage outputs).
+        ``out`` may supply any of these tensors. ``nosync`` makes the call free
+        of host synchronization (device-side bounds check with NaN outputs for
+        bad windows, no FP64 re-run), as needed under graph capture.
         """
         if self.params is None:
             raise RuntimeError("setup() must be called first")
@@ -98,20 +102,22 @@ class Engine:
             if img.dtype != want:
                 raise ValueError(f"image dtype {img.dtype} does not match image_complex={self.params.image_complex}")
         o = out or {}
-        offset = o.get("offset", torch.empty((n, 2), dtype=torch.float32, device=dev))
-        snr = o.get("snr", torch.empty((n,), dtype=torch.float32, device=dev))
-        cov = o.get("cov", torch.empty((n, 3), dtype=torch.float32, device=dev))
+        offset = o["offset"] if "offset" in o else torch.empty((n, 2), dtype=torch.float32, device=dev)
+        snr = o["snr"] if "snr" in o else torch.empty((n,), dtype=torch.float32, device=dev)
+        cov = o["cov"] if "cov" in o else torch.empty((n, 3), dtype=torch.float32, device=dev)
         res = dict(offset=offset, snr=snr, cov=cov)
         st = C.c_void_p(stream.cuda_stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream)
         common = [self._h, _ptr(ref), ref.shape[0], ref.shape[1], ref.stride(0),
                   _ptr(sec), sec.shape[0], sec.shape[1], sec.stride(0),
                   _ptr(ref_d), _ptr(ref_a), _ptr(sec_d), _ptr(sec_a), n,
                   _ptr(offset), _ptr(snr), _ptr(cov)]
+        if nosync:
+            mode = int(mode) | FLAG_NOSYNC
         if debug:
             for k in ("maxloc", "start", "shift", "zoom_idx"):
-                res[k] = torch.empty((n, 2), dtype=torch.int32, device=dev)
+                res[k] = o[k] if k in o else torch.empty((n, 2), dtype=torch.int32, device=dev)
             for k in ("maxval", "margin_coarse", "margin_fine"):
-                res[k] = torch.full((n,), float("nan"), dtype=torch.float32, device=dev)
+                res[k] = o[k] if k in o else torch.full((n,), float("nan"), dtype=torch.float32, device=dev)
             code = self.lib.ampcor_run_windows_debug(
                 *common, _ptr(res["maxloc"]), _ptr(res["start"]), _ptr(res["shift"]),
                 _ptr(res["zoom_idx"]), _ptr(res["maxval"]), _ptr(res["margin_coarse"]),
@@ -124,6 +130,12 @@ class Engine:
     def set_precision(self, mode: int, tau_coarse: float = -1.0, tau_fine: float = -1.0):
         """0: FP32 only; 1: FP32 + FP64 re-run of near-tie windows (default); 2: FP64 for all."""
         check(self.lib.ampcor_set_precision(self._h, int(mode), float(tau_coarse), float(tau_fine)), self._h)
+        self.precision = int(mode)
+
+    def set_fft_radices(self, n: int, radices):
+        """Radix schedule of the staged pipeline's length-n FFTs (autotuning)."""
+        arr = (C.c_int32 * len(radices))(*[int(r) for r in radices])
+        check(self.lib.ampcor_set_fft_radices(self._h, int(n), C.cast(arr, C.c_void_p), len(radices)), self._h)
 
     def precision_stats(self) -> tuple:
         """(windows checked, windows re-run in FP64) since the previous call."""

@@ -1,0 +1,335 @@
+"""Offset-tracking request server with continuous batching on one GPU.
+
+Requests (offset-field jobs: a reference/secondary image pair plus a window
+grid) arrive on a queue. One scheduler thread owns the device:
+
+* admission: the oldest waiting job is admitted when its image rows fit in the
+  device image arena and its footprint fits the HBM budget; its images go to
+  the arena on a copy stream (H2D overlaps the compute of admitted jobs);
+* batching: every iteration takes the oldest loaded job's window class
+  ``(window, half search, deramp)`` and fills one launch with pending windows
+  of every loaded job of that class, oldest first — windows of different
+  requests share a launch because all images live in one arena with a common
+  row pitch (a window's start pixel is its job's arena row plus its row);
+* completion: a job whose last window has run copies its outputs to the host,
+  frees its arena rows (so waiting jobs can be admitted) and resolves its
+  future with the outputs and its latency (arrival to host-visible result).
+
+Each window class keeps a resident engine context (FFT plans, buffers). The
+per-window results do not depend on which other windows share a launch, so a
+job's outputs are the same alone and in a mix.
+"""
+from __future__ import annotations
+
+import queue
+import threading
+import time
+from concurrent.futures import Future
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from .engine import MODE_AUTO, Engine, WindowParams
+
+
+@dataclass
+class OffsetJob:
+    """One offset-field request (PyCuAmpcor's window-grid attributes)."""
+    ref: object                     # complex64 [H, Wd] host array / CPU tensor (rows may be wider: ``width``)
+    sec: object
+    window: int = 64
+    half_search: int = 20
+    skip_down: int = 64
+    skip_across: int = 64
+    start_down: int | None = None   # reference start pixel of window (0, 0); default half_search
+    start_across: int | None = None
+    n_down: int | None = None       # window grid; default: as many as fit
+    n_across: int | None = None
+    gross_down: int = 0
+    gross_across: int = 0
+    deramp: int = 1
+    height: int | None = None       # valid image size when the arrays carry padding columns
+    width: int | None = None
+    job_id: int = -1
+    arrival: float = field(default_factory=time.perf_counter)
+
+
+def job_windows(job: OffsetJob):
+    """Start pixels of a job's windows (reference and secondary search chip)."""
+    H = job.height or int(job.ref.shape[0])
+    Wd = job.width or int(job.ref.shape[1])
+    W, hs = job.window, job.half_search
+    s0d = hs if job.start_down is None else job.start_down
+    s0a = hs if job.start_across is None else job.start_across
+    nd = job.n_down if job.n_down is not None else (H - s0d - W - hs - max(job.gross_down, 0)) // job.skip_down + 1
+    na = job.n_across if job.n_across is not None else (Wd - s0a - W - hs - max(job.gross_across, 0)) // job.skip_across + 1
+    if nd <= 0 or na <= 0:
+        raise ValueError(f"job {job.job_id}: no window fits a {H}x{Wd} image")
+    r, c = np.meshgrid(np.arange(nd), np.arange(na), indexing="ij")
+    rd = (s0d + r * job.skip_down).ravel()
+    ra = (s0a + c * job.skip_across).ravel()
+    sd = rd + job.gross_down - hs
+    sa = ra + job.gross_across - hs
+    S = W + 2 * hs
+    if rd.min() < 0 or ra.min() < 0 or sd.min() < 0 or sa.min() < 0 or rd.max() + W > H or ra.max() + W > Wd \
+            or sd.max() + S > H or sa.max() + S > Wd:
+        raise IndexError(f"job {job.job_id}: a window reads outside its {H}x{Wd} image")
+    return nd, na, rd.astype(np.int64), ra.astype(np.int64), sd.astype(np.int64), sa.astype(np.int64)
+
+
+class RowAllocator:
+    """First-fit allocator of row ranges ``[a, a + n)`` in ``[0, rows)`` (coalescing frees)."""
+
+    def __init__(self, rows):
+        self.rows = rows
+        self.free = [(0, rows)]
+
+    def alloc(self, n):
+        for i, (a, b) in enumerate(self.free):
+            if b - a >= n:
+                if a + n == b:
+                    self.free.pop(i)
+                else:
+                    self.free[i] = (a + n, b)
+                return a
+        return None
+
+    def release(self, a, n):
+        self.free.append((a, a + n))
+        self.free.sort()
+        merged = []
+        for x, y in self.free:
+            if merged and merged[-1][1] > x:
+                raise ValueError("double free of arena rows")
+            if merged and merged[-1][1] == x:
+                merged[-1] = (merged[-1][0], y)
+            else:
+                merged.append((x, y))
+        self.free = merged
+
+    def free_rows(self):
+        return sum(b - a for a, b in self.free)
+
+
+class _RowArena(RowAllocator):
+    """A [rows, pitch] pair of device images whose rows are handed out to jobs."""
+
+    def __init__(self, rows, pitch, device):
+        super().__init__(rows)
+        self.pitch = pitch
+        self.ref = torch.empty((rows, pitch), dtype=torch.complex64, device=device)
+        self.sec = torch.empty((rows, pitch), dtype=torch.complex64, device=device)
+
+
+class _Active:
+    def __init__(self, job, fut, row0, starts, nd, na, device):
+        self.job, self.fut, self.row0 = job, fut, row0
+        self.nd, self.na = nd, na
+        self.n = nd * na
+        rd, ra, sd, sa = starts
+        dev = torch.device("cuda", device)
+        self.starts = torch.as_tensor(np.stack([rd + row0, ra, sd + row0, sa]).astype(np.int32), device=dev)
+        self.out = dict(offset=torch.empty((self.n, 2), dtype=torch.float32, device=dev),
+                        snr=torch.empty(self.n, dtype=torch.float32, device=dev),
+                        cov=torch.empty((self.n, 3), dtype=torch.float32, device=dev))
+        self.cursor = 0
+        self.loaded = None          # CUDA event: images are in the arena
+        self.key = (job.window, job.half_search, job.deramp)
+        self.admitted_at = time.perf_counter()
+
+
+class OffsetServer:
+    """Continuous-batching offset-tracking server (see module docstring)."""
+
+    def __init__(self, device=0, hbm_budget=200e9, arena_rows=None, pitch=4096, max_batch=16384, precision=1,
+                 tuner=None):
+        self.device = device
+        self.dev = torch.device("cuda", device)
+        self.pitch = pitch
+        self.max_batch = max_batch
+        self.precision = precision
+        self.budget = float(hbm_budget)
+        if arena_rows is None:
+            arena_rows = int(self.budget // (2 * pitch * 8))
+        self.arena = _RowArena(arena_rows, pitch, self.dev)
+        self.used = 0.0
+        self.copy_stream = torch.cuda.Stream(device=self.dev)
+        self.compute_stream = torch.cuda.Stream(device=self.dev)
+        self.engines = {}
+        self.inbox = queue.Queue()
+        self.waiting = []
+        self.active = []
+        self.stats = dict(batches=0, windows=0, jobs=0)
+        self._stop = threading.Event()
+        self._thread = None
+        self._next_id = 0
+        self.tuner = tuner
+        self.schedules = {}
+
+    # -- client side ------------------------------------------------------
+    def submit(self, job: OffsetJob) -> Future:
+        fut = Future()
+        job.arrival = time.perf_counter()
+        if job.job_id < 0:
+            job.job_id = self._next_id
+            self._next_id += 1
+        self.inbox.put((job, fut))
+        return fut
+
+    def start(self):
+        self._thread = threading.Thread(target=self._loop, name="offset-server", daemon=True)
+        self._thread.start()
+        return self
+
+    def stop(self):
+        self._stop.set()
+        if self._thread is not None:
+            self._thread.join()
+        for e in self.engines.values():
+            e.close()
+        self.engines.clear()
+
+    def __enter__(self):
+        return self.start()
+
+    def __exit__(self, *exc):
+        self.stop()
+
+    # -- scheduler --------------------------------------------------------
+    def _footprint(self, job, n):
+        rows = int(job.height or job.ref.shape[0])
+        return 2.0 * rows * self.pitch * 8 + n * (24 + 16) + 64e6
+
+    def _engine(self, key):
+        if key not in self.engines:
+            W, hs, deramp = key
+            e = Engine(self.device)
+            e.setup(WindowParams(window_h=W, window_w=W, hs_down=hs, hs_across=hs, deramp=deramp))
+            e.set_precision(self.precision)
+            self.engines[key] = e
+        return self.engines[key]
+
+    def _admit(self):
+        while True:
+            try:
+                self.waiting.append(self.inbox.get_nowait())
+            except queue.Empty:
+                break
+        while self.waiting:
+            job, fut = self.waiting[0]
+            try:
+                nd, na, *starts = job_windows(job)
+                rows = int(job.height or job.ref.shape[0])
+                width = int(job.width or job.ref.shape[1])
+                if width > self.pitch or job.window + 2 * job.half_search > self.pitch:
+                    raise ValueError(f"job {job.job_id}: width {width} exceeds the arena pitch {self.pitch}")
+            except Exception as exc:   # bad request: fail it, keep serving
+                self.waiting.pop(0)
+                fut.set_exception(exc)
+                continue
+            need = self._footprint(job, nd * na)
+            if self.used + need > self.budget:
+                return
+            row0 = self.arena.alloc(rows)
+            if row0 is None:
+                return
+            self.waiting.pop(0)
+            self.used += need
+            a = _Active(job, fut, row0, starts, nd, na, self.device)
+            a.footprint = need
+            with torch.cuda.stream(self.copy_stream):
+                for src, dst in ((job.ref, self.arena.ref), (job.sec, self.arena.sec)):
+                    t = src if isinstance(src, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(src))
+                    if t.shape[1] == self.pitch:
+                        dst[row0:row0 + rows].copy_(t[:rows], non_blocking=True)
+                    else:
+                        dst[row0:row0 + rows, :t.shape[1]].copy_(t[:rows], non_blocking=True)
+                ev = torch.cuda.Event()
+                ev.record(self.copy_stream)
+            a.loaded = ev
+            self.active.append(a)
+
+    def _pick_batch(self):
+        ready = [a for a in self.active if a.cursor < a.n and a.loaded.query()]
+        if not ready:
+            return None, []
+        key = ready[0].key
+        parts, total = [], 0
+        for a in ready:
+            if a.key != key:
+                continue
+            take = min(a.n - a.cursor, self.max_batch - total)
+            if take <= 0:
+                break
+            parts.append((a, a.cursor, take))
+            total += take
+        return key, parts
+
+    def _run_batch(self, key, parts):
+        eng = self._engine(key)
+        for a, _, _ in parts:
+            self.compute_stream.wait_event(a.loaded)
+        with torch.cuda.stream(self.compute_stream):
+            st = torch.cat([a.starts[:, c0:c0 + k] for a, c0, k in parts], dim=1)
+            rd, ra, sd, sa = (st[i].contiguous() for i in range(4))
+            if key not in self.schedules:
+                if self.tuner is not None:    # first batch of a class: tune on its windows
+                    sch = self.tuner.tune(eng, eng.params, (self.arena.ref, self.arena.sec, rd, ra, sd, sa), len(rd))
+                    sch.apply(eng)
+                    self.schedules[key] = (sch.mode, sch.chunk)
+                else:
+                    self.schedules[key] = (MODE_AUTO, 0)
+            mode, chunk = self.schedules[key]
+            out = eng.run_windows(self.arena.ref, self.arena.sec, rd, ra, sd, sa, mode=mode, chunk=chunk)
+            pos = 0
+            for a, c0, k in parts:
+                for name in ("offset", "snr", "cov"):
+                    a.out[name][c0:c0 + k].copy_(out[name][pos:pos + k])
+                pos += k
+                a.cursor = c0 + k
+        self.stats["batches"] += 1
+        self.stats["windows"] += pos
+
+    def _finish(self):
+        done = [a for a in self.active if a.cursor >= a.n]
+        if not done:
+            return
+        self.compute_stream.synchronize()
+        for a in done:
+            res = {k: v.cpu().numpy() for k, v in a.out.items()}
+            t = time.perf_counter()
+            res["offset"] = res["offset"].reshape(a.nd, a.na, 2)
+            res["snr"] = res["snr"].reshape(a.nd, a.na)
+            res["cov"] = res["cov"].reshape(a.nd, a.na, 3)
+            res.update(job_id=a.job.job_id, latency_s=t - a.job.arrival, queue_s=a.admitted_at - a.job.arrival,
+                       windows=a.n)
+            self.active.remove(a)
+            self.arena.release(a.row0, int(a.job.height or a.job.ref.shape[0]))
+            self.used -= a.footprint
+            self.stats["jobs"] += 1
+            a.fut.set_result(res)
+
+    def _loop(self):
+        torch.cuda.set_device(self.dev)
+        while not self._stop.is_set() or self.active or self.waiting or not self.inbox.empty():
+            try:
+                self._admit()
+                key, parts = self._pick_batch()
+                if parts:
+                    self._run_batch(key, parts)
+                    self._finish()
+                elif self.active or self.waiting:
+                    time.sleep(0.0002)
+                else:
+                    try:
+                        item = self.inbox.get(timeout=0.01)
+                        self.waiting.append(item)
+                    except queue.Empty:
+                        pass
+            except Exception as exc:    # fail every in-flight job rather than hang its client
+                for a in self.active:
+                    if not a.fut.done():
+                        a.fut.set_exception(exc)
+                self.active.clear()
+                raise

@@ -163,6 +163,51 @@ def chip_pairs_uneven(n=8192, seed=0, windows=UNEVEN_WINDOWS, ranges=UNEVEN_RANG
     return pairs
 
 
+def chip_pairs_uneven_grouped(n=8192, seed=0, windows=UNEVEN_WINDOWS, ranges=UNEVEN_RANGES, device=None,
+                              block=128):
+    """The uneven-batch workload generated per size class, vectorized.
+
+    Window ``W`` uniform over ``windows``, half search ``hs`` uniform over
+    ``ranges``, true offsets uniform in +-0.8*hs per axis (continuous). Each
+    pair comes from its own speckle field (smoothed by the 3x3 box) and its
+    secondary is that field shifted by an FFT phase ramp. Returns
+    ``{(W, hs): {"idx": int64[m], "ref": complex64[m, W, W], "sec":
+    complex64[m, S, S], "shift": float64[m, 2]}}`` with tensors on ``device``.
+    """
+    dev = _dev(device)
+    rng = np.random.default_rng(seed)
+    Ws = rng.choice(np.asarray(windows), size=n)
+    hss = rng.choice(np.asarray(ranges), size=n)
+    shifts = rng.uniform(-0.8, 0.8, size=(n, 2)) * hss[:, None]
+    out = {}
+    for W in windows:
+        for hs in ranges:
+            idx = np.nonzero((Ws == W) & (hss == hs))[0]
+            if len(idx) == 0:
+                continue
+            S = W + 2 * hs
+            F = S + 32
+            g = torch.Generator(device="cpu").manual_seed(int(seed) * 7919 + int(W) * 131 + int(hs))
+            fy = torch.fft.fftfreq(F, device=dev, dtype=torch.float64)
+            box = (1 + 2 * torch.cos(2 * math.pi * fy)) / 3.0
+            refs, secs = [], []
+            for b0 in range(0, len(idx), block):
+                m = min(block, len(idx) - b0)
+                re = torch.randn(m, F, F, generator=g, dtype=torch.float64)
+                im = torch.randn(m, F, F, generator=g, dtype=torch.float64)
+                z = torch.complex(re, im).to(dev) * math.sqrt(0.5)
+                spec = torch.fft.fft2(z) * (box[:, None] * box[None, :])
+                field = torch.fft.ifft2(spec)
+                sh = torch.as_tensor(shifts[idx[b0:b0 + m]], device=dev)
+                ramp = torch.exp(-2j * math.pi * (fy[None, :, None] * sh[:, 0, None, None] +
+                                                  fy[None, None, :] * sh[:, 1, None, None]))
+                moved = torch.fft.ifft2(spec * ramp)
+                refs.append(field[:, 16 + hs:16 + hs + W, 16 + hs:16 + hs + W].to(torch.complex64))
+                secs.append(moved[:, 16:16 + S, 16:16 + S].to(torch.complex64))
+            out[(int(W), int(hs))] = dict(idx=idx, ref=torch.cat(refs), sec=torch.cat(secs), shift=shifts[idx])
+    return out
+
+
 def window_grid(h, w, window, hs, skip, margin=0):
     """Window counts and first start pixel as cuDenseOffsets computes them
     (contrib/PyCuAmpcor/examples/cuDenseOffsets.py:221-241)."""

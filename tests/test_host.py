@@ -115,3 +115,42 @@ def test_raster_readers_roundtrip(tmp_path):
     np.testing.assert_array_equal(raster.open_raster(str(fb) + ".vrt"), b[:, :, 0])
     with pytest.raises(raster.RasterError):
         raster.open_raster(str(tmp_path / "missing.slc"))
+
+
+def test_row_allocator_first_fit_and_coalescing():
+    from isce2_amd.serving import RowAllocator
+    a = RowAllocator(100)
+    x = a.alloc(40)
+    y = a.alloc(30)
+    z = a.alloc(30)
+    assert (x, y, z) == (0, 40, 70) and a.alloc(1) is None
+    a.release(y, 30)
+    assert a.alloc(31) is None and a.alloc(30) == 40
+    a.release(x, 40)
+    a.release(40, 30)
+    assert a.free == [(0, 70)] and a.free_rows() == 70
+    with pytest.raises(ValueError):
+        a.release(10, 5)
+
+
+def test_job_windows_grid_and_range_check():
+    from isce2_amd.serving import OffsetJob, job_windows
+    img = np.zeros((300, 260), np.complex64)
+    j = OffsetJob(ref=img, sec=img, window=64, half_search=16, skip_down=32, skip_across=32)
+    nd, na, rd, ra, sd, sa = job_windows(j)
+    assert nd == (300 - 16 - 64 - 16) // 32 + 1 and na == (260 - 16 - 64 - 16) // 32 + 1
+    assert rd[0] == 16 and ra[0] == 16 and sd[0] == 0 and sa[0] == 0
+    assert (sd + 96).max() <= 300 and (sa + 96).max() <= 260
+    j2 = OffsetJob(ref=img, sec=img, window=64, half_search=16, n_down=50, n_across=1)
+    with pytest.raises(IndexError):
+        job_windows(j2)
+
+
+def test_fft_factorizations_multiply_to_length():
+    from isce2_amd.autotune import factorizations
+    for n in (16, 40, 48, 64, 72, 80, 96, 104, 136, 144, 160, 192, 256, 264, 272, 288, 320, 528):
+        orders = factorizations(n)
+        assert orders, n
+        for r in orders:
+            assert int(np.prod(r)) == n and set(r) <= {2, 3, 4, 5, 7, 8, 11, 13, 16, 17}
+    assert factorizations(2 * 19) == []
