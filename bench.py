@@ -46,6 +46,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--serve-jobs", type=int, default=60)
     ap.add_argument("--serve-rate", type=float, default=20.0, help="Poisson arrival rate, jobs/s")
+    ap.add_argument("--no-stress", action="store_true", help="skip the precision-stress leg")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 1 GiB device copy (scale reference for HBM byte counters)")
     args = ap.parse_args()
@@ -106,6 +107,22 @@ def main():
     wps = n * args.steps / elapsed
     ms_step = 1000 * elapsed / args.steps
     gpu_out = {k: v.cpu().numpy() for k, v in out.items()}
+
+    # host-resident variant: pinned host images -> HBM -> field -> host outputs, per pass
+    h_ref = torch.from_numpy(ref_np).pin_memory()
+    h_sec = torch.from_numpy(sec_np).pin_memory()
+    d_ref, d_sec = torch.empty_like(ref), torch.empty_like(sec)
+    torch.cuda.synchronize(dev)
+    t1 = time.perf_counter()
+    for _ in range(2):
+        d_ref.copy_(h_ref, non_blocking=True)
+        d_sec.copy_(h_sec, non_blocking=True)
+        o_h = eng.run_windows(d_ref, d_sec, *starts, mode=mode)
+        host_out = {k: v.cpu() for k, v in o_h.items()}
+    t_host = (time.perf_counter() - t1) / 2
+    pcie = {"windows_per_s": round(n / t_host, 1), "ms_per_field": round(1e3 * t_host, 2),
+            "note": "pinned host images copied to HBM, whole field, outputs copied back; not the headline value"}
+    del d_ref, d_sec, h_ref, h_sec, host_out
 
     # roofline of the dominant kernel (largest total time)
     roof = None
@@ -187,6 +204,7 @@ def main():
 
     uneven = None if args.no_uneven else bench_uneven(dev, args.uneven_pairs, max(1, args.steps // 2))
     serving = None if args.no_serve else bench_serving(dev, ref_np, sec_np, args.serve_jobs, args.serve_rate)
+    stress = None if args.no_stress else bench_precision_stress(dev)
 
     line = {
         "metric": "windows/sec @64x64 ±20px, offset err<0.01px vs FP64; serve p50/p99 latency",
@@ -201,7 +219,7 @@ def main():
                       "windows_rerun_fp64_per_step": rerun // max(1, args.steps),
                       "rerun_fraction": round(rerun / max(1, checked), 5)},
         "roofline": roof, "cpu_baseline": cpu, "kernels": kernels, "plain_staged": plain, "accuracy": accuracy,
-        "uneven_batch": uneven, "serving": serving,
+        "host_resident_inputs": pcie, "uneven_batch": uneven, "serving": serving, "precision_stress": stress,
         "scene_gen_s": round(gen_s, 1),
     }
     print(json.dumps(line), flush=True)
@@ -303,6 +321,49 @@ def bench_serving(dev, ref_np, sec_np, n_jobs, rate, seed=5):
             "mean_queue_ms": round(float(np.mean([r["queue_s"] for r in res])) * 1e3, 2),
             "makespan_s": round(t_end - t_start, 2), "batches": srv.stats["batches"],
             "hbm_budget_gb": 200}
+
+
+def bench_precision_stress(dev):
+    """4096x4096 precision-stress pair (log-normal sigma 2 amplitude, 200 +60 dB points, 20% of
+    windows at amplitude 1e-6, per-128x128-tile shifts up to +-4 px), 32x32 windows centred in
+    the tiles: error of every output against the FP64 oracle, FP32-only and default precision."""
+    import torch
+    from isce2_amd.engine import Engine, WindowParams
+    from isce2_amd.synth import precision_stress_scene
+    from oracle import ampcor_oracle as O
+    ref, sec, shifts, tiny = precision_stress_scene(4096, seed=0, device=dev)
+    ty, tx = np.meshgrid(np.arange(32), np.arange(32), indexing="ij")
+    rd = (ty * 128 + 32).ravel()
+    ra = (tx * 128 + 32).ravel()
+    starts = (rd, ra, rd - 20, ra - 20)
+    t0 = time.time()
+    o = O.run_windows(ref, sec, *starts, O.AmpcorConfig())
+    oracle_s = time.time() - t0
+    e = Engine(dev.index or 0)
+    e.setup(WindowParams())
+    st = [torch.as_tensor(s_.astype(np.int32), device=dev) for s_ in starts]
+    rt, stt = torch.from_numpy(ref).to(dev), torch.from_numpy(sec).to(dev)
+    out = {"windows": len(rd), "tiny_windows": int(tiny.sum()), "oracle_s": round(oracle_s, 1)}
+    for prec, name in ((0, "fp32_only"), (1, "default")):
+        e.set_precision(prec)
+        g = e.run_windows(rt, stt, *st, debug=True)
+        torch.cuda.synchronize(dev)
+        _, rerun = e.precision_stats()
+        g = {k: v.cpu().numpy() for k, v in g.items()}
+        err = np.abs(g["offset"] - o["offset"]).max(axis=1)
+        snr_rel = np.abs(g["snr"] - o["snr"]) / np.abs(o["snr"])
+        sent_o = np.all(o["cov"] == [99, 99, 0], axis=1)
+        sent_g = np.all(g["cov"] == [99, 99, 0], axis=1)
+        ok = ~sent_o & ~sent_g
+        cov_rel = np.abs(g["cov"] - o["cov"]).max(axis=1) / np.abs(o["cov"]).max(axis=1)
+        out[name] = {"offset_max_err_px": float(err.max()), "windows_over_0.01px": int((err > 0.01).sum()),
+                     "peak_mismatches": int(np.any(g["maxloc"] != o["maxloc"], axis=1).sum()
+                                            + np.any(g["zoom_idx"] != o["zoom_idx"], axis=1).sum()),
+                     "snr_max_rel_err": float(snr_rel.max()),
+                     "cov_max_rel_err": float(cov_rel[ok].max()) if ok.any() else 0.0,
+                     "cov_sentinel_mismatches": int((sent_o != sent_g).sum()), "fp64_rerun_windows": int(rerun)}
+    e.close()
+    return out
 
 
 if __name__ == "__main__":

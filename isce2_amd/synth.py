@@ -208,6 +208,52 @@ def chip_pairs_uneven_grouped(n=8192, seed=0, windows=UNEVEN_WINDOWS, ranges=UNE
     return out
 
 
+def precision_stress_scene(size=4096, tile=128, sigma=2.0, n_points=200, point_db=60.0, tiny_frac=0.2,
+                           tiny_amp=1e-6, max_shift=4.0, seed=0, device=None):
+    """Precision-stress pair: heavy-tailed amplitude, bright points, near-zero windows.
+
+    Reference = unit speckle (3x3 box) times a per-pixel log-normal amplitude
+    ``exp(sigma * N(0, 1))``, plus ``n_points`` band-limited point targets of
+    amplitude ``10**(point_db/20)`` x the mean amplitude at random sub-pixel
+    positions. The image is cut into ``tile x tile`` tiles; each tile's
+    secondary is the tile shifted by its own ``U(+-max_shift)`` offset through
+    an FFT phase ramp (circular within the tile), and a ``tiny_frac`` share of
+    tiles (reference and secondary) is scaled to ``tiny_amp``. Returns
+    ``(ref, sec, shifts [nt, nt, 2], tiny [nt, nt] bool)`` as numpy arrays.
+    """
+    dev = _dev(device)
+    rng = np.random.default_rng(seed)
+    nt = size // tile
+    z = speckle(size, size, seed, dev)
+    g = torch.Generator(device="cpu").manual_seed(int(seed) + 17)
+    amp = torch.exp(sigma * torch.randn(size, size, generator=g, dtype=torch.float64)).to(dev)
+    z = z * amp
+    mean_amp = float(z.abs().mean())
+    t = z.reshape(nt, tile, nt, tile).permute(0, 2, 1, 3)                  # [nt, nt, tile, tile]
+    spec = torch.fft.fft2(t)
+    f = torch.fft.fftfreq(tile, device=dev, dtype=torch.float64)
+    py = rng.uniform(0, size, n_points)
+    px = rng.uniform(0, size, n_points)
+    a_pt = mean_amp * 10 ** (point_db / 20.0)
+    for y, x in zip(py, px):
+        ty, tx = int(y // tile), int(x // tile)
+        yl, xl = y - ty * tile, x - tx * tile
+        # band-limited impulse of peak amplitude a_pt at (yl, xl): spectrum a_pt * e^{-2 pi i f.r}
+        spec[ty, tx] += a_pt * torch.exp(-2j * math.pi * (f[:, None] * yl + f[None, :] * xl))
+    shifts = rng.uniform(-max_shift, max_shift, size=(nt, nt, 2))
+    sh = torch.as_tensor(shifts, device=dev)
+    ramp = torch.exp(-2j * math.pi * (f[None, None, :, None] * sh[..., 0, None, None] +
+                                      f[None, None, None, :] * sh[..., 1, None, None]))
+    ref_t = torch.fft.ifft2(spec)
+    sec_t = torch.fft.ifft2(spec * ramp)
+    tiny = rng.uniform(size=(nt, nt)) < tiny_frac
+    scale = torch.where(torch.as_tensor(tiny, device=dev), tiny_amp, 1.0)[..., None, None]
+    ref_t = ref_t * scale
+    sec_t = sec_t * scale
+    back = lambda u: u.permute(0, 2, 1, 3).reshape(size, size).to(torch.complex64).cpu().numpy()  # noqa: E731
+    return back(ref_t), back(sec_t), shifts, tiny
+
+
 def window_grid(h, w, window, hs, skip, margin=0):
     """Window counts and first start pixel as cuDenseOffsets computes them
     (contrib/PyCuAmpcor/examples/cuDenseOffsets.py:221-241)."""
