@@ -37,6 +37,8 @@ struct FftPlan {
     int n = 0;
     FftDesc desc32{}, desc64{};
     DevBuf tw32, tw64;
+    const void *tw(float) const { return tw32.ptr; }
+    const void *tw(double) const { return tw64.ptr; }
 };
 
 struct Sizes {

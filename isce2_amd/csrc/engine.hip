@@ -253,9 +253,11 @@ void Engine::run_staged_t(const RunArgs &a) {
     cx<T> *c_sec_os = buf<cx<T>>(tn + "c_sec_os", size_t(nb) * L2h * L2w);
     T *r_sec_os = buf<T>(tn + "r_sec_os", size_t(nb) * L2h * L2w);
     T *corr_os = buf<T>(tn + "corr_os", size_t(nb) * sz.os_corr_h * sz.os_corr_w);
-    cx<T> *c_surf_in = buf<cx<T>>(tn + "c_surf_in", size_t(nb) * Z * Z);
-    cx<T> *c_surf = buf<cx<T>>(tn + "c_surf", size_t(nb) * So * So);
-    T *r_surf = buf<T>(tn + "r_surf", size_t(nb) * So * So);
+    // the 16-sparse surface kernel needs no 256x256 buffers
+    const bool fast_surface = !p.surf_method && Z == 16 && So == 256;
+    cx<T> *c_surf_in = fast_surface ? nullptr : buf<cx<T>>(tn + "c_surf_in", size_t(nb) * Z * Z);
+    cx<T> *c_surf = fast_surface || p.surf_method ? nullptr : buf<cx<T>>(tn + "c_surf", size_t(nb) * So * So);
+    T *r_surf = fast_surface ? nullptr : buf<T>(tn + "r_surf", size_t(nb) * So * So);
     T *r_adj = buf<T>(tn + "r_adj", size_t(nb) * Z * Z);
     if (p.surf_method) ensure_sinc_table();
 
@@ -322,26 +324,34 @@ void Engine::run_staged_t(const RunArgs &a) {
         hipLaunchKernelGGL(k_sat<T>, dim3(n), dim3(256), 0, st, r_sec_os, L2h, L2w, sat1, sat2);
         hipLaunchKernelGGL(k_normalize<T>, dim3(n), dim3(256), 0, st, corr_os, sz.os_corr_h, sz.os_corr_w,
                            r_ref_os, sz.os_win_h, sz.os_win_w, sat1, sat2, L2h, L2w);
-        // trim + surface oversampling (cuAmpcorChunk.cpp:169-186)
-        if (p.surf_method) {
-            hipLaunchKernelGGL(k_trim_real<T>, grid2(Z * Z, n), dim3(256), 0, st, corr_os, sz.os_corr_h,
-                               sz.os_corr_w, r_adj, Z);
-            hipLaunchKernelGGL(k_sinc<T>, dim3(1, n), dim3(256), 0, st, r_adj, Z, r_surf, So, shift,
-                               p.surf_ovs * p.raw_ovs, sinc_table.as<float>(), p.surf_ovs, sinc_dec, sinc_len);
+        // trim + surface oversampling + argmax + sub-pixel offset (cuAmpcorChunk.cpp:169-207)
+        float *mf = a.margin_fine ? a.margin_fine + w0 : nullptr;
+        const float ratio = 1.0f / float(p.surf_ovs * p.raw_ovs);
+        if (fast_surface) {
+            // 16-sparse spectrum: one kernel instead of trim, FFT, pad, 256x256 FFT, real part, argmax
+            const size_t lds = sizeof(cx<T>) * (2 * 256 + 16 * 256);
+            hipLaunchKernelGGL(k_surface<T>, dim3(n), dim3(256), lds, st, corr_os, sz.os_corr_h, sz.os_corr_w,
+                               start, p.hs_down, p.hs_across, ratio, static_cast<const cx<T> *>(plan(256).tw(T())),
+                               zidx, mf, a.offset + 2 * size_t(w0));
         } else {
-            hipLaunchKernelGGL(k_trim_complex<T>, grid2(Z * Z, n), dim3(256), 0, st, corr_os, sz.os_corr_h,
-                               sz.os_corr_w, c_surf_in, Z);
-            fft2<T, +1>(*this, c_surf_in, n, Z, Z, st);
-            hipLaunchKernelGGL(k_quadpad<T>, grid2(So * So, n), dim3(256), 0, st, c_surf_in, Z, Z, c_surf, So, So);
-            fft2<T, -1>(*this, c_surf, n, So, So, st);
-            hipLaunchKernelGGL(k_real<T>, dim3(idivup(n * So * So, 256)), dim3(256), 0, st, c_surf, r_surf,
-                               int64_t(n) * So * So);
+            if (p.surf_method) {
+                hipLaunchKernelGGL(k_trim_real<T>, grid2(Z * Z, n), dim3(256), 0, st, corr_os, sz.os_corr_h,
+                                   sz.os_corr_w, r_adj, Z);
+                hipLaunchKernelGGL(k_sinc<T>, dim3(1, n), dim3(256), 0, st, r_adj, Z, r_surf, So, shift,
+                                   p.surf_ovs * p.raw_ovs, sinc_table.as<float>(), p.surf_ovs, sinc_dec, sinc_len);
+            } else {
+                hipLaunchKernelGGL(k_trim_complex<T>, grid2(Z * Z, n), dim3(256), 0, st, corr_os, sz.os_corr_h,
+                                   sz.os_corr_w, c_surf_in, Z);
+                fft2<T, +1>(*this, c_surf_in, n, Z, Z, st);
+                hipLaunchKernelGGL(k_quadpad<T>, grid2(So * So, n), dim3(256), 0, st, c_surf_in, Z, Z, c_surf, So, So);
+                fft2<T, -1>(*this, c_surf, n, So, So, st);
+                hipLaunchKernelGGL(k_real<T>, dim3(idivup(n * So * So, 256)), dim3(256), 0, st, c_surf, r_surf,
+                                   int64_t(n) * So * So);
+            }
+            hipLaunchKernelGGL(k_maxloc<T>, dim3(n), dim3(256), 0, st, r_surf, So, So, zidx, (T *)nullptr, mf);
+            hipLaunchKernelGGL(k_subpixel, dim3(idivup(n, 64)), dim3(64), 0, st, start, zidx, n, ratio, p.hs_down,
+                               p.hs_across, a.offset + 2 * size_t(w0));
         }
-        // sub-pixel argmax and final offset (cuAmpcorChunk.cpp:194-207)
-        hipLaunchKernelGGL(k_maxloc<T>, dim3(n), dim3(256), 0, st, r_surf, So, So, zidx, (T *)nullptr,
-                           a.margin_fine ? a.margin_fine + w0 : nullptr);
-        hipLaunchKernelGGL(k_subpixel, dim3(idivup(n, 64)), dim3(64), 0, st, start, zidx, n,
-                           1.0f / float(p.surf_ovs * p.raw_ovs), p.hs_down, p.hs_across, a.offset + 2 * size_t(w0));
         if (a.maxloc) HIP_OK(hipMemcpyAsync(a.maxloc + 2 * size_t(w0), loc, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToDevice, st));
         if (a.start) HIP_OK(hipMemcpyAsync(a.start + 2 * size_t(w0), start, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToDevice, st));
         if (a.shift) HIP_OK(hipMemcpyAsync(a.shift + 2 * size_t(w0), shift, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToDevice, st));

@@ -155,7 +155,7 @@ struct OsArgs {
     int deramp;
     const cf *hsh;                    // [N] e^{-i pi k_s/N} / N
     float *out;                       // [n][2N][2N]
-    double *mean;                     // [n] or null
+    double *mean;                     // [n][2]: mean and sum of squared deviations of |.|, or null
 };
 
 __host__ __device__ inline int os_lds_bytes(int N) { return al16(N * (2 * N + 1) * 8 + N * 8) + 32 * 8; }
@@ -221,13 +221,14 @@ __global__ void __launch_bounds__(NT) k_oversample(OsArgs a, const cf *tw, int w
     fft_dr2nat<float, -1, R1, R2, NT>(C + N, N, PC, 1, tw);
     // even output rows 2r: interleave X (even columns) and Y (odd columns)
     float *out = a.out + size_t(win) * 4 * N * N;
-    double sum = 0.0;
+    double sum = 0.0, sum2 = 0.0;
     for (int i = threadIdx.x; i < N * 2 * N; i += NT) {
         const int r = i / (2 * N), x = i - r * 2 * N;
         const cf v = C[r * PC + ((x & 1) ? N : 0) + (x >> 1)];
         const float amp = sqrtf(v.x * v.x + v.y * v.y);
         out[size_t(2 * r) * 2 * N + x] = amp;
         sum += amp;
+        sum2 += double(amp) * amp;
     }
     __syncthreads();
     // interpolate all 2N columns of [X | Y] -> odd output rows
@@ -245,10 +246,17 @@ __global__ void __launch_bounds__(NT) k_oversample(OsArgs a, const cf *tw, int w
         const float amp = sqrtf(v.x * v.x + v.y * v.y);
         out[size_t(2 * r + 1) * 2 * N + x] = amp;
         sum += amp;
+        sum2 += double(amp) * amp;
     }
     if (a.mean) {
+        // FP64 mean and sum of squared deviations (sum2 - sum^2 / count)
         sum = block_sum(sum, red);
-        if (threadIdx.x == 0) a.mean[win] = sum / double(4 * N * N);
+        sum2 = block_sum(sum2, red);
+        const double cnt = double(4 * N * N);
+        if (threadIdx.x == 0) {
+            a.mean[2 * win] = sum / cnt;
+            a.mean[2 * win + 1] = fmax(sum2 - sum * sum / cnt, 0.0);
+        }
     }
 }
 
@@ -281,7 +289,7 @@ struct FineLds {
 };
 
 struct FineArgs {
-    const float *tamp; const double *tmean;   // [n][T][T], [n]
+    const float *tamp; const double *tmean;   // [n][T][T], [n][2] (mean, sum of squared deviations)
     const float *samp;                         // [n][P][P]
     const int32_t *start;
     int hs_d, hs_a;
@@ -308,14 +316,7 @@ __global__ void __launch_bounds__(NT) k_fine(FineArgs a, const cf *tw, int w0) {
     const int win = w0 + blockIdx.x;
     const float *tp = a.tamp + size_t(win) * T * T;
     const float *sg = a.samp + size_t(win) * P * P;
-    const double mean = a.tmean[win];
-
-    double t2 = 0.0;
-    for (int i = threadIdx.x; i < T * T; i += NT) {
-        const double d = double(tp[i]) - mean;
-        t2 += d * d;
-    }
-    t2 = block_sum(t2, red);
+    const double mean = a.tmean[2 * win], t2 = a.tmean[2 * win + 1];
     for (int i = threadIdx.x; i < nS * L; i += NT) {
         const int j = i / L, x = i - j * L;
         S[j * PP + x] = {sg[(2 * j) * P + x], sg[(2 * j + 1) * P + x]};
@@ -485,7 +486,7 @@ void Engine::run_fused(const RunArgs &a) {
     int32_t *shift = a.shift ? a.shift : buf<int32_t>("f_shift", size_t(a.n) * 2);
     int32_t *zidx = a.zoom_idx ? a.zoom_idx : buf<int32_t>("f_zidx", size_t(a.n) * 2);
     float *tamp = buf<float>("f_tamp", size_t(nb) * T2 * T2);
-    double *tmean = buf<double>("f_tmean", nb);
+    double *tmean = buf<double>("f_tmean", 2 * size_t(nb));
     float *samp = buf<float>("f_samp", size_t(nb) * P * P);
     const bool want_margin = precision == 1 || a.margin_coarse || a.margin_fine;
     float *mc = a.margin_coarse ? a.margin_coarse : (want_margin ? buf<float>("f_mc", a.n) : nullptr);
@@ -530,10 +531,10 @@ void Engine::run_fused(const RunArgs &a) {
         });
         // chips use absolute window indices; oversampler outputs are per chunk
         OsArgs oref{a.ref, a.ref_pitch, mode, a.ref_d, a.ref_a, nullptr, p.deramp, h_ref,
-                    tamp - size_t(w0) * T2 * T2, tmean - w0};
+                    tamp - size_t(w0) * T2 * T2, tmean - 2 * size_t(w0)};
         OsArgs osec{a.sec, a.sec_pitch, mode, a.sec_d, a.sec_a, start, p.deramp, h_zoom,
                     samp - size_t(w0) * P * P, nullptr};
-        FineArgs fa{tamp - size_t(w0) * T2 * T2, tmean - w0, samp - size_t(w0) * P * P, start, p.hs_down,
+        FineArgs fa{tamp - size_t(w0) * T2 * T2, tmean - 2 * size_t(w0), samp - size_t(w0) * P * P, start, p.hs_down,
                     p.hs_across, 1.0f / float(p.surf_ovs * p.raw_ovs), tw256, zidx, a.offset, mf};
         if (W == 64) {
             timed("k_oversample_ref", n, st, [&] {

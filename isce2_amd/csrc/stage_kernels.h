@@ -453,6 +453,92 @@ __global__ void k_to_float(const T *in, float *out, int n) {
 }
 
 // final offset (cuOffset.cu:137-146)
+// ---------------------------------------------------------------------------
+// 16x16 -> 256x256 surface oversampling (cuOverSampler.cpp:141-148, real part
+// cuArraysCopy.cu:326-339), argmax (cuOffset.cu:23-84) and sub-pixel offset
+// (cuOffset.cu:137-187) in one kernel, one 256-thread block per window. The
+// padded 256x256 spectrum has only 16x16 non-zero bins, so with p = 16a + b,
+// q = 16c + d and signed bins ks:
+//   G[k1][16c+d]   = DFT16_k2( A[k1][k2] e^{-2 pi i d ks2/256} )[c]
+//   out[16a+b][q]  = Re DFT16_k1( G[k1][q] e^{-2 pi i b ks1/256} )[a]
+// where A = (+i 2-D DFT of the trimmed patch) / 256. Needs zoom = 16 and a
+// surface oversampling factor of 16. LDS: (2*256 + 16*256) cx<T>.
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(256) k_surface(const T *corr, int ch, int cw, const int32_t *start, int hs_down,
+                                                 int hs_across, float ratio, const cx<T> *tw256, int32_t *zidx,
+                                                 float *margin, float *offset) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T> *a16 = reinterpret_cast<cx<T> *>(smem_raw);
+    cx<T> *tws = a16 + 256;
+    cx<T> *G = tws + 256;
+    __shared__ T shv[4], shv2[4];
+    __shared__ uint64_t shk[4];
+    const int win = blockIdx.x;
+    const T *c = corr + size_t(win) * ch * cw;
+    {
+        const int r = threadIdx.x >> 4, q = threadIdx.x & 15;
+        a16[threadIdx.x] = {c[r * cw + q] * T(1.0 / 256.0), T(0)};
+        tws[threadIdx.x] = tw256[threadIdx.x];
+    }
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        cx<T> x[16];
+#pragma unroll
+        for (int q = 0; q < 16; ++q) x[q] = a16[threadIdx.x * 16 + q];
+        dft16<T, +1>(x);
+#pragma unroll
+        for (int q = 0; q < 16; ++q) a16[threadIdx.x * 16 + q] = x[q];
+    }
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        cx<T> x[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) x[r] = a16[r * 16 + threadIdx.x];
+        dft16<T, +1>(x);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) a16[r * 16 + threadIdx.x] = x[r];
+    }
+    __syncthreads();
+    {
+        const int k1 = threadIdx.x >> 4, dd = threadIdx.x & 15;
+        cx<T> x[16];
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) {
+            const int ks = k2 < 8 ? k2 : k2 - 16;
+            x[k2] = cmul(a16[k1 * 16 + k2], tws[(dd * ks + 4096) & 255]);
+        }
+        dft16<T, -1>(x);
+#pragma unroll
+        for (int cc = 0; cc < 16; ++cc) G[k1 * 256 + 16 * cc + dd] = x[cc];
+    }
+    __syncthreads();
+    Top2<T> top;
+    #pragma unroll 1
+    for (int task = threadIdx.x; task < 4096; task += 256) {
+        const int qq = task & 255, b = task >> 8;
+        cx<T> x[16];
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) {
+            const int ks = k1 < 8 ? k1 : k1 - 16;
+            x[k1] = cmul(G[k1 * 256 + qq], tws[(b * ks + 4096) & 255]);
+        }
+        dft16<T, -1>(x);
+#pragma unroll
+        for (int aa = 0; aa < 16; ++aa) top.add(x[aa].x, maxloc_key(uint32_t((16 * aa + b) * 256 + qq)));
+    }
+    top.template reduce<256>(shv, shv2, shk);
+    if (threadIdx.x == 0) {
+        const int idx = top.index();
+        const int px = idx >> 8, py = idx & 255;
+        zidx[2 * win] = px;
+        zidx[2 * win + 1] = py;
+        offset[2 * win] = ratio * float(px) + float(start[2 * win]) - float(hs_down);
+        offset[2 * win + 1] = ratio * float(py) + float(start[2 * win + 1]) - float(hs_across);
+        if (margin) margin[win] = top.margin();
+    }
+}
+
 static __global__ void k_subpixel(const int32_t *start, const int32_t *zidx, int n, float ratio, int hs_down,
                            int hs_across, float *offset) {
     const int w = blockIdx.x * blockDim.x + threadIdx.x;
