@@ -18,7 +18,7 @@ __host__ __device__ inline int al16(int x) { return (x + 15) & ~15; }
 // coarse stage (cuAmpcorChunk.cpp:18-106), one workgroup per window
 // ---------------------------------------------------------------------------
 struct CoarseLds {
-    int spec, nrm, corr, red, total, y_off;   // bytes; y_off in cf units from spec
+    int spec, nrm, corr, part, red, total, y_off;   // bytes; y_off in cf units from spec
     __host__ __device__ CoarseLds(int W, int L, int SH, int NLh, int NLw) {
         const int P = L + 1, H = L / 2;
         const int nS = (SH + 1) / 2, nT = W / 2;
@@ -29,7 +29,8 @@ struct CoarseLds {
         spec = 0;
         nrm = al16(spec + cf_units * 8);
         corr = al16(nrm + NLh * NLw * 4);
-        red = al16(corr + NLh * NLw * 4);
+        part = al16(corr + NLh * NLw * 4);                 // window-sum scratch
+        red = al16(part + 2 * NORM_BLOCKS * L * 8);        // L >= search width
         total = red + 32 * 8;
     }
 };
@@ -102,7 +103,7 @@ __global__ void __launch_bounds__(NT) k_coarse(CoarseArgs a, const cf *tw, int w
             const cf v = S[(r >> 1) * P + c];
             return (r & 1) ? v.y : v.x;
         },
-        SW, W, W, NLh, NLw, t2, nrm);
+        SW, W, W, NLh, NLw, t2, nrm, reinterpret_cast<double *>(smem + lay.part));
     __syncthreads();
 
     // row spectra of search and template rows (one in-place pass over both)
@@ -327,7 +328,7 @@ __global__ void __launch_bounds__(NT) k_fine(FineArgs a, const cf *tw, int w0) {
             const cf v = S[(r >> 1) * PP + c];
             return (r & 1) ? v.y : v.x;
         },
-        P, T, T, NL, NL, t2, nrm);
+        P, T, T, NL, NL, t2, nrm, reinterpret_cast<double *>(Tb));   // template chunks not loaded yet
     __syncthreads();
     fft_nat2dr<float, -1, R1, R2, NT>(S, nS, PP, 1, tw);
     split_pairs_inplace<R1, R2>(S, nS, PP);

@@ -49,12 +49,39 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
 // ---------------------------------------------------------------------------
 // FP64 window sums -> nrm[u][v] = 1/sqrt((s2 - s1^2/n) t2 + FLT_EPSILON)
 // (cuCorrNormalization.cu:402-423, cuCorrNormalizationSAT.cu:237-240).
-// Waves own contiguous lag rows, lanes own columns (CPL per lane); vertical
-// sums slide down the rows, horizontal window sums come from wave prefix
-// scans (differences of FP64 prefix sums). No barriers inside.
+// Vertical sums of s and s^2 over the first TH rows of every column come from
+// NB row blocks summed by all threads at once (`part`: 2*NB*SW doubles of
+// scratch); each wave then slides them down to its first lag row, and walks
+// its contiguous lag rows with the horizontal window sums taken as differences
+// of wave prefix scans (lanes own columns, CPL per lane). One barrier, which
+// every thread of the workgroup must reach.
 // ---------------------------------------------------------------------------
+constexpr int NORM_BLOCKS = 4;
+
 template <int CPL, class Amp>
-__device__ void window_norms(Amp amp, int SW, int TH, int TW, int NLh, int NLw, double t2, float *nrm) {
+__device__ void window_norms(Amp amp, int SW, int TH, int TW, int NLh, int NLw, double t2, float *nrm,
+                             double *part) {
+    constexpr int NB = NORM_BLOCKS;
+    const int rows_per = (TH + NB - 1) / NB;
+    #pragma unroll 1
+    for (int t = threadIdx.x; t < NB * SW; t += NT) {
+        const int b = t / SW, c = t - b * SW;
+        const int r0 = b * rows_per, r1 = min(TH, r0 + rows_per);
+        double a1 = 0.0, a2 = 0.0, b1 = 0.0, b2 = 0.0;
+        int r = r0;
+        for (; r + 1 < r1; r += 2) {
+            const double x = amp(r, c), y = amp(r + 1, c);
+            a1 += x; a2 += x * x;
+            b1 += y; b2 += y * y;
+        }
+        if (r < r1) {
+            const double x = amp(r, c);
+            a1 += x; a2 += x * x;
+        }
+        part[b * SW + c] = a1 + b1;
+        part[(NB + b) * SW + c] = a2 + b2;
+    }
+    __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     constexpr int NW = NT / 64;
     const int per = (NLh + NW - 1) / NW;
@@ -65,13 +92,19 @@ __device__ void window_norms(Amp amp, int SW, int TH, int TW, int NLh, int NLw, 
     for (int m = 0; m < CPL; ++m) {
         g1[m] = 0.0; g2[m] = 0.0;
         const int c = lane + 64 * m;
-        if (c < SW)
-            #pragma unroll 1
-            for (int i = 0; i < TH; ++i) {
-                const double v = amp(u_beg + i, c);
-                g1[m] += v;
-                g2[m] += v * v;
+        if (c < SW) {
+#pragma unroll
+            for (int b = 0; b < NB; ++b) {
+                g1[m] += part[b * SW + c];
+                g2[m] += part[(NB + b) * SW + c];
             }
+            #pragma unroll 1
+            for (int u = 0; u < u_beg; ++u) {
+                const double vn = amp(u + TH, c), vo = amp(u, c);
+                g1[m] += vn - vo;
+                g2[m] += vn * vn - vo * vo;
+            }
+        }
     }
     const double inv_n = 1.0 / double(TH * TW);
     #pragma unroll 1
