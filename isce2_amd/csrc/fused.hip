@@ -60,9 +60,13 @@ __global__ void __launch_bounds__(NT) k_coarse(CoarseArgs a, const cf *tw, int w
     double *red = reinterpret_cast<double *>(smem + lay.red);
     const int win = w0 + blockIdx.x;
 
-    // template amplitudes, row pairs packed as (row 2j, row 2j+1); FP64 mean and sum of squares
+    // template and search amplitudes, row pairs packed as (row 2j, row 2j+1);
+    // both gathers are issued before the first reduction (unrolled for memory
+    // parallelism), then the template's FP64 mean and sum of squares
     const int64_t rd = a.ref_d[win], ra = a.ref_a[win];
+    const int64_t sd = a.sec_d[win], sa = a.sec_a[win];
     double sum = 0.0;
+#pragma unroll 4
     for (int i = threadIdx.x; i < nT * L; i += NT) {
         const int j = i / L, x = i - j * L;
         cf v = {0.f, 0.f};
@@ -72,6 +76,16 @@ __global__ void __launch_bounds__(NT) k_coarse(CoarseArgs a, const cf *tw, int w
             sum += double(v.x) + double(v.y);
         }
         Tb[j * P + x] = v;
+    }
+#pragma unroll 4
+    for (int i = threadIdx.x; i < nS * L; i += NT) {
+        const int j = i / L, x = i - j * L;
+        cf v = {0.f, 0.f};
+        if (x < SW) {
+            v.x = pixel_amp(a.sec, a.mode, (sd + 2 * j) * a.sec_pitch + sa + x);
+            if (2 * j + 1 < SH) v.y = pixel_amp(a.sec, a.mode, (sd + 2 * j + 1) * a.sec_pitch + sa + x);
+        }
+        S[j * P + x] = v;
     }
     const double mean = block_sum(sum, red) / double(W * W);
     double t2 = 0.0;
@@ -85,19 +99,6 @@ __global__ void __launch_bounds__(NT) k_coarse(CoarseArgs a, const cf *tw, int w
         }
     }
     t2 = block_sum(t2, red);
-
-    // search amplitudes packed the same way
-    const int64_t sd = a.sec_d[win], sa = a.sec_a[win];
-    for (int i = threadIdx.x; i < nS * L; i += NT) {
-        const int j = i / L, x = i - j * L;
-        cf v = {0.f, 0.f};
-        if (x < SW) {
-            v.x = pixel_amp(a.sec, a.mode, (sd + 2 * j) * a.sec_pitch + sa + x);
-            if (2 * j + 1 < SH) v.y = pixel_amp(a.sec, a.mode, (sd + 2 * j + 1) * a.sec_pitch + sa + x);
-        }
-        S[j * P + x] = v;
-    }
-    __syncthreads();
     window_norms<(L + 63) / 64>(
         [&](int r, int c) -> double {
             const cf v = S[(r >> 1) * P + c];
@@ -171,6 +172,7 @@ __global__ void __launch_bounds__(NT) k_oversample(OsArgs a, const cf *tw, int w
     const int win = w0 + blockIdx.x;
     int64_t r0 = a.sd[win], c0 = a.sa[win];
     if (a.start) { r0 += a.start[2 * win]; c0 += a.start[2 * win + 1]; }
+#pragma unroll 4
     for (int i = threadIdx.x; i < N * N; i += NT) {
         const int r = i / N, c = i - r * N;
         C[r * PC + c] = pixel_cx(a.img, a.mode, (r0 + r) * a.pitch + c0 + c);
@@ -318,6 +320,7 @@ __global__ void __launch_bounds__(NT) k_fine(FineArgs a, const cf *tw, int w0) {
     const float *tp = a.tamp + size_t(win) * T * T;
     const float *sg = a.samp + size_t(win) * P * P;
     const double mean = a.tmean[2 * win], t2 = a.tmean[2 * win + 1];
+#pragma unroll 4
     for (int i = threadIdx.x; i < nS * L; i += NT) {
         const int j = i / L, x = i - j * L;
         S[j * PP + x] = {sg[(2 * j) * P + x], sg[(2 * j + 1) * P + x]};
@@ -340,6 +343,7 @@ __global__ void __launch_bounds__(NT) k_fine(FineArgs a, const cf *tw, int w0) {
     for (int k = 0; k < K; ++k) q[k] = {0.f, 0.f};
     for (int c0 = 0; c0 < T; c0 += TCH) {
         const int nr = min(TCH, T - c0);
+#pragma unroll 3
         for (int i = threadIdx.x; i < (TCH / 2) * L; i += NT) {
             const int j = i / L, x = i - j * L;
             const int r = c0 + 2 * j;
