@@ -266,13 +266,15 @@ __global__ void __launch_bounds__(NT) k_oversample(OsArgs a, const cf *tw, int w
 // ---------------------------------------------------------------------------
 // fine stage (cuAmpcorChunk.cpp:148-207): oversampled correlation, normalization,
 // trim to 16x16, 16x surface interpolation (16-sparse spectrum evaluated with
-// 16-point DFTs), argmax with the reference tie rule, sub-pixel offset
+// 16-point DFTs), argmax with the reference tie rule, sub-pixel offset. The
+// whole template's row spectra sit in LDS next to the search's (for window 64:
+// 72 + 64 packed signals of 144 bins, 158 KB), so both are transformed and
+// split together and the per-bin sums run over all template rows in one pass.
 // ---------------------------------------------------------------------------
-constexpr int TCH = 64;   // template rows per spectrum chunk
 
 struct FineLds {
     int spec, tchunk, nrm, corr, red, total, y_off;
-    __host__ __device__ FineLds(int P, int NL) {
+    __host__ __device__ FineLds(int P, int NL, int T) {
         const int L = P, PP = L + 1, H = L / 2;
         const int nS = P / 2;
         y_off = (NL * H + 1) & ~1;
@@ -280,7 +282,7 @@ struct FineLds {
         const int npair = (NL + 1) / 2;
         if (y_off + npair * PP > s_units) s_units = y_off + npair * PP;
         if (s_units < 16 * 256) s_units = 16 * 256;            // surface G[16][256]
-        int t_units = (TCH / 2) * PP;
+        int t_units = (T / 2) * PP;                           // template row spectra
         if (t_units < 2 * 256) t_units = 2 * 256;             // 16x16 surface + twiddles
         spec = 0;
         tchunk = al16(s_units * 8);
@@ -310,7 +312,7 @@ __global__ void __launch_bounds__(NT) k_fine(FineArgs a, const cf *tw, int w0) {
     constexpr int Z = NL - 1;          // 16 (trimmed surface)
     constexpr int SO = 256;            // oversampled surface side
     static_assert(Z == 16, "fused surface stage assumes a 16x16 zoom surface");
-    const FineLds lay(P, NL);
+    const FineLds lay(P, NL, T);
     cf *S = reinterpret_cast<cf *>(smem + lay.spec);
     cf *Tb = reinterpret_cast<cf *>(smem + lay.tchunk);
     float *nrm = reinterpret_cast<float *>(smem + lay.nrm);
@@ -331,35 +333,38 @@ __global__ void __launch_bounds__(NT) k_fine(FineArgs a, const cf *tw, int w0) {
             const cf v = S[(r >> 1) * PP + c];
             return (r & 1) ? v.y : v.x;
         },
-        P, T, T, NL, NL, t2, nrm, reinterpret_cast<double *>(Tb));   // template chunks not loaded yet
+        P, T, T, NL, NL, t2, nrm, reinterpret_cast<double *>(Tb));   // template not loaded yet
     __syncthreads();
-    fft_nat2dr<float, -1, R1, R2, NT>(S, nS, PP, 1, tw);
-    split_pairs_inplace<R1, R2>(S, nS, PP);
+    // template rows minus the FP64 mean, row pairs packed, zero beyond column T
+#pragma unroll 3
+    for (int i = threadIdx.x; i < (T / 2) * L; i += NT) {
+        const int j = i / L, x = i - j * L;
+        cf v = {0.f, 0.f};
+        if (x < T) {
+            v.x = float(double(tp[(2 * j) * T + x]) - mean);
+            v.y = float(double(tp[(2 * j + 1) * T + x]) - mean);
+        }
+        Tb[j * PP + x] = v;
+    }
+    __syncthreads();
+    if (lay.tchunk == nS * PP * 8) {
+        // search and template spectra are one evenly spaced set of signals
+        fft_nat2dr<float, -1, R1, R2, NT>(S, nS + T / 2, PP, 1, tw);
+        split_pairs_inplace<R1, R2>(S, nS + T / 2, PP);
+    } else {
+        fft_nat2dr<float, -1, R1, R2, NT>(S, nS, PP, 1, tw);
+        fft_nat2dr<float, -1, R1, R2, NT>(Tb, T / 2, PP, 1, tw);
+        split_pairs_inplace<R1, R2>(S, nS, PP);
+        split_pairs_inplace<R1, R2>(Tb, T / 2, PP);
+    }
 
     const int f = threadIdx.x % H, blk = threadIdx.x / H, u0 = blk * K;
     const bool active = (blk < NT / H) && (u0 < NL);
     cf q[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) q[k] = {0.f, 0.f};
-    for (int c0 = 0; c0 < T; c0 += TCH) {
-        const int nr = min(TCH, T - c0);
-#pragma unroll 3
-        for (int i = threadIdx.x; i < (TCH / 2) * L; i += NT) {
-            const int j = i / L, x = i - j * L;
-            const int r = c0 + 2 * j;
-            cf v = {0.f, 0.f};
-            if (x < T && 2 * j < nr) {
-                v.x = float(double(tp[r * T + x]) - mean);
-                if (2 * j + 1 < nr) v.y = float(double(tp[(r + 1) * T + x]) - mean);
-            }
-            Tb[j * PP + x] = v;
-        }
-        __syncthreads();
-        fft_nat2dr<float, -1, R1, R2, NT>(Tb, TCH / 2, PP, 1, tw);
-        split_pairs_inplace<R1, R2>(Tb, TCH / 2, PP);
-        if (active) mac_rows<H, K>(q, S, PP, Tb, PP, c0, c0, c0 + nr, u0, P, f);
-        __syncthreads();
-    }
+    if (active) mac_rows<H, K>(q, S, PP, Tb, PP, 0, 0, T, u0, P, f);
+    __syncthreads();
     cf *Qs = S;
     if (active) {
 #pragma unroll
@@ -471,6 +476,8 @@ bool fused_supported(const ampcor_params &p, const ampcor_sizes &s) {
     if (L == 0) return false;
     if (pick_k(s.raw_corr_h, NT / (L / 2)) == 0) return false;
     if (CoarseLds(p.window_w, L, s.search_h, s.raw_corr_h, s.raw_corr_w).total > 160 * 1024) return false;
+    const int P = 2 * s.zoom_in_w;
+    if (FineLds(P, P - 2 * p.window_w + 1, 2 * p.window_w).total > 160 * 1024) return false;
     return true;
 }
 
@@ -525,7 +532,7 @@ void Engine::run_fused(const RunArgs &a) {
                   loc, val, start, shift, a.snr, a.cov, mc};
     const size_t lds_c = CoarseLds(W, L1, sz.search_h, sz.raw_corr_h, sz.raw_corr_w).total;
     const size_t lds_or = os_lds_bytes(W), lds_oz = os_lds_bytes(Zc);
-    const size_t lds_f = FineLds(P, P - T2 + 1).total;
+    const size_t lds_f = FineLds(P, P - T2 + 1, T2).total;
 
     for (int w0 = 0; w0 < a.n; w0 += nb) {
         const int n = std::min(nb, a.n - w0);
