@@ -238,9 +238,7 @@ void Engine::run_staged_t(const RunArgs &a) {
     T *r_sec = buf<T>(tn + "r_sec", size_t(nb) * S_h * S_w);
     const size_t zmax = std::max(size_t(L1h) * L1w, size_t(L2h) * L2w);
     cx<T> *zc = buf<cx<T>>(tn + "zc", size_t(nb) * zmax);
-    const size_t satmax = std::max(size_t(S_h + 1) * (S_w + 1), size_t(L2h + 1) * (L2w + 1));
-    double *sat1 = buf<double>(tn + "sat1", size_t(nb) * satmax);
-    double *sat2 = buf<double>(tn + "sat2", size_t(nb) * satmax);
+    auto norm_lds = [](int sw) { return size_t(12) * sw * sizeof(double) + 32; };
     T *corr = buf<T>(tn + "corr", size_t(nb) * sz.raw_corr_h * sz.raw_corr_w);
     int32_t *loc = buf<int32_t>(tn + "loc", size_t(nb) * 2);
     T *val = buf<T>(tn + "val", nb);
@@ -283,9 +281,8 @@ void Engine::run_staged_t(const RunArgs &a) {
         hipLaunchKernelGGL(k_crop_real<T>, grid2(sz.raw_corr_h * sz.raw_corr_w, n), dim3(256), 0, st, zc, L1h, L1w,
                            corr, sz.raw_corr_h, sz.raw_corr_w);
         // normalize (cuAmpcorChunk.cpp:60)
-        hipLaunchKernelGGL(k_sat<T>, dim3(n), dim3(256), 0, st, r_sec, S_h, S_w, sat1, sat2);
-        hipLaunchKernelGGL(k_normalize<T>, dim3(n), dim3(256), 0, st, corr, sz.raw_corr_h, sz.raw_corr_w, r_ref,
-                           W_h, W_w, sat1, sat2, S_h, S_w);
+        hipLaunchKernelGGL(k_normalize<T>, dim3(n), dim3(256), norm_lds(S_w), st, corr, sz.raw_corr_h,
+                           sz.raw_corr_w, r_ref, W_h, W_w, r_sec, S_h, S_w);
         // argmax, covariance, SNR, zoom start/shift (cuAmpcorChunk.cpp:69-106)
         hipLaunchKernelGGL(k_maxloc<T>, dim3(n), dim3(256), 0, st, corr, sz.raw_corr_h, sz.raw_corr_w, loc, val,
                            a.margin_coarse ? a.margin_coarse + w0 : nullptr);
@@ -293,7 +290,8 @@ void Engine::run_staged_t(const RunArgs &a) {
                            loc, val, W_h * W_w, sz.corr_raw_zoom_h, sz.corr_raw_zoom_w, p.hs_down, p.hs_across,
                            sz.half_zoom, a.cov + 3 * size_t(w0), a.snr + w0, start, shift);
         // reference oversampling (cuAmpcorChunk.cpp:115-126)
-        if (p.deramp == 1) hipLaunchKernelGGL(k_deramp<T>, dim3(n), dim3(256), 0, st, c_ref, W_h, W_w);
+        if (p.deramp == 1)
+            hipLaunchKernelGGL(k_deramp<T>, dim3(n), dim3(256), size_t(W_h + W_w) * 16, st, c_ref, W_h, W_w);
         fft2<T, +1>(*this, c_ref, n, W_h, W_w, st);
         hipLaunchKernelGGL(k_quadpad<T>, grid2(sz.os_win_h * sz.os_win_w, n), dim3(256), 0, st, c_ref, W_h, W_w,
                            c_ref_os, sz.os_win_h, sz.os_win_w);
@@ -305,7 +303,8 @@ void Engine::run_staged_t(const RunArgs &a) {
         hipLaunchKernelGGL(k_crop_start<T>, grid2(sz.zoom_in_h * sz.zoom_in_w, n), dim3(256), 0, st, c_sec, S_h,
                            S_w, start, c_zoom, sz.zoom_in_h, sz.zoom_in_w);
         if (p.deramp == 1)
-            hipLaunchKernelGGL(k_deramp<T>, dim3(n), dim3(256), 0, st, c_zoom, sz.zoom_in_h, sz.zoom_in_w);
+            hipLaunchKernelGGL(k_deramp<T>, dim3(n), dim3(256), size_t(sz.zoom_in_h + sz.zoom_in_w) * 16, st, c_zoom,
+                               sz.zoom_in_h, sz.zoom_in_w);
         fft2<T, +1>(*this, c_zoom, n, sz.zoom_in_h, sz.zoom_in_w, st);
         hipLaunchKernelGGL(k_quadpad<T>, grid2(L2h * L2w, n), dim3(256), 0, st, c_zoom, sz.zoom_in_h, sz.zoom_in_w,
                            c_sec_os, L2h, L2w);
@@ -321,9 +320,8 @@ void Engine::run_staged_t(const RunArgs &a) {
         fft2<T, +1>(*this, zc, n, L2h, L2w, st);
         hipLaunchKernelGGL(k_crop_real<T>, grid2(sz.os_corr_h * sz.os_corr_w, n), dim3(256), 0, st, zc, L2h, L2w,
                            corr_os, sz.os_corr_h, sz.os_corr_w);
-        hipLaunchKernelGGL(k_sat<T>, dim3(n), dim3(256), 0, st, r_sec_os, L2h, L2w, sat1, sat2);
-        hipLaunchKernelGGL(k_normalize<T>, dim3(n), dim3(256), 0, st, corr_os, sz.os_corr_h, sz.os_corr_w,
-                           r_ref_os, sz.os_win_h, sz.os_win_w, sat1, sat2, L2h, L2w);
+        hipLaunchKernelGGL(k_normalize<T>, dim3(n), dim3(256), norm_lds(L2w), st, corr_os, sz.os_corr_h,
+                           sz.os_corr_w, r_ref_os, sz.os_win_h, sz.os_win_w, r_sec_os, L2h, L2w);
         // trim + surface oversampling + argmax + sub-pixel offset (cuAmpcorChunk.cpp:169-207)
         float *mf = a.margin_fine ? a.margin_fine + w0 : nullptr;
         const float ratio = 1.0f / float(p.surf_ovs * p.raw_ovs);

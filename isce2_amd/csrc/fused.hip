@@ -160,15 +160,19 @@ struct OsArgs {
     double *mean;                     // [n][2]: mean and sum of squared deviations of |.|, or null
 };
 
-__host__ __device__ inline int os_lds_bytes(int N) { return al16(N * (2 * N + 1) * 8 + N * 8) + 32 * 8; }
+// [X | Y] chip (pitch 2N+1), shift table, FP64 deramp row/column phase factors, reduction scratch
+__host__ __device__ inline int os_lds_bytes(int N) { return al16(N * (2 * N + 1) * 8 + N * 8) + 2 * N * 16 + 32 * 8; }
 
 template <int R1, int R2>
 __global__ void __launch_bounds__(NT) k_oversample(OsArgs a, const cf *tw, int w0) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int N = R1 * R2, PC = 2 * N + 1;
+    static_assert(2 * N <= NT, "one thread per deramp row and column factor");
     cf *C = reinterpret_cast<cf *>(smem);
     cf *hsh = C + N * PC;
-    double *red = reinterpret_cast<double *>(smem + al16(N * PC * 8 + N * 8));
+    cx<double> *rowf = reinterpret_cast<cx<double> *>(smem + al16(N * PC * 8 + N * 8));   // e^{i r phy}
+    cx<double> *colf = rowf + N;                                                         // e^{i c phx}
+    double *red = reinterpret_cast<double *>(colf + N);
     const int win = w0 + blockIdx.x;
     int64_t r0 = a.sd[win], c0 = a.sa[win];
     if (a.start) { r0 += a.start[2 * win]; c0 += a.start[2 * win + 1]; }
@@ -198,13 +202,19 @@ __global__ void __launch_bounds__(NT) k_oversample(OsArgs a, const cf *tw, int w
         ayr = block_sum(ayr, red); ayi = block_sum(ayi, red);
         axr = block_sum(axr, red); axi = block_sum(axi, red);
         const double phy = atan2(ayi, ayr), phx = atan2(axi, axr);
+        // the phase c*phx + r*phy is separable: one FP64 sincos per row and per column
+        if (threadIdx.x < N) sincos(double(threadIdx.x) * phy, &rowf[threadIdx.x].y, &rowf[threadIdx.x].x);
+        else if (threadIdx.x - N < N) {
+            const int c = threadIdx.x - N;
+            sincos(double(c) * phx, &colf[c].y, &colf[c].x);
+        }
+        __syncthreads();
         for (int i = threadIdx.x; i < N * N; i += NT) {
             const int r = i / N, c = i - r * N;
-            double sn, cs;
-            sincos(double(c) * phx + double(r) * phy, &sn, &cs);
+            const cx<double> f = cmul(colf[c], rowf[r]);
             const cf z = C[r * PC + c];
-            const double nx = double(z.x) * cs - double(z.y) * sn;
-            C[r * PC + c] = {float(nx), float(double(z.y) * cs + nx * sn)};
+            const double nx = double(z.x) * f.x - double(z.y) * f.y;
+            C[r * PC + c] = {float(nx), float(double(z.y) * f.x + nx * f.y)};
         }
         __syncthreads();
     }

@@ -162,64 +162,95 @@ __global__ void k_crop_real(const cx<T> *z, int Lh, int Lw, T *out, int oh, int 
 }
 
 // ---------------------------------------------------------------------------
-// normalization with FP64 summed-area tables (cuCorrNormalizationSAT.cu:150-242,
-// formula of cuCorrNormalization.cu:402-423)
+// normalization (formula of cuCorrNormalization.cu:402-423 and
+// cuCorrNormalizationSAT.cu:237-240) with FP64 window sums computed directly:
+// per-column vertical sums over the template height, slid down one lag row at
+// a time, and horizontal window sums from a block prefix scan of them. One
+// 256-thread block per window; dynamic LDS 12*sw doubles.
 // ---------------------------------------------------------------------------
-template <class T>
-__global__ void __launch_bounds__(256) k_sat(const T *srch, int sh, int sw, double *sat1, double *sat2) {
-    const int win = blockIdx.x;
-    const T *s = srch + size_t(win) * sh * sw;
-    const int p = sw + 1;
-    double *a1 = sat1 + size_t(win) * (sh + 1) * p;
-    double *a2 = sat2 + size_t(win) * (sh + 1) * p;
-    for (int c = threadIdx.x; c < p; c += blockDim.x) { a1[c] = 0.0; a2[c] = 0.0; }
-    for (int r = threadIdx.x; r < sh; r += blockDim.x) {
-        double x1 = 0.0, x2 = 0.0;
-        a1[(r + 1) * p] = 0.0;
-        a2[(r + 1) * p] = 0.0;
-        for (int c = 0; c < sw; ++c) {
-            const double v = double(s[r * sw + c]);
-            x1 += v;
-            x2 += v * v;
-            a1[(r + 1) * p + c + 1] = x1;
-            a2[(r + 1) * p + c + 1] = x2;
-        }
+__device__ __forceinline__ void block_scan256(double &x1, double &x2, double *sh4) {
+    // inclusive scan of (x1, x2) over the 256 threads; sh4 holds 8 doubles
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const double y1 = __shfl_up(x1, d), y2 = __shfl_up(x2, d);
+        if (lane >= d) { x1 += y1; x2 += y2; }
     }
     __syncthreads();
-    for (int c = threadIdx.x; c < p; c += blockDim.x) {
-        double x1 = 0.0, x2 = 0.0;
-        for (int r = 1; r <= sh; ++r) {
-            x1 += a1[r * p + c];
-            x2 += a2[r * p + c];
-            a1[r * p + c] = x1;
-            a2[r * p + c] = x2;
-        }
-    }
+    if (lane == 63) { sh4[wv] = x1; sh4[4 + wv] = x2; }
+    __syncthreads();
+    for (int k = 0; k < wv; ++k) { x1 += sh4[k]; x2 += sh4[4 + k]; }
 }
 
 template <class T>
 __global__ void __launch_bounds__(256) k_normalize(T *corr, int ch, int cw, const T *tmpl, int th, int tw,
-                                                   const double *sat1, const double *sat2, int sh, int sw) {
+                                                   const T *srch, int sh, int sw) {
+    constexpr int NB = 4;
     __shared__ double sh_red[8];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double *V1 = reinterpret_cast<double *>(smem_raw);
+    double *V2 = V1 + sw;
+    double *P1 = V2 + sw;            // [sw + 1] prefix sums
+    double *P2 = P1 + sw + 1;
+    double *part = P2 + sw + 1;      // [2][NB][sw]
     const int win = blockIdx.x;
     const T *t = tmpl + size_t(win) * th * tw;
+    const T *s = srch + size_t(win) * sh * sw;
     double t2 = 0.0;
     for (int i = threadIdx.x; i < th * tw; i += blockDim.x) {
         const double v = double(t[i]);
         t2 += v * v;
     }
     t2 = block_sum256(t2, sh_red);
-    const int p = sw + 1;
-    const double *a1 = sat1 + size_t(win) * (sh + 1) * p;
-    const double *a2 = sat2 + size_t(win) * (sh + 1) * p;
+    const int rows_per = (th + NB - 1) / NB;
+    for (int task = threadIdx.x; task < NB * sw; task += blockDim.x) {
+        const int b = task / sw, c = task - b * sw;
+        const int r1 = min(th, (b + 1) * rows_per);
+        double a1 = 0.0, a2 = 0.0;
+        for (int r = b * rows_per; r < r1; ++r) {
+            const double v = double(s[size_t(r) * sw + c]);
+            a1 += v;
+            a2 += v * v;
+        }
+        part[b * sw + c] = a1;
+        part[(NB + b) * sw + c] = a2;
+    }
+    __syncthreads();
+    for (int c = threadIdx.x; c < sw; c += blockDim.x) {
+        double a1 = 0.0, a2 = 0.0;
+        for (int b = 0; b < NB; ++b) { a1 += part[b * sw + c]; a2 += part[(NB + b) * sw + c]; }
+        V1[c] = a1;
+        V2[c] = a2;
+    }
+    if (threadIdx.x == 0) { P1[0] = 0.0; P2[0] = 0.0; }
+    __syncthreads();
+    const int per = (sw + blockDim.x - 1) / blockDim.x;     // columns per thread, contiguous
+    const int c0 = threadIdx.x * per, c1 = min(sw, c0 + per);
     const double inv = 1.0 / double(th * tw);
-    T *c = corr + size_t(win) * ch * cw;
-    for (int i = threadIdx.x; i < ch * cw; i += blockDim.x) {
-        const int u = i / cw, v = i - u * cw;
-        const double s1 = a1[(u + th) * p + v + tw] - a1[u * p + v + tw] - a1[(u + th) * p + v] + a1[u * p + v];
-        const double s2 = a2[(u + th) * p + v + tw] - a2[u * p + v + tw] - a2[(u + th) * p + v] + a2[u * p + v];
-        const double norm2 = (s2 - s1 * s1 * inv) * t2;
-        c[i] = T(double(c[i]) / sqrt(norm2 + double(FLT_EPSILON)));
+    T *cc = corr + size_t(win) * ch * cw;
+    for (int u = 0; u < ch; ++u) {
+        double x1 = 0.0, x2 = 0.0;
+        for (int c = c0; c < c1; ++c) { x1 += V1[c]; x2 += V2[c]; }
+        block_scan256(x1, x2, part);                          // part is free now
+        double e1 = x1, e2 = x2;                              // inclusive end of this thread's segment
+        for (int c = c1 - 1; c >= c0; --c) {
+            P1[c + 1] = e1; P2[c + 1] = e2;
+            e1 -= V1[c]; e2 -= V2[c];
+        }
+        __syncthreads();
+        for (int v = threadIdx.x; v < cw; v += blockDim.x) {
+            const double s1 = P1[v + tw] - P1[v], s2 = P2[v + tw] - P2[v];
+            const double norm2 = (s2 - s1 * s1 * inv) * t2;
+            cc[u * cw + v] = T(double(cc[u * cw + v]) / sqrt(norm2 + double(FLT_EPSILON)));
+        }
+        if (u + 1 < ch) {
+            for (int c = threadIdx.x; c < sw; c += blockDim.x) {
+                const double vn = double(s[size_t(u + th) * sw + c]), vo = double(s[size_t(u) * sw + c]);
+                V1[c] += vn - vo;
+                V2[c] += vn * vn - vo * vo;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -319,6 +350,9 @@ __global__ void k_stats(const T *corr, int nx, int ny, int n, const int32_t *loc
 template <class T>
 __global__ void __launch_bounds__(256) k_deramp(cx<T> *chips, int h, int w) {
     __shared__ double sh[8];
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<double> *rowf = reinterpret_cast<cx<double> *>(smem_raw);   // e^{i r phy}
+    cx<double> *colf = rowf + h;                                   // e^{i c phx}
     cx<T> *z = chips + size_t(blockIdx.x) * h * w;
     double ayr = 0, ayi = 0, axr = 0, axi = 0;
     for (int i = threadIdx.x; i < h * w; i += blockDim.x) {
@@ -341,14 +375,16 @@ __global__ void __launch_bounds__(256) k_deramp(cx<T> *chips, int h, int w) {
     axi = block_sum256(axi, sh);
     const double phy = atan2(ayi, ayr);
     const double phx = atan2(axi, axr);
+    // the phase c*phx + r*phy is separable: one FP64 sincos per row and per column
+    for (int r = threadIdx.x; r < h; r += blockDim.x) sincos(double(r) * phy, &rowf[r].y, &rowf[r].x);
+    for (int c = threadIdx.x; c < w; c += blockDim.x) sincos(double(c) * phx, &colf[c].y, &colf[c].x);
+    __syncthreads();
     for (int i = threadIdx.x; i < h * w; i += blockDim.x) {
         const int r = i / w, c = i - r * w;
-        const double ph = double(c) * phx + double(r) * phy;
-        double s, co;
-        sincos(ph, &s, &co);
+        const cx<double> f = cmul(colf[c], rowf[r]);
         const double x = double(z[i].x), y = double(z[i].y);
-        const double nx = x * co - y * s;
-        const double ny = y * co + nx * s;
+        const double nx = x * f.x - y * f.y;
+        const double ny = y * f.x + nx * f.y;
         z[i] = {T(nx), T(ny)};
     }
 }
