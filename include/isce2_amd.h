@@ -63,6 +63,8 @@ typedef struct ampcor_sizes {
     int32_t surf_out;                    /* oversampled surface side */
     int32_t corr_raw_zoom_h, corr_raw_zoom_w; /* SNR patch */
     int32_t fused;                       /* 1 if the fused per-window kernels cover this config */
+    int32_t fft_corr_raw_h, fft_corr_raw_w; /* staged correlation transform sizes (>= search chip, */
+    int32_t fft_corr_os_h, fft_corr_os_w;   /* smallest 2^a 3^b 5^c: any length >= the chip gives the same lags) */
 } ampcor_sizes;
 
 /* Execution modes of ampcor_run_windows. */

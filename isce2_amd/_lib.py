@@ -38,7 +38,8 @@ class Sizes(C.Structure):
     _fields_ = [(n, C.c_int32) for n in (
         "zoom", "half_zoom", "search_h", "search_w", "zoom_in_h", "zoom_in_w", "os_win_h",
         "os_win_w", "os_search_h", "os_search_w", "raw_corr_h", "raw_corr_w", "os_corr_h",
-        "os_corr_w", "surf_out", "corr_raw_zoom_h", "corr_raw_zoom_w", "fused")]
+        "os_corr_w", "surf_out", "corr_raw_zoom_h", "corr_raw_zoom_w", "fused",
+        "fft_corr_raw_h", "fft_corr_raw_w", "fft_corr_os_h", "fft_corr_os_w")]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_}

@@ -67,9 +67,11 @@ def factorizations(n: int, limit: int = 4):
 
 def staged_lengths(sizes: dict, params: WindowParams):
     """Transform lengths the staged pipeline uses for a configuration."""
-    lens = {sizes["search_h"], sizes["search_w"], params.window_h, params.window_w,
+    lens = {sizes["fft_corr_raw_h"], sizes["fft_corr_raw_w"], params.window_h, params.window_w,
             sizes["os_win_h"], sizes["os_win_w"], sizes["zoom_in_h"], sizes["zoom_in_w"],
-            sizes["os_search_h"], sizes["os_search_w"], sizes["zoom"], sizes["surf_out"]}
+            sizes["os_search_h"], sizes["os_search_w"], sizes["fft_corr_os_h"], sizes["fft_corr_os_w"]}
+    if params.surf_method or sizes["zoom"] != 16 or sizes["surf_out"] != 256:
+        lens |= {sizes["zoom"], sizes["surf_out"]}
     return sorted(lens)
 
 

@@ -135,6 +135,20 @@ void Engine::setup(const ampcor_params &q) {
     if (std::max({s.search_h, s.search_w, s.os_search_h, s.os_search_w, s.surf_out}) > 4096)
         throw Error(AMPCOR_E_UNSUPPORTED, "chip or surface side above 4096");
     s.fused = fused_supported(q, s) ? 1 : 0;
+    // the correlation is linear for every transform length >= the search chip
+    // (cuCorrFrequency.cu:16-43 uses the chip size): take the next 5-smooth length
+    auto smooth = [](int n) {
+        for (int m = n;; ++m) {
+            int r = m;
+            for (int f : {2, 3, 5})
+                while (r % f == 0) r /= f;
+            if (r == 1) return m;
+        }
+    };
+    s.fft_corr_raw_h = smooth(s.search_h);
+    s.fft_corr_raw_w = smooth(s.search_w);
+    s.fft_corr_os_h = smooth(s.os_search_h);
+    s.fft_corr_os_w = smooth(s.os_search_w);
     p = q;
     sz = s;
     configured = true;
@@ -227,8 +241,8 @@ void Engine::run_staged_t(const RunArgs &a) {
     const int S_h = sz.search_h, S_w = sz.search_w;
     const int chunk = std::min(a.chunk > 0 ? a.chunk : 1024, 65535);
     const int mode = p.image_complex ? (p.deramp == 0 ? 1 : 0) : 2;
-    // correlation transform sizes: the search chip sizes (cuCorrFrequency.cu:16-43)
-    const int L1h = S_h, L1w = S_w, L2h = sz.os_search_h, L2w = sz.os_search_w;
+    // correlation transform sizes (5-smooth, >= the search chips)
+    const int L1h = sz.fft_corr_raw_h, L1w = sz.fft_corr_raw_w, L2h = sz.fft_corr_os_h, L2w = sz.fft_corr_os_w;
     const int Z = sz.zoom, So = sz.surf_out;
     const int nb = std::min(chunk, a.n);
 
@@ -248,8 +262,9 @@ void Engine::run_staged_t(const RunArgs &a) {
     cx<T> *c_ref_os = buf<cx<T>>(tn + "c_ref_os", size_t(nb) * sz.os_win_h * sz.os_win_w);
     T *r_ref_os = buf<T>(tn + "r_ref_os", size_t(nb) * sz.os_win_h * sz.os_win_w);
     cx<T> *c_zoom = buf<cx<T>>(tn + "c_zoom", size_t(nb) * sz.zoom_in_h * sz.zoom_in_w);
-    cx<T> *c_sec_os = buf<cx<T>>(tn + "c_sec_os", size_t(nb) * L2h * L2w);
-    T *r_sec_os = buf<T>(tn + "r_sec_os", size_t(nb) * L2h * L2w);
+    const int OSh = sz.os_search_h, OSw = sz.os_search_w;   // oversampled zoom chip (fixed by the interpolation)
+    cx<T> *c_sec_os = buf<cx<T>>(tn + "c_sec_os", size_t(nb) * OSh * OSw);
+    T *r_sec_os = buf<T>(tn + "r_sec_os", size_t(nb) * OSh * OSw);
     T *corr_os = buf<T>(tn + "corr_os", size_t(nb) * sz.os_corr_h * sz.os_corr_w);
     // the 16-sparse surface kernel needs no 256x256 buffers
     const bool fast_surface = !p.surf_method && Z == 16 && So == 256;
@@ -306,22 +321,22 @@ void Engine::run_staged_t(const RunArgs &a) {
             hipLaunchKernelGGL(k_deramp<T>, dim3(n), dim3(256), size_t(sz.zoom_in_h + sz.zoom_in_w) * 16, st, c_zoom,
                                sz.zoom_in_h, sz.zoom_in_w);
         fft2<T, +1>(*this, c_zoom, n, sz.zoom_in_h, sz.zoom_in_w, st);
-        hipLaunchKernelGGL(k_quadpad<T>, grid2(L2h * L2w, n), dim3(256), 0, st, c_zoom, sz.zoom_in_h, sz.zoom_in_w,
-                           c_sec_os, L2h, L2w);
-        fft2<T, -1>(*this, c_sec_os, n, L2h, L2w, st);
-        hipLaunchKernelGGL(k_abs<T>, dim3(idivup(n * L2h * L2w, 256)), dim3(256), 0, st, c_sec_os, r_sec_os,
-                           int64_t(n) * L2h * L2w);
+        hipLaunchKernelGGL(k_quadpad<T>, grid2(OSh * OSw, n), dim3(256), 0, st, c_zoom, sz.zoom_in_h, sz.zoom_in_w,
+                           c_sec_os, OSh, OSw);
+        fft2<T, -1>(*this, c_sec_os, n, OSh, OSw, st);
+        hipLaunchKernelGGL(k_abs<T>, dim3(idivup(n * OSh * OSw, 256)), dim3(256), 0, st, c_sec_os, r_sec_os,
+                           int64_t(n) * OSh * OSw);
         // oversampled correlation + normalize (cuAmpcorChunk.cpp:148-161)
         hipLaunchKernelGGL(k_pack_corr<T>, grid2(L2h * L2w, n), dim3(256), 0, st, r_ref_os, sz.os_win_h,
-                           sz.os_win_w, r_sec_os, L2h, L2w, zc, L2h, L2w);
+                           sz.os_win_w, r_sec_os, OSh, OSw, zc, L2h, L2w);
         fft2<T, -1>(*this, zc, n, L2h, L2w, st);
         hipLaunchKernelGGL(k_spec_product<T>, grid2(L2h * L2w, n), dim3(256), 0, st, zc, L2h, L2w,
                            T(1.0 / (double(L2h) * L2w)));
         fft2<T, +1>(*this, zc, n, L2h, L2w, st);
         hipLaunchKernelGGL(k_crop_real<T>, grid2(sz.os_corr_h * sz.os_corr_w, n), dim3(256), 0, st, zc, L2h, L2w,
                            corr_os, sz.os_corr_h, sz.os_corr_w);
-        hipLaunchKernelGGL(k_normalize<T>, dim3(n), dim3(256), norm_lds(L2w), st, corr_os, sz.os_corr_h,
-                           sz.os_corr_w, r_ref_os, sz.os_win_h, sz.os_win_w, r_sec_os, L2h, L2w);
+        hipLaunchKernelGGL(k_normalize<T>, dim3(n), dim3(256), norm_lds(OSw), st, corr_os, sz.os_corr_h,
+                           sz.os_corr_w, r_ref_os, sz.os_win_h, sz.os_win_w, r_sec_os, OSh, OSw);
         // trim + surface oversampling + argmax + sub-pixel offset (cuAmpcorChunk.cpp:169-207)
         float *mf = a.margin_fine ? a.margin_fine + w0 : nullptr;
         const float ratio = 1.0f / float(p.surf_ovs * p.raw_ovs);

@@ -104,20 +104,48 @@ __device__ __forceinline__ void dft3(cx<T> *x) {
     x[2] = m - r;
 }
 
-// generic odd-prime DFT with table twiddles: X_q = sum_r x_r w_R^{rq}
+// odd-prime DFT using the conjugate symmetry of the kernel: with
+// a_r = x_r + x_{R-r}, b_r = x_r - x_{R-r} (r = 1..M, M = (R-1)/2),
+//   X_q     = x_0 + sum_r a_r cos(2 pi rq/R) + i DIR sum_r b_r sin(2 pi rq/R)
+//   X_{R-q} = x_0 + sum_r a_r cos(2 pi rq/R) - i DIR sum_r b_r sin(2 pi rq/R)
+// The M distinct cos/sin values are read once from the length-n table.
 template <class T, int DIR, int R>
-__device__ __forceinline__ void dft_generic(cx<T> *x, const cx<T> *tw, int n) {
+__device__ __forceinline__ void dft_odd(cx<T> *x, const cx<T> *tw, int n) {
+    constexpr int M = (R - 1) / 2;
     const int st = n / R;
-    cx<T> y[R];
+    T c[M + 1], s[M + 1];
 #pragma unroll
-    for (int q = 0; q < R; ++q) {
-        cx<T> acc = x[0];
-#pragma unroll
-        for (int r = 1; r < R; ++r) acc = acc + cmul(x[r], twid<T, DIR>(tw, ((r * q) % R) * st));
-        y[q] = acc;
+    for (int k = 1; k <= M; ++k) {
+        const cx<T> w = tw[k * st];           // e^{-2 pi i k/R}
+        c[k] = w.x;
+        s[k] = -w.y;                          // sin(2 pi k/R) > 0
     }
+    cx<T> a[M + 1], b[M + 1];
+    cx<T> sum = x[0];
 #pragma unroll
-    for (int q = 0; q < R; ++q) x[q] = y[q];
+    for (int r = 1; r <= M; ++r) {
+        a[r] = x[r] + x[R - r];
+        b[r] = x[r] - x[R - r];
+        sum = sum + a[r];
+    }
+    const cx<T> x0 = x[0];
+    x[0] = sum;
+#pragma unroll
+    for (int q = 1; q <= M; ++q) {
+        cx<T> A = x0, B = {T(0), T(0)};
+#pragma unroll
+        for (int r = 1; r <= M; ++r) {
+            const int m = (r * q) % R;
+            const T cv = m <= M ? c[m] : c[R - m];
+            const T sv = m <= M ? s[m] : -s[R - m];
+            A.x += a[r].x * cv; A.y += a[r].y * cv;
+            B.x += b[r].x * sv; B.y += b[r].y * sv;
+        }
+        // i DIR B
+        const cx<T> iB = DIR > 0 ? cx<T>{-B.y, B.x} : cx<T>{B.y, -B.x};
+        x[q] = A + iB;
+        x[R - q] = A - iB;
+    }
 }
 
 template <class T, int DIR, int R>
@@ -127,7 +155,7 @@ __device__ __forceinline__ void dft_radix(cx<T> *x, const cx<T> *tw, int n) {
     else if constexpr (R == 8) dft8<T, DIR>(x);
     else if constexpr (R == 16) dft16<T, DIR>(x);
     else if constexpr (R == 3) dft3<T, DIR>(x);
-    else dft_generic<T, DIR, R>(x, tw, n);
+    else dft_odd<T, DIR, R>(x, tw, n);
 }
 
 // one Stockham pass of radix R: ns = product of the radices already applied
