@@ -17,20 +17,24 @@ __host__ __device__ inline int al16(int x) { return (x + 15) & ~15; }
 // ---------------------------------------------------------------------------
 // coarse stage (cuAmpcorChunk.cpp:18-106), one workgroup per window
 // ---------------------------------------------------------------------------
+// template rows per spectrum chunk of the coarse stage: the window-64 layout
+// then fits two workgroups per CU (77 KB of LDS each)
+__host__ __device__ constexpr int coarse_tc(int W) { return W > 32 ? 32 : W; }
+
 struct CoarseLds {
-    int spec, nrm, corr, part, red, total, y_off;   // bytes; y_off in cf units from spec
+    int spec, nrm, red, total, y_off, corr_off;   // bytes (spec, nrm, red, total); cf units (y_off, corr_off)
     __host__ __device__ CoarseLds(int W, int L, int SH, int NLh, int NLw) {
         const int P = L + 1, H = L / 2;
-        const int nS = (SH + 1) / 2, nT = W / 2;
-        y_off = (NLh * H + 1) & ~1;
+        const int nS = (SH + 1) / 2, nTc = coarse_tc(W) / 2;
+        y_off = (NLh * H + 1) & ~1;                   // lag spectra Q, then the inverse pairs Y
         const int npair = (NLh + 1) / 2;
-        int cf_units = (nS + nT) * P;
-        if (y_off + npair * P > cf_units) cf_units = y_off + npair * P;
+        corr_off = y_off + npair * P;                 // then the normalized surface
+        int cf_units = (nS + nTc) * P;                // search + one template chunk
+        const int need = corr_off + (NLh * NLw + 1) / 2;
+        if (need > cf_units) cf_units = need;
         spec = 0;
         nrm = al16(spec + cf_units * 8);
-        corr = al16(nrm + NLh * NLw * 4);
-        part = al16(corr + NLh * NLw * 4);                 // window-sum scratch
-        red = al16(part + 2 * NORM_BLOCKS * L * 8);        // L >= search width
+        red = al16(nrm + NLh * NLw * 4);
         total = red + 32 * 8;
     }
 };
@@ -46,9 +50,11 @@ struct CoarseArgs {
 };
 
 template <int W, int R1, int R2, int K>
-__global__ void __launch_bounds__(NT) k_coarse(CoarseArgs a, const cf *tw, int w0) {
+__global__ void __launch_bounds__(NT, (K >= 9 ? 2 : 4)) k_coarse(CoarseArgs a, const cf *tw, int w0) {
     extern __shared__ __align__(16) unsigned char smem[];
-    constexpr int L = R1 * R2, H = L / 2, P = L + 1, nT = W / 2;
+    constexpr int L = R1 * R2, H = L / 2, P = L + 1, TC = coarse_tc(W), nTc = TC / 2;
+    static_assert(W % TC == 0, "template chunks cover the window");
+    static_assert(2 * NORM_BLOCKS * L <= nTc * P, "window-sum scratch fits the template chunk region");
     const int SH = W + 2 * a.hs_d, SW = W + 2 * a.hs_a;
     const int NLh = SH - W + 1, NLw = SW - W + 1;
     const int nS = (SH + 1) / 2;
@@ -56,28 +62,15 @@ __global__ void __launch_bounds__(NT) k_coarse(CoarseArgs a, const cf *tw, int w
     cf *S = reinterpret_cast<cf *>(smem + lay.spec);
     cf *Tb = S + nS * P;
     float *nrm = reinterpret_cast<float *>(smem + lay.nrm);
-    float *corr = reinterpret_cast<float *>(smem + lay.corr);
+    float *corr = reinterpret_cast<float *>(S + lay.corr_off);
     double *red = reinterpret_cast<double *>(smem + lay.red);
     const int win = w0 + blockIdx.x;
 
-    // template and search amplitudes, row pairs packed as (row 2j, row 2j+1);
-    // both gathers are issued before the first reduction (unrolled for memory
-    // parallelism), then the template's FP64 mean and sum of squares
+    // search amplitudes, row pairs packed as (row 2j, row 2j+1), and the
+    // template's FP64 sum and sum of squares, gathers unrolled for memory parallelism
     const int64_t rd = a.ref_d[win], ra = a.ref_a[win];
     const int64_t sd = a.sec_d[win], sa = a.sec_a[win];
-    double sum = 0.0;
-#pragma unroll 4
-    for (int i = threadIdx.x; i < nT * L; i += NT) {
-        const int j = i / L, x = i - j * L;
-        cf v = {0.f, 0.f};
-        if (x < W) {
-            v.x = pixel_amp(a.ref, a.mode, (rd + 2 * j) * a.ref_pitch + ra + x);
-            v.y = pixel_amp(a.ref, a.mode, (rd + 2 * j + 1) * a.ref_pitch + ra + x);
-            sum += double(v.x) + double(v.y);
-        }
-        Tb[j * P + x] = v;
-    }
-#pragma unroll 4
+#pragma unroll 2
     for (int i = threadIdx.x; i < nS * L; i += NT) {
         const int j = i / L, x = i - j * L;
         cf v = {0.f, 0.f};
@@ -87,37 +80,56 @@ __global__ void __launch_bounds__(NT) k_coarse(CoarseArgs a, const cf *tw, int w
         }
         S[j * P + x] = v;
     }
-    const double mean = block_sum(sum, red) / double(W * W);
-    double t2 = 0.0;
-    for (int i = threadIdx.x; i < nT * L; i += NT) {
-        const int j = i / L, x = i - j * L;
-        if (x < W) {
-            const cf v = Tb[j * P + x];
-            const double d0 = double(v.x) - mean, d1 = double(v.y) - mean;
-            t2 += d0 * d0 + d1 * d1;
-            Tb[j * P + x] = {float(d0), float(d1)};
-        }
+    double sum = 0.0, sum2 = 0.0;
+#pragma unroll 2
+    for (int i = threadIdx.x; i < W * W; i += NT) {
+        const int r = i / W, x = i - r * W;
+        const double v = pixel_amp(a.ref, a.mode, (rd + r) * a.ref_pitch + ra + x);
+        sum += v;
+        sum2 += v * v;
     }
-    t2 = block_sum(t2, red);
+    sum = block_sum(sum, red);
+    sum2 = block_sum(sum2, red);
+    const double mean = sum / double(W * W);
+    const double t2 = fmax(sum2 - sum * sum / double(W * W), 0.0);   // sum of squared deviations
     window_norms<(L + 63) / 64>(
         [&](int r, int c) -> double {
             const cf v = S[(r >> 1) * P + c];
             return (r & 1) ? v.y : v.x;
         },
-        SW, W, W, NLh, NLw, t2, nrm, reinterpret_cast<double *>(smem + lay.part));
+        SW, W, W, NLh, NLw, t2, nrm, reinterpret_cast<double *>(Tb));   // template chunk not loaded yet
     __syncthreads();
-
-    // row spectra of search and template rows (one in-place pass over both)
-    fft_nat2dr<float, -1, R1, R2, NT>(S, nS + nT, P, 1, tw);
-    split_pairs_inplace<R1, R2>(S, nS + nT, P);
 
     const int f = threadIdx.x % H, blk = threadIdx.x / H, u0 = blk * K;
     const bool active = (blk < NT / H) && (u0 < NLh);
     cf q[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) q[k] = {0.f, 0.f};
-    if (active) mac_rows<H, K>(q, S, P, Tb, P, 0, 0, W, u0, SH, f);
-    __syncthreads();
+    #pragma unroll 1
+    for (int c0 = 0; c0 < W; c0 += TC) {
+        // template rows [c0, c0 + TC) minus the FP64 mean, packed in pairs
+#pragma unroll 2
+        for (int i = threadIdx.x; i < nTc * L; i += NT) {
+            const int j = i / L, x = i - j * L;
+            cf v = {0.f, 0.f};
+            if (x < W) {
+                v.x = float(double(pixel_amp(a.ref, a.mode, (rd + c0 + 2 * j) * a.ref_pitch + ra + x)) - mean);
+                v.y = float(double(pixel_amp(a.ref, a.mode, (rd + c0 + 2 * j + 1) * a.ref_pitch + ra + x)) - mean);
+            }
+            Tb[j * P + x] = v;
+        }
+        __syncthreads();
+        if (c0 == 0) {
+            // row spectra of the search and of the first template chunk in one pass
+            fft_nat2dr<float, -1, R1, R2, NT>(S, nS + nTc, P, 1, tw);
+            split_pairs_inplace<R1, R2>(S, nS + nTc, P);
+        } else {
+            fft_nat2dr<float, -1, R1, R2, NT>(Tb, nTc, P, 1, tw);
+            split_pairs_inplace<R1, R2>(Tb, nTc, P);
+        }
+        if (active) mac_rows<H, K>(q, S, P, Tb, P, c0, c0, c0 + TC, u0, SH, f);
+        __syncthreads();
+    }
     cf *Qs = S;
     if (active) {
 #pragma unroll
@@ -547,8 +559,8 @@ void Engine::run_fused(const RunArgs &a) {
     for (int w0 = 0; w0 < a.n; w0 += nb) {
         const int n = std::min(nb, a.n - w0);
         timed("k_coarse", n, st, [&] {
-            if (W == 64) launch_coarse<64, 16, 8>(K1, ca, tw1, n, w0, lds_c, st);
-            else if (L1 == 128) launch_coarse<32, 16, 8>(K1, ca, tw1, n, w0, lds_c, st);
+            if (W == 64) launch_coarse<64, 8, 16>(K1, ca, tw1, n, w0, lds_c, st);
+            else if (L1 == 128) launch_coarse<32, 8, 16>(K1, ca, tw1, n, w0, lds_c, st);
             else launch_coarse<32, 8, 8>(K1, ca, tw1, n, w0, lds_c, st);
         });
         // chips use absolute window indices; oversampler outputs are per chunk
