@@ -64,6 +64,12 @@ struct Top2 {
             v2 = mx(v2, v);
         }
     }
+    // same as add(v, maxloc_key(idx)) for finite v above -FLT_MAX; the key is only formed on a tie
+    __device__ __forceinline__ void add_idx(T v, uint32_t idx) {
+        const T lo = v < v1 ? v : v1;
+        v2 = mx(v2, lo);
+        if (v > v1 || (v == v1 && maxloc_key(idx) < k1)) { v1 = v; k1 = maxloc_key(idx); }
+    }
     __device__ __forceinline__ void merge(T ov1, uint64_t ok1, T ov2) {
         if (better(ov1, ok1, v1, k1)) {
             v2 = mx(mx(v2, ov2), v1);

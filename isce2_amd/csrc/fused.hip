@@ -160,8 +160,12 @@ __global__ void __launch_bounds__(NT, (K >= 9 ? 2 : 4)) k_coarse(CoarseArgs a, c
 // ---------------------------------------------------------------------------
 // x2 chip oversampling (cuOverSampler.cpp:72-78, cuArraysPadding.cu:223-266,
 // cuDeramp.cu:343-390): per axis, odd samples = (1/N) FFT_-(FFT_+(x) e^{-i pi k_s/N}),
-// even samples = x. Rows of the chip X go to Y (odd columns); then every
-// column of [X | Y] is interpolated the same way. |.| goes to HBM.
+// even samples = x. Output pixel (2r+p, 2c+q) comes from:
+//   p=0,q=0: X;  p=1,q=0: columns of X interpolated;
+//   p=0,q=1: Y = rows of X interpolated;  p=1,q=1: columns of Y interpolated.
+// Only one N x N complex chip lives in LDS (so several workgroups share a CU):
+// the chip is gathered (and deramped) twice, once for the X column pass and
+// once for the Y passes. |.| goes to HBM.
 // ---------------------------------------------------------------------------
 struct OsArgs {
     const void *img; int64_t pitch; int mode;
@@ -172,13 +176,13 @@ struct OsArgs {
     double *mean;                     // [n][2]: mean and sum of squared deviations of |.|, or null
 };
 
-// [X | Y] chip (pitch 2N+1), shift table, FP64 deramp row/column phase factors, reduction scratch
-__host__ __device__ inline int os_lds_bytes(int N) { return al16(N * (2 * N + 1) * 8 + N * 8) + 2 * N * 16 + 32 * 8; }
+// chip (pitch N+1), shift table, FP64 deramp row/column phase factors, reduction scratch
+__host__ __device__ inline int os_lds_bytes(int N) { return al16(N * (N + 1) * 8 + N * 8) + 2 * N * 16 + 32 * 8; }
 
 template <int R1, int R2>
 __global__ void __launch_bounds__(NT) k_oversample(OsArgs a, const cf *tw, int w0) {
     extern __shared__ __align__(16) unsigned char smem[];
-    constexpr int N = R1 * R2, PC = 2 * N + 1;
+    constexpr int N = R1 * R2, PC = N + 1;
     static_assert(2 * N <= NT, "one thread per deramp row and column factor");
     cf *C = reinterpret_cast<cf *>(smem);
     cf *hsh = C + N * PC;
@@ -188,15 +192,27 @@ __global__ void __launch_bounds__(NT) k_oversample(OsArgs a, const cf *tw, int w
     const int win = w0 + blockIdx.x;
     int64_t r0 = a.sd[win], c0 = a.sa[win];
     if (a.start) { r0 += a.start[2 * win]; c0 += a.start[2 * win + 1]; }
+    auto gather = [&]() {
 #pragma unroll 4
-    for (int i = threadIdx.x; i < N * N; i += NT) {
-        const int r = i / N, c = i - r * N;
-        C[r * PC + c] = pixel_cx(a.img, a.mode, (r0 + r) * a.pitch + c0 + c);
-    }
+        for (int i = threadIdx.x; i < N * N; i += NT) {
+            const int r = i / N, c = i - r * N;
+            C[r * PC + c] = pixel_cx(a.img, a.mode, (r0 + r) * a.pitch + c0 + c);
+        }
+    };
+    // deramp (float2's operator*= (float2.h:71-75): the imaginary part uses the new real part)
+    auto apply_deramp = [&]() {
+        for (int i = threadIdx.x; i < N * N; i += NT) {
+            const int r = i / N, c = i - r * N;
+            const cx<double> f = cmul(colf[c], rowf[r]);
+            const cf z = C[r * PC + c];
+            const double nx = double(z.x) * f.x - double(z.y) * f.y;
+            C[r * PC + c] = {float(nx), float(double(z.y) * f.x + nx * f.y)};
+        }
+    };
+    gather();
     for (int i = threadIdx.x; i < N; i += NT) hsh[i] = a.hsh[i];
     __syncthreads();
     if (a.deramp == 1) {
-        // float2's operator*= (float2.h:71-75): the imaginary part uses the new real part
         double ayr = 0, ayi = 0, axr = 0, axi = 0;
         for (int i = threadIdx.x; i < N * N; i += NT) {
             const int r = i / N, c = i - r * N;
@@ -221,58 +237,56 @@ __global__ void __launch_bounds__(NT) k_oversample(OsArgs a, const cf *tw, int w
             sincos(double(c) * phx, &colf[c].y, &colf[c].x);
         }
         __syncthreads();
-        for (int i = threadIdx.x; i < N * N; i += NT) {
-            const int r = i / N, c = i - r * N;
-            const cx<double> f = cmul(colf[c], rowf[r]);
-            const cf z = C[r * PC + c];
-            const double nx = double(z.x) * f.x - double(z.y) * f.y;
-            C[r * PC + c] = {float(nx), float(double(z.y) * f.x + nx * f.y)};
-        }
+        apply_deramp();
         __syncthreads();
     }
-    // Y = X, then interpolate along rows
-    for (int i = threadIdx.x; i < N * N; i += NT) {
-        const int r = i / N, c = i - r * N;
-        C[r * PC + N + c] = C[r * PC + c];
-    }
-    __syncthreads();
-    fft_nat2dr<float, +1, R1, R2, NT>(C + N, N, PC, 1, tw);
-    for (int i = threadIdx.x; i < N * N; i += NT) {
-        const int r = i / N, k = i - r * N;
-        cf &z = C[r * PC + N + dr_pos<R1, R2>(k)];
-        z = cmul(z, hsh[k]);
-    }
-    __syncthreads();
-    fft_dr2nat<float, -1, R1, R2, NT>(C + N, N, PC, 1, tw);
-    // even output rows 2r: interleave X (even columns) and Y (odd columns)
     float *out = a.out + size_t(win) * 4 * N * N;
     double sum = 0.0, sum2 = 0.0;
-    for (int i = threadIdx.x; i < N * 2 * N; i += NT) {
-        const int r = i / (2 * N), x = i - r * 2 * N;
-        const cf v = C[r * PC + ((x & 1) ? N : 0) + (x >> 1)];
-        const float amp = sqrtf(v.x * v.x + v.y * v.y);
-        out[size_t(2 * r) * 2 * N + x] = amp;
-        sum += amp;
-        sum2 += double(amp) * amp;
-    }
+    auto emit = [&](int p, int q) {      // |C| -> output pixels (2r+p, 2c+q)
+        for (int i = threadIdx.x; i < N * N; i += NT) {
+            const int r = i / N, c = i - r * N;
+            const cf v = C[r * PC + c];
+            const float amp = sqrtf(v.x * v.x + v.y * v.y);
+            out[size_t(2 * r + p) * 2 * N + 2 * c + q] = amp;
+            sum += amp;
+            sum2 += double(amp) * amp;
+        }
+    };
+    auto interp_cols = [&]() {          // C <- C at half-row positions (every column)
+        fft_nat2dr<float, +1, R1, R2, NT>(C, N, 1, PC, tw);
+        for (int i = threadIdx.x; i < N * N; i += NT) {
+            const int k = i / N, c = i - k * N;
+            cf &z = C[dr_pos<R1, R2>(k) * PC + c];
+            z = cmul(z, hsh[k]);
+        }
+        __syncthreads();
+        fft_dr2nat<float, -1, R1, R2, NT>(C, N, 1, PC, tw);
+    };
+    // X: even rows, even columns; then its column interpolation: odd rows, even columns
+    emit(0, 0);
     __syncthreads();
-    // interpolate all 2N columns of [X | Y] -> odd output rows
-    fft_nat2dr<float, +1, R1, R2, NT>(C, 2 * N, 1, PC, tw);
-    for (int i = threadIdx.x; i < N * 2 * N; i += NT) {
-        const int k = i / (2 * N), c = i - k * 2 * N;
-        cf &z = C[dr_pos<R1, R2>(k) * PC + c];
+    interp_cols();
+    emit(1, 0);
+    __syncthreads();
+    // Y = X interpolated along rows: even rows, odd columns; then its columns: odd rows, odd columns
+    gather();
+    __syncthreads();
+    if (a.deramp == 1) {
+        apply_deramp();
+        __syncthreads();
+    }
+    fft_nat2dr<float, +1, R1, R2, NT>(C, N, PC, 1, tw);
+    for (int i = threadIdx.x; i < N * N; i += NT) {
+        const int r = i / N, k = i - r * N;
+        cf &z = C[r * PC + dr_pos<R1, R2>(k)];
         z = cmul(z, hsh[k]);
     }
     __syncthreads();
-    fft_dr2nat<float, -1, R1, R2, NT>(C, 2 * N, 1, PC, tw);
-    for (int i = threadIdx.x; i < N * 2 * N; i += NT) {
-        const int r = i / (2 * N), x = i - r * 2 * N;
-        const cf v = C[r * PC + ((x & 1) ? N : 0) + (x >> 1)];
-        const float amp = sqrtf(v.x * v.x + v.y * v.y);
-        out[size_t(2 * r + 1) * 2 * N + x] = amp;
-        sum += amp;
-        sum2 += double(amp) * amp;
-    }
+    fft_dr2nat<float, -1, R1, R2, NT>(C, N, PC, 1, tw);
+    emit(0, 1);
+    __syncthreads();
+    interp_cols();
+    emit(1, 1);
     if (a.mean) {
         // FP64 mean and sum of squared deviations (sum2 - sum^2 / count)
         sum = block_sum(sum, red);
@@ -450,7 +464,7 @@ __global__ void __launch_bounds__(NT) k_fine(FineArgs a, const cf *tw, int w0) {
         }
         dft16<float, -1>(x);
 #pragma unroll
-        for (int aa = 0; aa < 16; ++aa) top.add(x[aa].x, maxloc_key(uint32_t((16 * aa + b) * SO + qq)));
+        for (int aa = 0; aa < 16; ++aa) top.add_idx(x[aa].x, uint32_t((16 * aa + b) * SO + qq));
     }
     top.reduce<NT>(reinterpret_cast<float *>(red + 8), reinterpret_cast<float *>(red + 8) + 8,
                reinterpret_cast<uint64_t *>(red + 16));

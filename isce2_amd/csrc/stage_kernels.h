@@ -561,7 +561,7 @@ __global__ void __launch_bounds__(256) k_surface(const T *corr, int ch, int cw, 
         }
         dft16<T, -1>(x);
 #pragma unroll
-        for (int aa = 0; aa < 16; ++aa) top.add(x[aa].x, maxloc_key(uint32_t((16 * aa + b) * 256 + qq)));
+        for (int aa = 0; aa < 16; ++aa) top.add_idx(x[aa].x, uint32_t((16 * aa + b) * 256 + qq));
     }
     top.template reduce<256>(shv, shv2, shk);
     if (threadIdx.x == 0) {
