@@ -453,18 +453,26 @@ __global__ void __launch_bounds__(NT) k_fine(FineArgs a, const cf *tw, int w0) {
     }
     __syncthreads();
     // out[16aa+b][q] = Re DFT16_aa( G[k1][q] e^{-2 pi i b ks1/256} ); running argmax
+    // each thread keeps one column q of G in registers and sweeps its b values
+    static_assert(NT % SO == 0, "whole surface columns per thread");
     Top2<float> top;
-    for (int task = threadIdx.x; task < SO * 16; task += NT) {
-        const int qq = task & (SO - 1), b = task >> 8;
-        cf x[16];
+    {
+        const int qq = threadIdx.x & (SO - 1);
+        cf g[16];
 #pragma unroll
-        for (int k1 = 0; k1 < 16; ++k1) {
-            const int ks = k1 < 8 ? k1 : k1 - 16;
-            x[k1] = cmul(G[k1 * SO + qq], tws[(b * ks + 4096) & 255]);
+        for (int k1 = 0; k1 < 16; ++k1) g[k1] = G[k1 * SO + qq];
+        #pragma unroll 1
+        for (int b = threadIdx.x / SO; b < 16; b += NT / SO) {
+            cf x[16];
+#pragma unroll
+            for (int k1 = 0; k1 < 16; ++k1) {
+                const int ks = k1 < 8 ? k1 : k1 - 16;
+                x[k1] = cmul(g[k1], tws[(b * ks + 4096) & 255]);
+            }
+            dft16<float, -1>(x);
+#pragma unroll
+            for (int aa = 0; aa < 16; ++aa) top.add_idx(x[aa].x, uint32_t((16 * aa + b) * SO + qq));
         }
-        dft16<float, -1>(x);
-#pragma unroll
-        for (int aa = 0; aa < 16; ++aa) top.add_idx(x[aa].x, uint32_t((16 * aa + b) * SO + qq));
     }
     top.reduce<NT>(reinterpret_cast<float *>(red + 8), reinterpret_cast<float *>(red + 8) + 8,
                reinterpret_cast<uint64_t *>(red + 16));

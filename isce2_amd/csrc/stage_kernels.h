@@ -550,18 +550,23 @@ __global__ void __launch_bounds__(256) k_surface(const T *corr, int ch, int cw, 
     }
     __syncthreads();
     Top2<T> top;
-    #pragma unroll 1
-    for (int task = threadIdx.x; task < 4096; task += 256) {
-        const int qq = task & 255, b = task >> 8;
-        cx<T> x[16];
+    {
+        const int qq = threadIdx.x;     // one surface column per thread, all 16 values of b
+        cx<T> g[16];
 #pragma unroll
-        for (int k1 = 0; k1 < 16; ++k1) {
-            const int ks = k1 < 8 ? k1 : k1 - 16;
-            x[k1] = cmul(G[k1 * 256 + qq], tws[(b * ks + 4096) & 255]);
+        for (int k1 = 0; k1 < 16; ++k1) g[k1] = G[k1 * 256 + qq];
+        #pragma unroll 1
+        for (int b = 0; b < 16; ++b) {
+            cx<T> x[16];
+#pragma unroll
+            for (int k1 = 0; k1 < 16; ++k1) {
+                const int ks = k1 < 8 ? k1 : k1 - 16;
+                x[k1] = cmul(g[k1], tws[(b * ks + 4096) & 255]);
+            }
+            dft16<T, -1>(x);
+#pragma unroll
+            for (int aa = 0; aa < 16; ++aa) top.add_idx(x[aa].x, uint32_t((16 * aa + b) * 256 + qq));
         }
-        dft16<T, -1>(x);
-#pragma unroll
-        for (int aa = 0; aa < 16; ++aa) top.add_idx(x[aa].x, uint32_t((16 * aa + b) * 256 + qq));
     }
     top.template reduce<256>(shv, shv2, shk);
     if (threadIdx.x == 0) {
