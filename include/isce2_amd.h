@@ -147,6 +147,11 @@ int ampcor_precision_stats(ampcor_ctx *ctx, int64_t *checked, int64_t *rerun);
  * No counterpart in the reference (cuFFT plans internally). */
 int ampcor_set_fft_radices(ampcor_ctx *ctx, int32_t n, const int32_t *radices, int32_t count);
 
+/* Launch-shape options, for autotuning. No counterpart in the reference.
+ *   "fine_threads": 512 or 1024 threads per workgroup of the window-64 fused
+ *                   fine kernel (default 1024). */
+int ampcor_set_option(ampcor_ctx *ctx, const char *name, int64_t value);
+
 /* Per-kernel timing with HIP events recorded on the launch stream around every
  * kernel of ampcor_run_windows. No counterpart in the reference (its timing is
  * host wall clock, cuDenseOffsets.py:465,484-485); for benchmarks and profiling.

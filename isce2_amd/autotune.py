@@ -28,11 +28,14 @@ class Schedule:
     mode: int = MODE_STAGED
     chunk: int = 0
     radices: dict = field(default_factory=dict)     # FFT length -> list of radices (staged only)
+    options: dict = field(default_factory=dict)     # launch-shape options (fused only)
     ms_per_window: float = float("nan")
 
     def apply(self, eng: Engine):
         for n, r in self.radices.items():
             eng.set_fft_radices(int(n), r)
+        for k, v in self.options.items():
+            eng.set_option(k, int(v))
 
 
 def factorizations(n: int, limit: int = 4):
@@ -131,6 +134,15 @@ class Autotuner:
                 t = self._time(eng, sub, mode, min(chunk, max(m, 1)), m)
                 if best is None or t < best.ms_per_window:
                     best = Schedule(mode=mode, chunk=chunk, ms_per_window=t)
+            if best.mode == MODE_FUSED and params.window_h == 64:
+                times = {}
+                for ft in (512, 1024):
+                    eng.set_option("fine_threads", ft)
+                    times[ft] = self._time(eng, sub, MODE_FUSED, min(best.chunk, m), m)
+                ft = min(times, key=times.get)
+                eng.set_option("fine_threads", ft)
+                best.options["fine_threads"] = ft
+                best.ms_per_window = min(best.ms_per_window, times[ft])
             if best.mode == MODE_STAGED:
                 m = min(m, self.radix_sample)
                 sub = list(args[:2]) + [t[:m].contiguous() for t in args[2:]]

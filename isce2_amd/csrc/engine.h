@@ -112,6 +112,8 @@ struct Engine {
     // margins fall below the thresholds; 2 FP64 staged pipeline for every window
     // FP32 surfaces (values of order 1) carry rounding errors of a few 1e-7;
     // on a 31k-window speckle field every FP32/FP64 flip had a margin <= 6e-8
+    // launch-shape options (ampcor_set_option): threads per workgroup of the window-64 fine kernel
+    int fine_threads = 1024;
     int precision = 1;
     float tau_coarse = 2e-6f, tau_fine = 2e-6f;
     long long prec_checked = 0, prec_rerun = 0;

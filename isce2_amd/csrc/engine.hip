@@ -625,6 +625,19 @@ int ampcor_set_precision(ampcor_ctx *ctx, int32_t mode, float tau_coarse, float 
     return AMPCOR_OK;
 }
 
+int ampcor_set_option(ampcor_ctx *ctx, const char *name, int64_t value) {
+    if (!ctx || !name) return AMPCOR_E_ARG;
+    return guarded(ctx, [&] {
+        const std::string k(name);
+        if (k == "fine_threads") {
+            if (value != 512 && value != 1024) throw ampcor::Error(AMPCOR_E_ARG, "fine_threads must be 512 or 1024");
+            ctx->eng.fine_threads = int(value);
+        } else {
+            throw ampcor::Error(AMPCOR_E_ARG, "unknown option " + k);
+        }
+    });
+}
+
 int ampcor_set_fft_radices(ampcor_ctx *ctx, int32_t n, const int32_t *radices, int32_t count) {
     if (!ctx) return AMPCOR_E_ARG;
     return guarded(ctx, [&] {

@@ -325,7 +325,7 @@ struct FineLds {
         nrm = al16(tchunk + t_units * 8);
         corr = al16(nrm + NL * NL * 4);
         red = al16(corr + NL * NL * 4);
-        total = red + 32 * 8;
+        total = red + 64 * 8;          // block sums + workgroup argmax scratch (up to 1024 threads)
     }
 };
 
@@ -340,8 +340,8 @@ struct FineArgs {
     float *margin;                             // [n] top-1 minus top-2 of the surface, or null
 };
 
-template <int T, int R1, int R2, int K>
-__global__ void __launch_bounds__(NT) k_fine(FineArgs a, const cf *tw, int w0) {
+template <int T, int R1, int R2, int K, int NTH>
+__global__ void __launch_bounds__(NTH) k_fine(FineArgs a, const cf *tw, int w0) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int P = R1 * R2, L = P, PP = L + 1, H = L / 2, nS = P / 2;
     constexpr int NL = P - T + 1;      // 17
@@ -359,12 +359,12 @@ __global__ void __launch_bounds__(NT) k_fine(FineArgs a, const cf *tw, int w0) {
     const float *sg = a.samp + size_t(win) * P * P;
     const double mean = a.tmean[2 * win], t2 = a.tmean[2 * win + 1];
 #pragma unroll 4
-    for (int i = threadIdx.x; i < nS * L; i += NT) {
+    for (int i = threadIdx.x; i < nS * L; i += NTH) {
         const int j = i / L, x = i - j * L;
         S[j * PP + x] = {sg[(2 * j) * P + x], sg[(2 * j + 1) * P + x]};
     }
     __syncthreads();
-    window_norms<(P + 63) / 64>(
+    window_norms<(P + 63) / 64, NTH>(
         [&](int r, int c) -> double {
             const cf v = S[(r >> 1) * PP + c];
             return (r & 1) ? v.y : v.x;
@@ -373,7 +373,7 @@ __global__ void __launch_bounds__(NT) k_fine(FineArgs a, const cf *tw, int w0) {
     __syncthreads();
     // template rows minus the FP64 mean, row pairs packed, zero beyond column T
 #pragma unroll 3
-    for (int i = threadIdx.x; i < (T / 2) * L; i += NT) {
+    for (int i = threadIdx.x; i < (T / 2) * L; i += NTH) {
         const int j = i / L, x = i - j * L;
         cf v = {0.f, 0.f};
         if (x < T) {
@@ -385,17 +385,17 @@ __global__ void __launch_bounds__(NT) k_fine(FineArgs a, const cf *tw, int w0) {
     __syncthreads();
     if (lay.tchunk == nS * PP * 8) {
         // search and template spectra are one evenly spaced set of signals
-        fft_nat2dr<float, -1, R1, R2, NT>(S, nS + T / 2, PP, 1, tw);
-        split_pairs_inplace<R1, R2>(S, nS + T / 2, PP);
+        fft_nat2dr<float, -1, R1, R2, NTH>(S, nS + T / 2, PP, 1, tw);
+        split_pairs_inplace<R1, R2, NTH>(S, nS + T / 2, PP);
     } else {
-        fft_nat2dr<float, -1, R1, R2, NT>(S, nS, PP, 1, tw);
-        fft_nat2dr<float, -1, R1, R2, NT>(Tb, T / 2, PP, 1, tw);
-        split_pairs_inplace<R1, R2>(S, nS, PP);
-        split_pairs_inplace<R1, R2>(Tb, T / 2, PP);
+        fft_nat2dr<float, -1, R1, R2, NTH>(S, nS, PP, 1, tw);
+        fft_nat2dr<float, -1, R1, R2, NTH>(Tb, T / 2, PP, 1, tw);
+        split_pairs_inplace<R1, R2, NTH>(S, nS, PP);
+        split_pairs_inplace<R1, R2, NTH>(Tb, T / 2, PP);
     }
 
     const int f = threadIdx.x % H, blk = threadIdx.x / H, u0 = blk * K;
-    const bool active = (blk < NT / H) && (u0 < NL);
+    const bool active = (blk < NTH / H) && (u0 < NL);
     cf q[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) q[k] = {0.f, 0.f};
@@ -408,16 +408,16 @@ __global__ void __launch_bounds__(NT) k_fine(FineArgs a, const cf *tw, int w0) {
             if (u0 + k < NL) Qs[(u0 + k) * H + f] = q[k];
     }
     __syncthreads();
-    inverse_lags<R1, R2>(Qs, NL, NL, S + lay.y_off, PP, tw, nrm, corr);
+    inverse_lags<R1, R2, NTH>(Qs, NL, NL, S + lay.y_off, PP, tw, nrm, corr);
 
     // trimmed 16x16 surface -> 2-D +i DFT / 256 (cuOverSampler.cpp:141-145)
     cf *a16 = Tb;            // [16][16]
     cf *tws = Tb + 256;      // e^{-2 pi i k/256}
-    for (int i = threadIdx.x; i < Z * Z; i += NT) {
+    for (int i = threadIdx.x; i < Z * Z; i += NTH) {
         const int r = i / Z, c = i - r * Z;
         a16[i] = {corr[r * NL + c] * (1.0f / 256.0f), 0.f};
     }
-    for (int i = threadIdx.x; i < SO; i += NT) tws[i] = a.tw256[i];
+    for (int i = threadIdx.x; i < SO; i += NTH) tws[i] = a.tw256[i];
     __syncthreads();
     if (threadIdx.x < 16) {
         cf x[16];
@@ -453,29 +453,27 @@ __global__ void __launch_bounds__(NT) k_fine(FineArgs a, const cf *tw, int w0) {
     }
     __syncthreads();
     // out[16aa+b][q] = Re DFT16_aa( G[k1][q] e^{-2 pi i b ks1/256} ); running argmax
-    // each thread keeps one column q of G in registers and sweeps its b values
-    static_assert(NT % SO == 0, "whole surface columns per thread");
+    // each thread takes one column q of G and sweeps its b values
+    static_assert(NTH % SO == 0, "whole surface columns per thread");
     Top2<float> top;
     {
         const int qq = threadIdx.x & (SO - 1);
-        cf g[16];
-#pragma unroll
-        for (int k1 = 0; k1 < 16; ++k1) g[k1] = G[k1 * SO + qq];
         #pragma unroll 1
-        for (int b = threadIdx.x / SO; b < 16; b += NT / SO) {
+        for (int b = threadIdx.x / SO; b < 16; b += NTH / SO) {
             cf x[16];
 #pragma unroll
             for (int k1 = 0; k1 < 16; ++k1) {
                 const int ks = k1 < 8 ? k1 : k1 - 16;
-                x[k1] = cmul(g[k1], tws[(b * ks + 4096) & 255]);
+                x[k1] = cmul(G[k1 * SO + qq], tws[(b * ks + 4096) & 255]);
             }
             dft16<float, -1>(x);
 #pragma unroll
             for (int aa = 0; aa < 16; ++aa) top.add_idx(x[aa].x, uint32_t((16 * aa + b) * SO + qq));
         }
     }
-    top.reduce<NT>(reinterpret_cast<float *>(red + 8), reinterpret_cast<float *>(red + 8) + 8,
-               reinterpret_cast<uint64_t *>(red + 16));
+    // red: [0, 16) block sums, then NTH/64 top-1 values, runner-up values and keys
+    top.reduce<NTH>(reinterpret_cast<float *>(red + 16), reinterpret_cast<float *>(red + 16) + 16,
+                    reinterpret_cast<uint64_t *>(red + 32));
     if (threadIdx.x == 0) {
         const int idx = top.index();
         const int px = idx / SO, py = idx % SO;
@@ -600,7 +598,10 @@ void Engine::run_fused(const RunArgs &a) {
                 hipLaunchKernelGGL((k_oversample<8, 9>), dim3(n), dim3(NT), lds_oz, st, osec, twZ, w0);
             });
             timed("k_fine", n, st, [&] {
-                hipLaunchKernelGGL((k_fine<128, 16, 9, 3>), dim3(n), dim3(NT), lds_f, st, fa, tw2, w0);
+                if (fine_threads == 1024)
+                    hipLaunchKernelGGL((k_fine<128, 9, 16, 2, 1024>), dim3(n), dim3(1024), lds_f, st, fa, tw2, w0);
+                else
+                    hipLaunchKernelGGL((k_fine<128, 16, 9, 3, NT>), dim3(n), dim3(NT), lds_f, st, fa, tw2, w0);
             });
         } else {
             timed("k_oversample_ref", n, st, [&] {
@@ -610,7 +611,7 @@ void Engine::run_fused(const RunArgs &a) {
                 hipLaunchKernelGGL((k_oversample<8, 5>), dim3(n), dim3(NT), lds_oz, st, osec, twZ, w0);
             });
             timed("k_fine", n, st, [&] {
-                hipLaunchKernelGGL((k_fine<64, 16, 5, 2>), dim3(n), dim3(NT), lds_f, st, fa, tw2, w0);
+                hipLaunchKernelGGL((k_fine<64, 16, 5, 2, NT>), dim3(n), dim3(NT), lds_f, st, fa, tw2, w0);
             });
         }
         HIP_OK(hipGetLastError());

@@ -32,7 +32,8 @@ __device__ __forceinline__ cf pixel_cx(const void *img, int mode, int64_t off) {
     return {z.x, z.y};
 }
 
-// FP64 workgroup sum; `sh` holds NT/64 doubles
+// FP64 workgroup sum over NTH threads; `sh` holds NTH/64 doubles
+template <int NTH = NT>
 __device__ __forceinline__ double block_sum(double v, double *sh) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -41,7 +42,7 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
     __syncthreads();
     double s = 0.0;
 #pragma unroll
-    for (int i = 0; i < NT / 64; ++i) s += sh[i];
+    for (int i = 0; i < NTH / 64; ++i) s += sh[i];
     __syncthreads();
     return s;
 }
@@ -58,13 +59,13 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
 // ---------------------------------------------------------------------------
 constexpr int NORM_BLOCKS = 4;
 
-template <int CPL, class Amp>
+template <int CPL, int NTH = NT, class Amp>
 __device__ void window_norms(Amp amp, int SW, int TH, int TW, int NLh, int NLw, double t2, float *nrm,
                              double *part) {
     constexpr int NB = NORM_BLOCKS;
     const int rows_per = (TH + NB - 1) / NB;
     #pragma unroll 1
-    for (int t = threadIdx.x; t < NB * SW; t += NT) {
+    for (int t = threadIdx.x; t < NB * SW; t += NTH) {
         const int b = t / SW, c = t - b * SW;
         const int r0 = b * rows_per, r1 = min(TH, r0 + rows_per);
         double a1 = 0.0, a2 = 0.0, b1 = 0.0, b2 = 0.0;
@@ -83,7 +84,7 @@ __device__ void window_norms(Amp amp, int SW, int TH, int TW, int NLh, int NLw, 
     }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    constexpr int NW = NT / 64;
+    constexpr int NW = NTH / 64;
     const int per = (NLh + NW - 1) / NW;
     const int u_beg = wave * per, u_end = min(NLh, u_beg + per);
     if (u_beg >= u_end) return;
@@ -163,10 +164,10 @@ __device__ void window_norms(Amp amp, int SW, int TH, int TW, int NLh, int NLw, 
 // slot 0 = (DC, Nyquist) of that row. A round covers whole signals only: the
 // reads of signal j (bins k and L-k) must all happen before any write to j.
 // ---------------------------------------------------------------------------
-template <int R1, int R2>
+template <int R1, int R2, int NTH = NT>
 __device__ void split_pairs_inplace(cf *buf, int nsig, int P) {
     constexpr int L = R1 * R2, H = L / 2, U = 4;
-    constexpr int SIG_PER_ROUND = (NT * U) / H;
+    constexpr int SIG_PER_ROUND = (NTH * U) / H;
     static_assert(SIG_PER_ROUND >= 1, "a round must hold at least one signal");
     const int total = nsig * H;
     #pragma unroll 1
@@ -176,7 +177,7 @@ __device__ void split_pairs_inplace(cf *buf, int nsig, int P) {
         cf av[U], bv[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int it = base + u * NT + threadIdx.x;
+            const int it = base + u * NTH + threadIdx.x;
             if (it < end) {
                 const int j = it / H, k = it - j * H;
                 const cf *z = buf + j * P;
@@ -195,7 +196,7 @@ __device__ void split_pairs_inplace(cf *buf, int nsig, int P) {
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int it = base + u * NT + threadIdx.x;
+            const int it = base + u * NTH + threadIdx.x;
             if (it < end) {
                 const int j = it / H, k = it - j * H;
                 buf[j * P + k] = av[u];
@@ -251,12 +252,12 @@ __device__ __forceinline__ void mac_rows(cf (&q)[K], const cf *S, int Sp, const 
 // Inverse of the accumulated Q[u][f] (Qs: [NLh][H]) into corr[u][v] * nrm[u][v]
 // for v < NLw: lag pairs (2p, 2p+1) share one complex transform placed in Y
 // (npair signals, pitch P, digit-reversed input order).
-template <int R1, int R2>
+template <int R1, int R2, int NTH = NT>
 __device__ void inverse_lags(const cf *Qs, int NLh, int NLw, cf *Y, int P, const cx<float> *tw, const float *nrm,
                              float *corr) {
     constexpr int L = R1 * R2, H = L / 2;
     const int npair = (NLh + 1) / 2;
-    for (int i = threadIdx.x; i < npair * L; i += NT) {
+    for (int i = threadIdx.x; i < npair * L; i += NTH) {
         const int p = i / L, k = i - p * L;
         const int ua = 2 * p, ub = ua + 1;
         const cf *qa = Qs + ua * H;
@@ -275,9 +276,9 @@ __device__ void inverse_lags(const cf *Qs, int NLh, int NLw, cf *Y, int P, const
         Y[p * P + dr_pos<R1, R2>(k)] = y;
     }
     __syncthreads();
-    fft_dr2nat<float, +1, R1, R2, NT>(Y, npair, P, 1, tw);
+    fft_dr2nat<float, +1, R1, R2, NTH>(Y, npair, P, 1, tw);
     const float invL = 1.0f / float(L);
-    for (int i = threadIdx.x; i < NLh * NLw; i += NT) {
+    for (int i = threadIdx.x; i < NLh * NLw; i += NTH) {
         const int u = i / NLw, v = i - u * NLw;
         const cf y = Y[(u >> 1) * P + v];
         corr[i] = ((u & 1) ? y.y : y.x) * invL * nrm[i];

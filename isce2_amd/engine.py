@@ -132,6 +132,11 @@ class Engine:
         check(self.lib.ampcor_set_precision(self._h, int(mode), float(tau_coarse), float(tau_fine)), self._h)
         self.precision = int(mode)
 
+    def set_option(self, name: str, value: int):
+        """Launch-shape option (``fine_threads``: 512 or 1024)."""
+        check(self.lib.ampcor_set_option(self._h, name.encode(), int(value)), self._h)
+        self.options = dict(getattr(self, "options", {}), **{name: int(value)})
+
     def set_fft_radices(self, n: int, radices):
         """Radix schedule of the staged pipeline's length-n FFTs (autotuning)."""
         arr = (C.c_int32 * len(radices))(*[int(r) for r in radices])
