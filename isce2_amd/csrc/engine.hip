@@ -252,7 +252,26 @@ void Engine::run_staged_t(const RunArgs &a) {
     T *r_sec = buf<T>(tn + "r_sec", size_t(nb) * S_h * S_w);
     const size_t zmax = std::max(size_t(L1h) * L1w, size_t(L2h) * L2w);
     cx<T> *zc = buf<cx<T>>(tn + "zc", size_t(nb) * zmax);
-    auto norm_lds = [](int sw) { return size_t(12) * sw * sizeof(double) + 32; };
+    auto norm_lds = [](int sw) { return size_t(4 * sw + 2) * sizeof(double); };
+    // block size of the one-block-per-window reductions
+    auto per_window_threads = [](int size) { return size >= 16384 ? 1024 : 256; };
+    double *ssd_raw = buf<double>(tn + "ssd_raw", nb), *ssd_os = buf<double>(tn + "ssd_os", nb);
+    const size_t vmax = std::max(size_t(sz.raw_corr_h) * S_w, size_t(sz.os_corr_h) * sz.os_search_w);
+    double *V1 = buf<double>(tn + "V1", size_t(nb) * vmax), *V2 = buf<double>(tn + "V2", size_t(nb) * vmax);
+    // NCC normalization: barrier-free column/row passes up to 1024 search columns, else one block per window
+    auto normalize = [&](T *c, int ch, int cw, const T *tm, int th, int tw, const T *sr, int sh, int sw,
+                         const double *ssd, int nw) {
+        if (sw <= 1024) {
+            hipLaunchKernelGGL(k_colsums<T>, dim3(idivup(sw, 64), nw), dim3(64), 0, st, sr, sh, sw, th, ch, V1, V2);
+            const dim3 g(idivup(nw * ch, 4)), b(256);
+            if (sw <= 128) hipLaunchKernelGGL((k_norm_rows<T, 2>), g, b, 0, st, c, nw, ch, cw, th, tw, V1, V2, sw, ssd);
+            else if (sw <= 256) hipLaunchKernelGGL((k_norm_rows<T, 4>), g, b, 0, st, c, nw, ch, cw, th, tw, V1, V2, sw, ssd);
+            else if (sw <= 576) hipLaunchKernelGGL((k_norm_rows<T, 9>), g, b, 0, st, c, nw, ch, cw, th, tw, V1, V2, sw, ssd);
+            else hipLaunchKernelGGL((k_norm_rows<T, 16>), g, b, 0, st, c, nw, ch, cw, th, tw, V1, V2, sw, ssd);
+        } else {
+            hipLaunchKernelGGL(k_normalize<T>, dim3(nw), dim3(256), norm_lds(sw), st, c, ch, cw, tm, th, tw, sr, sh, sw);
+        }
+    };
     T *corr = buf<T>(tn + "corr", size_t(nb) * sz.raw_corr_h * sz.raw_corr_w);
     int32_t *loc = buf<int32_t>(tn + "loc", size_t(nb) * 2);
     T *val = buf<T>(tn + "val", nb);
@@ -281,7 +300,8 @@ void Engine::run_staged_t(const RunArgs &a) {
                            a.ref_d + w0, a.ref_a + w0, W_h, W_w, mode, c_ref);
         hipLaunchKernelGGL(k_abs<T>, dim3(idivup(n * W_h * W_w, 256)), dim3(256), 0, st, c_ref, r_ref,
                            int64_t(n) * W_h * W_w);
-        hipLaunchKernelGGL(k_sub_mean<T>, dim3(n), dim3(256), 0, st, r_ref, W_h * W_w);
+        hipLaunchKernelGGL(k_sub_mean<T>, dim3(n), dim3(per_window_threads(W_h * W_w)), 0, st, r_ref, W_h * W_w,
+                           ssd_raw);
         hipLaunchKernelGGL(k_gather<T>, grid2(S_h * S_w, n), dim3(256), 0, st, a.sec, a.sec_pitch,
                            a.sec_d + w0, a.sec_a + w0, S_h, S_w, mode, c_sec);
         hipLaunchKernelGGL(k_abs<T>, dim3(idivup(n * S_h * S_w, 256)), dim3(256), 0, st, c_sec, r_sec,
@@ -296,8 +316,7 @@ void Engine::run_staged_t(const RunArgs &a) {
         hipLaunchKernelGGL(k_crop_real<T>, grid2(sz.raw_corr_h * sz.raw_corr_w, n), dim3(256), 0, st, zc, L1h, L1w,
                            corr, sz.raw_corr_h, sz.raw_corr_w);
         // normalize (cuAmpcorChunk.cpp:60)
-        hipLaunchKernelGGL(k_normalize<T>, dim3(n), dim3(256), norm_lds(S_w), st, corr, sz.raw_corr_h,
-                           sz.raw_corr_w, r_ref, W_h, W_w, r_sec, S_h, S_w);
+        normalize(corr, sz.raw_corr_h, sz.raw_corr_w, r_ref, W_h, W_w, r_sec, S_h, S_w, ssd_raw, n);
         // argmax, covariance, SNR, zoom start/shift (cuAmpcorChunk.cpp:69-106)
         hipLaunchKernelGGL(k_maxloc<T>, dim3(n), dim3(256), 0, st, corr, sz.raw_corr_h, sz.raw_corr_w, loc, val,
                            a.margin_coarse ? a.margin_coarse + w0 : nullptr);
@@ -306,20 +325,22 @@ void Engine::run_staged_t(const RunArgs &a) {
                            sz.half_zoom, a.cov + 3 * size_t(w0), a.snr + w0, start, shift);
         // reference oversampling (cuAmpcorChunk.cpp:115-126)
         if (p.deramp == 1)
-            hipLaunchKernelGGL(k_deramp<T>, dim3(n), dim3(256), size_t(W_h + W_w) * 16, st, c_ref, W_h, W_w);
+            hipLaunchKernelGGL(k_deramp<T>, dim3(n), dim3(per_window_threads(W_h * W_w)), size_t(W_h + W_w) * 16, st,
+                               c_ref, W_h, W_w);
         fft2<T, +1>(*this, c_ref, n, W_h, W_w, st);
         hipLaunchKernelGGL(k_quadpad<T>, grid2(sz.os_win_h * sz.os_win_w, n), dim3(256), 0, st, c_ref, W_h, W_w,
                            c_ref_os, sz.os_win_h, sz.os_win_w);
         fft2<T, -1>(*this, c_ref_os, n, sz.os_win_h, sz.os_win_w, st);
         hipLaunchKernelGGL(k_abs<T>, dim3(idivup(n * sz.os_win_h * sz.os_win_w, 256)), dim3(256), 0, st, c_ref_os,
                            r_ref_os, int64_t(n) * sz.os_win_h * sz.os_win_w);
-        hipLaunchKernelGGL(k_sub_mean<T>, dim3(n), dim3(256), 0, st, r_ref_os, sz.os_win_h * sz.os_win_w);
+        hipLaunchKernelGGL(k_sub_mean<T>, dim3(n), dim3(per_window_threads(sz.os_win_h * sz.os_win_w)), 0, st,
+                           r_ref_os, sz.os_win_h * sz.os_win_w, ssd_os);
         // secondary zoom-in crop and oversampling (cuAmpcorChunk.cpp:134-137)
         hipLaunchKernelGGL(k_crop_start<T>, grid2(sz.zoom_in_h * sz.zoom_in_w, n), dim3(256), 0, st, c_sec, S_h,
                            S_w, start, c_zoom, sz.zoom_in_h, sz.zoom_in_w);
         if (p.deramp == 1)
-            hipLaunchKernelGGL(k_deramp<T>, dim3(n), dim3(256), size_t(sz.zoom_in_h + sz.zoom_in_w) * 16, st, c_zoom,
-                               sz.zoom_in_h, sz.zoom_in_w);
+            hipLaunchKernelGGL(k_deramp<T>, dim3(n), dim3(per_window_threads(sz.zoom_in_h * sz.zoom_in_w)),
+                               size_t(sz.zoom_in_h + sz.zoom_in_w) * 16, st, c_zoom, sz.zoom_in_h, sz.zoom_in_w);
         fft2<T, +1>(*this, c_zoom, n, sz.zoom_in_h, sz.zoom_in_w, st);
         hipLaunchKernelGGL(k_quadpad<T>, grid2(OSh * OSw, n), dim3(256), 0, st, c_zoom, sz.zoom_in_h, sz.zoom_in_w,
                            c_sec_os, OSh, OSw);
@@ -335,8 +356,8 @@ void Engine::run_staged_t(const RunArgs &a) {
         fft2<T, +1>(*this, zc, n, L2h, L2w, st);
         hipLaunchKernelGGL(k_crop_real<T>, grid2(sz.os_corr_h * sz.os_corr_w, n), dim3(256), 0, st, zc, L2h, L2w,
                            corr_os, sz.os_corr_h, sz.os_corr_w);
-        hipLaunchKernelGGL(k_normalize<T>, dim3(n), dim3(256), norm_lds(OSw), st, corr_os, sz.os_corr_h,
-                           sz.os_corr_w, r_ref_os, sz.os_win_h, sz.os_win_w, r_sec_os, OSh, OSw);
+        normalize(corr_os, sz.os_corr_h, sz.os_corr_w, r_ref_os, sz.os_win_h, sz.os_win_w, r_sec_os, OSh, OSw,
+                  ssd_os, n);
         // trim + surface oversampling + argmax + sub-pixel offset (cuAmpcorChunk.cpp:169-207)
         float *mf = a.margin_fine ? a.margin_fine + w0 : nullptr;
         const float ratio = 1.0f / float(p.surf_ovs * p.raw_ovs);

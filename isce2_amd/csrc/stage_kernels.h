@@ -83,6 +83,7 @@ __global__ void k_abs(const cx<T> *in, T *out, int64_t total) {
 }
 
 // block-wide FP64 sum (256 threads)
+// block sum for blocks of up to 64 * (size of sh) threads
 __device__ __forceinline__ double block_sum256(double v, double *sh) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -99,14 +100,23 @@ __device__ __forceinline__ double block_sum256(double v, double *sh) {
 
 // mean subtraction per image, FP64 sum (cuCorrNormalization.cu:210-235)
 template <class T>
-__global__ void __launch_bounds__(256) k_sub_mean(T *data, int size) {
-    __shared__ double sh[8];
+__global__ void __launch_bounds__(1024) k_sub_mean(T *data, int size, double *ssd) {
+    __shared__ double sh[16];
     T *img = data + size_t(blockIdx.x) * size;
-    double s = 0.0;
-    for (int i = threadIdx.x; i < size; i += blockDim.x) s += double(img[i]);
-    s = block_sum256(s, sh);
-    const T mean = T(s / size);
+    double s1 = 0.0, s2 = 0.0;
+    for (int i = threadIdx.x; i < size; i += blockDim.x) {
+        const double v = double(img[i]);
+        s1 += v;
+        s2 += v * v;
+    }
+    s1 = block_sum256(s1, sh);
+    const T mean = T(s1 / size);
     for (int i = threadIdx.x; i < size; i += blockDim.x) img[i] -= mean;
+    if (ssd) {
+        // sum of squares of the mean-subtracted image (the normalization's template term)
+        s2 = block_sum256(s2, sh);
+        if (threadIdx.x == 0) ssd[blockIdx.x] = fmax(s2 - s1 * s1 / size, 0.0);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -166,7 +176,8 @@ __global__ void k_crop_real(const cx<T> *z, int Lh, int Lw, T *out, int oh, int 
 // cuCorrNormalizationSAT.cu:237-240) with FP64 window sums computed directly:
 // per-column vertical sums over the template height, slid down one lag row at
 // a time, and horizontal window sums from a block prefix scan of them. One
-// 256-thread block per window; dynamic LDS 12*sw doubles.
+// 256-thread block per window; dynamic LDS 4*sw + 2 doubles. Used for search
+// chips wider than the two-pass path below handles.
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void block_scan256(double &x1, double &x2, double *sh4) {
     // inclusive scan of (x1, x2) over the 256 threads; sh4 holds 8 doubles
@@ -185,14 +196,12 @@ __device__ __forceinline__ void block_scan256(double &x1, double &x2, double *sh
 template <class T>
 __global__ void __launch_bounds__(256) k_normalize(T *corr, int ch, int cw, const T *tmpl, int th, int tw,
                                                    const T *srch, int sh, int sw) {
-    constexpr int NB = 4;
     __shared__ double sh_red[8];
     extern __shared__ __align__(16) unsigned char smem_raw[];
     double *V1 = reinterpret_cast<double *>(smem_raw);
     double *V2 = V1 + sw;
     double *P1 = V2 + sw;            // [sw + 1] prefix sums
     double *P2 = P1 + sw + 1;
-    double *part = P2 + sw + 1;      // [2][NB][sw]
     const int win = blockIdx.x;
     const T *t = tmpl + size_t(win) * th * tw;
     const T *s = srch + size_t(win) * sh * sw;
@@ -202,23 +211,13 @@ __global__ void __launch_bounds__(256) k_normalize(T *corr, int ch, int cw, cons
         t2 += v * v;
     }
     t2 = block_sum256(t2, sh_red);
-    const int rows_per = (th + NB - 1) / NB;
-    for (int task = threadIdx.x; task < NB * sw; task += blockDim.x) {
-        const int b = task / sw, c = task - b * sw;
-        const int r1 = min(th, (b + 1) * rows_per);
+    for (int c = threadIdx.x; c < sw; c += blockDim.x) {
         double a1 = 0.0, a2 = 0.0;
-        for (int r = b * rows_per; r < r1; ++r) {
+        for (int r = 0; r < th; ++r) {
             const double v = double(s[size_t(r) * sw + c]);
             a1 += v;
             a2 += v * v;
         }
-        part[b * sw + c] = a1;
-        part[(NB + b) * sw + c] = a2;
-    }
-    __syncthreads();
-    for (int c = threadIdx.x; c < sw; c += blockDim.x) {
-        double a1 = 0.0, a2 = 0.0;
-        for (int b = 0; b < NB; ++b) { a1 += part[b * sw + c]; a2 += part[(NB + b) * sw + c]; }
         V1[c] = a1;
         V2[c] = a2;
     }
@@ -231,7 +230,7 @@ __global__ void __launch_bounds__(256) k_normalize(T *corr, int ch, int cw, cons
     for (int u = 0; u < ch; ++u) {
         double x1 = 0.0, x2 = 0.0;
         for (int c = c0; c < c1; ++c) { x1 += V1[c]; x2 += V2[c]; }
-        block_scan256(x1, x2, part);                          // part is free now
+        block_scan256(x1, x2, sh_red);
         double e1 = x1, e2 = x2;                              // inclusive end of this thread's segment
         for (int c = c1 - 1; c >= c0; --c) {
             P1[c + 1] = e1; P2[c + 1] = e2;
@@ -251,6 +250,94 @@ __global__ void __launch_bounds__(256) k_normalize(T *corr, int ch, int cw, cons
             }
         }
         __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// normalization in two barrier-free passes (same formula and FP64 sums as
+// k_normalize): k_colsums slides the vertical window sums of s and s^2 down
+// every column (one thread per column) into V[w][u][c]; k_norm_rows takes one
+// lag row per wave, forms the horizontal window sums from a wave prefix scan of
+// V and scales the correlation row. ssd[w] is the template's sum of squared
+// deviations (k_sub_mean).
+// ---------------------------------------------------------------------------
+template <class T>
+__global__ void __launch_bounds__(64) k_colsums(const T *srch, int sh, int sw, int th, int nl, double *V1,
+                                                double *V2) {
+    const int w = blockIdx.y, c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= sw) return;
+    const T *s = srch + size_t(w) * sh * sw + c;
+    double b1[4] = {0.0, 0.0, 0.0, 0.0}, b2[4] = {0.0, 0.0, 0.0, 0.0};
+    int r = 0;
+    for (; r + 4 <= th; r += 4) {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const double v = double(s[size_t(r + k) * sw]);
+            b1[k] += v;
+            b2[k] += v * v;
+        }
+    }
+    for (; r < th; ++r) {
+        const double v = double(s[size_t(r) * sw]);
+        b1[0] += v;
+        b2[0] += v * v;
+    }
+    double a1 = (b1[0] + b1[1]) + (b1[2] + b1[3]), a2 = (b2[0] + b2[1]) + (b2[2] + b2[3]);
+    double *o1 = V1 + size_t(w) * nl * sw + c, *o2 = V2 + size_t(w) * nl * sw + c;
+    o1[0] = a1;
+    o2[0] = a2;
+    for (int u = 1; u < nl; ++u) {
+        const double vn = double(s[size_t(u + th - 1) * sw]), vo = double(s[size_t(u - 1) * sw]);
+        a1 += vn - vo;
+        a2 += vn * vn - vo * vo;
+        o1[size_t(u) * sw] = a1;
+        o2[size_t(u) * sw] = a2;
+    }
+}
+
+template <class T, int CPL>
+__global__ void __launch_bounds__(256) k_norm_rows(T *corr, int n, int ch, int cw, int th, int tw, const double *V1,
+                                                   const double *V2, int sw, const double *ssd) {
+    const int lane = threadIdx.x & 63;
+    const int g = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const int w = g / ch, u = g - w * ch;
+    if (w >= n) return;
+    const double *v1 = V1 + (size_t(w) * ch + u) * sw, *v2 = V2 + (size_t(w) * ch + u) * sw;
+    // inclusive prefix over columns: p[m] at column lane + 64 m
+    double p1[CPL], p2[CPL];
+    double c1 = 0.0, c2 = 0.0;
+#pragma unroll
+    for (int m = 0; m < CPL; ++m) {
+        const int c = lane + 64 * m;
+        double x1 = c < sw ? v1[c] : 0.0, x2 = c < sw ? v2[c] : 0.0;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const double y1 = __shfl_up(x1, d), y2 = __shfl_up(x2, d);
+            if (lane >= d) { x1 += y1; x2 += y2; }
+        }
+        x1 += c1; x2 += c2;
+        p1[m] = x1; p2[m] = x2;
+        c1 = __shfl(x1, 63);
+        c2 = __shfl(x2, 63);
+    }
+    const double t2 = ssd[w], inv = 1.0 / double(th * tw);
+    T *cr = corr + (size_t(w) * ch + u) * cw;
+    for (int k = 0; k * 64 < cw; ++k) {
+        const int v = lane + 64 * k;
+        const int hi = min(v + tw - 1, sw - 1), lo = v - 1;
+        double ph1 = 0, ph2 = 0, pl1 = 0, pl2 = 0;
+#pragma unroll
+        for (int m = 0; m < CPL; ++m) {
+            const double a1 = __shfl(p1[m], hi & 63), a2 = __shfl(p2[m], hi & 63);
+            const double b1 = __shfl(p1[m], lo & 63), b2 = __shfl(p2[m], lo & 63);
+            if ((hi >> 6) == m) { ph1 = a1; ph2 = a2; }
+            if (lo >= 0 && (lo >> 6) == m) { pl1 = b1; pl2 = b2; }
+        }
+        if (v < cw) {
+            const double s1 = ph1 - pl1, s2 = ph2 - pl2;
+            const double norm2 = (s2 - s1 * s1 * inv) * t2;
+            cr[v] = T(double(cr[v]) / sqrt(norm2 + double(FLT_EPSILON)));
+        }
     }
 }
 
@@ -348,8 +435,8 @@ __global__ void k_stats(const T *corr, int nx, int ny, int n, const int32_t *loc
 // part reuses the updated real part); sums, phases and rotation in FP64
 // ---------------------------------------------------------------------------
 template <class T>
-__global__ void __launch_bounds__(256) k_deramp(cx<T> *chips, int h, int w) {
-    __shared__ double sh[8];
+__global__ void __launch_bounds__(1024) k_deramp(cx<T> *chips, int h, int w) {
+    __shared__ double sh[16];
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<double> *rowf = reinterpret_cast<cx<double> *>(smem_raw);   // e^{i r phy}
     cx<double> *colf = rowf + h;                                   // e^{i c phx}
