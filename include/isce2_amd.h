@@ -149,7 +149,9 @@ int ampcor_set_fft_radices(ampcor_ctx *ctx, int32_t n, const int32_t *radices, i
 
 /* Launch-shape options, for autotuning. No counterpart in the reference.
  *   "fine_threads": 512 or 1024 threads per workgroup of the window-64 fused
- *                   fine kernel (default 1024). */
+ *                   fine kernel (default 1024).
+ *   "fft_block_elems": 1024, 2048, 4096 (default) or 8192 complex elements per
+ *                   workgroup of the staged pipeline's row/column FFT kernels. */
 int ampcor_set_option(ampcor_ctx *ctx, const char *name, int64_t value);
 
 /* Per-kernel timing with HIP events recorded on the launch stream around every

@@ -146,6 +146,13 @@ class Autotuner:
             if best.mode == MODE_STAGED:
                 m = min(m, self.radix_sample)
                 sub = list(args[:2]) + [t[:m].contiguous() for t in args[2:]]
+                times = {}
+                for fe in (2048, 4096, 8192):
+                    eng.set_option("fft_block_elems", fe)
+                    times[fe] = self._time(eng, sub, MODE_STAGED, min(best.chunk, m), m)
+                fe = min(times, key=times.get)
+                eng.set_option("fft_block_elems", fe)
+                best.options["fft_block_elems"] = fe
                 for n_fft in staged_lengths(sizes, params):
                     orders = factorizations(n_fft)
                     if len(orders) < 2:

@@ -114,6 +114,8 @@ struct Engine {
     // on a 31k-window speckle field every FP32/FP64 flip had a margin <= 6e-8
     // launch-shape options (ampcor_set_option): threads per workgroup of the window-64 fine kernel
     int fine_threads = 1024;
+    // complex elements per workgroup of the staged FFT kernels (float; double uses half)
+    int fft_block_elems = 4096;
     int precision = 1;
     float tau_coarse = 2e-6f, tau_fine = 2e-6f;
     long long prec_checked = 0, prec_rerun = 0;

@@ -199,12 +199,12 @@ void Engine::ensure_sinc_table() {
 // staged pipeline: launch helpers
 // ---------------------------------------------------------------------------
 namespace {
-constexpr int kElemPerBlock = 4096;   // complex elements of one FFT block (x2 buffers in LDS)
-
+// elems: complex elements one FFT block holds (x2 buffers in LDS), the
+// "fft_block_elems" launch option
 template <class T, int DIR>
-void fft_rows(const FftPlan &pl, cx<T> *data, int64_t nrows, hipStream_t st) {
+void fft_rows(const FftPlan &pl, cx<T> *data, int64_t nrows, int elems, hipStream_t st) {
     const FftDesc &d = std::is_same<T, float>::value ? pl.desc32 : pl.desc64;
-    const int g = std::max(1, kElemPerBlock / pl.n);
+    const int g = std::max(1, elems / pl.n);
     const size_t lds = 2 * size_t(g) * pl.n * sizeof(cx<T>);
     const int64_t nblk = (nrows + g - 1) / g;
     for (int64_t b0 = 0; b0 < nblk; b0 += 1 << 30) {
@@ -215,9 +215,9 @@ void fft_rows(const FftPlan &pl, cx<T> *data, int64_t nrows, hipStream_t st) {
 }
 
 template <class T, int DIR>
-void fft_cols(const FftPlan &pl, cx<T> *data, int nimg, int w, hipStream_t st) {
+void fft_cols(const FftPlan &pl, cx<T> *data, int nimg, int w, int elems, hipStream_t st) {
     const FftDesc &d = std::is_same<T, float>::value ? pl.desc32 : pl.desc64;
-    const int c = std::max(1, std::min(w, std::min(16, kElemPerBlock / pl.n)));
+    const int c = std::max(1, std::min(w, std::min(16, elems / pl.n)));
     const size_t lds = 2 * size_t(c) * pl.n * sizeof(cx<T>);
     hipLaunchKernelGGL((k_fft_cols<T, DIR>), dim3(idivup(w, c), nimg), dim3(256), lds, st, data, w, c, d);
 }
@@ -225,8 +225,10 @@ void fft_cols(const FftPlan &pl, cx<T> *data, int nimg, int w, hipStream_t st) {
 // 2D FFT of nimg images h x w (rows then columns), in place
 template <class T, int DIR>
 void fft2(Engine &e, cx<T> *data, int nimg, int h, int w, hipStream_t st) {
-    fft_rows<T, DIR>(e.plan(w), data, int64_t(nimg) * h, st);
-    fft_cols<T, DIR>(e.plan(h), data, nimg, w, st);
+    // double elements take twice the LDS: halve the element budget
+    const int elems = sizeof(T) == 8 ? std::max(1024, e.fft_block_elems / 2) : e.fft_block_elems;
+    fft_rows<T, DIR>(e.plan(w), data, int64_t(nimg) * h, elems, st);
+    fft_cols<T, DIR>(e.plan(h), data, nimg, w, elems, st);
 }
 
 inline dim3 grid2(int per_img, int nimg) { return dim3(std::min(idivup(per_img, 256), 64), nimg); }
@@ -650,7 +652,11 @@ int ampcor_set_option(ampcor_ctx *ctx, const char *name, int64_t value) {
     if (!ctx || !name) return AMPCOR_E_ARG;
     return guarded(ctx, [&] {
         const std::string k(name);
-        if (k == "fine_threads") {
+        if (k == "fft_block_elems") {
+            if (value != 1024 && value != 2048 && value != 4096 && value != 8192)
+                throw ampcor::Error(AMPCOR_E_ARG, "fft_block_elems must be 1024, 2048, 4096 or 8192");
+            ctx->eng.fft_block_elems = int(value);
+        } else if (k == "fine_threads") {
             if (value != 512 && value != 1024) throw ampcor::Error(AMPCOR_E_ARG, "fine_threads must be 512 or 1024");
             ctx->eng.fine_threads = int(value);
         } else {
