@@ -169,3 +169,35 @@ def test_staged_chunk_invariance(eng, scene1):
     torch.cuda.synchronize()
     for k in ("offset", "snr", "cov"):
         assert torch.equal(a[k], b[k]), k
+
+
+def test_launch_options_do_not_change_results(eng, scene1):
+    """fine_threads (fused) and fft_block_elems (staged) change only launch shapes."""
+    ref, sec = scene1
+    cfg = O.AmpcorConfig(hs_down=20, hs_across=20)
+    starts = _grid(5, 5, 20, 80, 20)
+    from .gpu_helpers import params_from_cfg
+    eng.setup(params_from_cfg(cfg))
+    dev = torch.device("cuda", 0)
+    rt, stt = torch.from_numpy(ref).to(dev), torch.from_numpy(sec).to(dev)
+    st = [torch.as_tensor(np.asarray(s, np.int32)).to(dev) for s in starts]
+    res = {}
+    for mode, name, vals in ((MODE_FUSED, "fine_threads", (512, 1024)),
+                             (MODE_STAGED, "fft_block_elems", (1024, 2048, 4096, 8192))):
+        for v in vals:
+            eng.set_option(name, v)
+            r = eng.run_windows(rt, stt, *st, mode=mode, debug=True)
+            torch.cuda.synchronize()
+            res[(name, v)] = {k: t.cpu().numpy() for k, t in r.items()}
+        eng.set_option(name, vals[-1] if name == "fft_block_elems" else 1024)
+    eng.set_option("fft_block_elems", 4096)
+    base_f, base_s = res[("fine_threads", 1024)], res[("fft_block_elems", 4096)]
+    for (name, v), r in res.items():
+        base = base_f if name == "fine_threads" else base_s
+        for k in ("maxloc", "zoom_idx", "offset"):
+            assert np.array_equal(r[k], base[k]), (name, v, k)
+        np.testing.assert_allclose(r["snr"], base["snr"], rtol=1e-5)
+    with pytest.raises(ValueError):
+        eng.set_option("fine_threads", 256)
+    with pytest.raises(ValueError):
+        eng.set_option("no_such_option", 1)
