@@ -172,7 +172,7 @@ struct OsArgs {
     const int32_t *sd, *sa, *start;   // start may be null
     int deramp;
     const cf *hsh;                    // [N] e^{-i pi k_s/N} / N
-    float *out;                       // [n][2N][2N]
+    float *out;                       // [n][4][N][N]: pixel (2r+p, 2c+q) in plane 2p+q at (r, c)
     double *mean;                     // [n][2]: mean and sum of squared deviations of |.|, or null
 };
 
@@ -247,7 +247,7 @@ __global__ void __launch_bounds__(NT) k_oversample(OsArgs a, const cf *tw, int w
             const int r = i / N, c = i - r * N;
             const cf v = C[r * PC + c];
             const float amp = sqrtf(v.x * v.x + v.y * v.y);
-            out[size_t(2 * r + p) * 2 * N + 2 * c + q] = amp;
+            out[size_t(2 * p + q) * N * N + r * N + c] = amp;     // whole lines per phase
             sum += amp;
             sum2 += double(amp) * amp;
         }
@@ -330,8 +330,8 @@ struct FineLds {
 };
 
 struct FineArgs {
-    const float *tamp; const double *tmean;   // [n][T][T], [n][2] (mean, sum of squared deviations)
-    const float *samp;                         // [n][P][P]
+    const float *tamp; const double *tmean;   // [n][4][T/2][T/2] planar, [n][2] (mean, sum of squared deviations)
+    const float *samp;                         // [n][4][P/2][P/2] planar (k_oversample's layout)
     const int32_t *start;
     int hs_d, hs_a;
     float ratio;                               // 1/(surf_ovs*raw_ovs)
@@ -361,7 +361,9 @@ __global__ void __launch_bounds__(NTH) k_fine(FineArgs a, const cf *tw, int w0) 
 #pragma unroll 4
     for (int i = threadIdx.x; i < nS * L; i += NTH) {
         const int j = i / L, x = i - j * L;
-        S[j * PP + x] = {sg[(2 * j) * P + x], sg[(2 * j + 1) * P + x]};
+        // planar oversampled chip: row 2j + p, column x -> plane 2p + (x & 1), (j, x >> 1)
+        const float *s0 = sg + size_t(x & 1) * (P / 2) * (P / 2) + j * (P / 2) + (x >> 1);
+        S[j * PP + x] = {s0[0], s0[size_t(2) * (P / 2) * (P / 2)]};
     }
     __syncthreads();
     window_norms<(P + 63) / 64, NTH>(
@@ -377,8 +379,9 @@ __global__ void __launch_bounds__(NTH) k_fine(FineArgs a, const cf *tw, int w0) 
         const int j = i / L, x = i - j * L;
         cf v = {0.f, 0.f};
         if (x < T) {
-            v.x = float(double(tp[(2 * j) * T + x]) - mean);
-            v.y = float(double(tp[(2 * j + 1) * T + x]) - mean);
+            const float *t0 = tp + size_t(x & 1) * (T / 2) * (T / 2) + j * (T / 2) + (x >> 1);
+            v.x = float(double(t0[0]) - mean);
+            v.y = float(double(t0[size_t(2) * (T / 2) * (T / 2)]) - mean);
         }
         Tb[j * PP + x] = v;
     }
@@ -460,11 +463,18 @@ __global__ void __launch_bounds__(NTH) k_fine(FineArgs a, const cf *tw, int w0) 
         const int qq = threadIdx.x & (SO - 1);
         #pragma unroll 1
         for (int b = threadIdx.x / SO; b < 16; b += NTH / SO) {
-            cf x[16];
+            // e^{-2 pi i b k/256} for k = 1..8 (negative bins use the conjugate);
+            // b is uniform over the wave, so these are scalar loads into SGPRs
+            const int bu = __builtin_amdgcn_readfirstlane(b);
+            cf w[9];
 #pragma unroll
-            for (int k1 = 0; k1 < 16; ++k1) {
-                const int ks = k1 < 8 ? k1 : k1 - 16;
-                x[k1] = cmul(G[k1 * SO + qq], tws[(b * ks + 4096) & 255]);
+            for (int k = 1; k <= 8; ++k) w[k] = a.tw256[(bu * k) & 255];
+            cf x[16];
+            x[0] = G[qq];
+#pragma unroll
+            for (int k1 = 1; k1 < 16; ++k1) {
+                const cf t = k1 < 8 ? w[k1] : conj(w[16 - k1]);
+                x[k1] = cmul(G[k1 * SO + qq], t);
             }
             dft16<float, -1>(x);
 #pragma unroll
