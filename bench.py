@@ -27,6 +27,11 @@ sys.path.insert(0, ROOT)
 MFLOP_PER_WINDOW = 12.364
 KERNEL_MFLOP = {"k_coarse": 1.235, "k_oversample_ref": 1.434, "k_oversample_search": 1.817, "k_fine": 7.878}
 FP32_PEAK_TFLOPS = 157.3
+# HBM bytes per full 2048-window launch of each fused kernel, from rocprofv3 FETCH_SIZE + WRITE_SIZE
+# (KiB per dispatch) scaled by a 1 GiB device copy's counts (FETCH x2, WRITE x1); re-measure after kernel changes
+HBM_BYTES_PER_LAUNCH = {"k_fine": (148940 * 2 + 192) * 1024, "k_coarse": (133033 * 2 + 49616) * 1024,
+                        "k_oversample_ref": (81388 * 2 + 131136) * 1024,
+                        "k_oversample_search": (85076 * 2 + 165888) * 1024}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -138,7 +143,8 @@ def main():
         avg_s = v["ms"] / v["launches"] * 1e-3
         achieved = KERNEL_MFLOP[dom] * 1e6 * (v["windows"] / v["launches"]) / avg_s / 1e12
         roof = {"bound": "valu", "kernel": dom, "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
-                "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
+                "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
+                "traffic": HBM_BYTES_PER_LAUNCH.get(dom),
                 "per_unit": f"{KERNEL_MFLOP[dom]} MFLOP/window (reference-size FFT model)",
                 "windows_per_launch": v["windows"] // v["launches"], "avg_launch_ms": round(v["ms"] / v["launches"], 4)}
 
