@@ -145,6 +145,7 @@ def main():
         roof = {"bound": "valu", "kernel": dom, "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / FP32_PEAK_TFLOPS, 4),
                 "traffic": HBM_BYTES_PER_LAUNCH.get(dom),
+                "traffic_note": "HBM bytes of a full 2048-window launch (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-scaled)",
                 "per_unit": f"{KERNEL_MFLOP[dom]} MFLOP/window (reference-size FFT model)",
                 "windows_per_launch": v["windows"] // v["launches"], "avg_launch_ms": round(v["ms"] / v["launches"], 4)}
 
