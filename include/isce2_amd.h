@@ -162,6 +162,9 @@ int ampcor_set_option(ampcor_ctx *ctx, const char *name, int64_t value);
  * "kernel_name launches total_ms windows\n" into buf (NUL-terminated). */
 int ampcor_profile(ampcor_ctx *ctx, int32_t enable);
 int ampcor_profile_report(ampcor_ctx *ctx, char *buf, int64_t len);
+/* Restrict the event timing to one kernel name (NULL or "" = every kernel),
+ * so that a timed region carries the fewest event records. */
+int ampcor_profile_only(ampcor_ctx *ctx, const char *name);
 
 /* Text of the last error on this context (or of the last ampcor_create failure
  * when ctx is NULL). */

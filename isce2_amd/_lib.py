@@ -25,7 +25,7 @@ FLAG_NOSYNC = 0x100
 EXPORTS = ("ampcor_create", "ampcor_destroy", "ampcor_setup", "ampcor_run_windows",
            "ampcor_run_windows_debug", "ampcor_set_precision", "ampcor_precision_stats",
            "ampcor_profile", "ampcor_profile_report", "ampcor_last_error", "ampcor_version",
-           "ampcor_set_fft_radices", "ampcor_set_option")
+           "ampcor_set_fft_radices", "ampcor_set_option", "ampcor_profile_only")
 
 
 class Params(C.Structure):
@@ -68,6 +68,7 @@ def load() -> C.CDLL:
     lib.ampcor_precision_stats.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
     lib.ampcor_set_fft_radices.argtypes = [vp, i32, vp, i32]
     lib.ampcor_set_option.argtypes = [vp, C.c_char_p, i64]
+    lib.ampcor_profile_only.argtypes = [vp, C.c_char_p]
     lib.ampcor_profile.argtypes = [vp, i32]
     lib.ampcor_profile_report.argtypes = [vp, C.c_char_p, i64]
     lib.ampcor_last_error.argtypes = [vp]
@@ -76,7 +77,7 @@ def load() -> C.CDLL:
     for name in ("ampcor_create", "ampcor_destroy", "ampcor_setup", "ampcor_run_windows",
                  "ampcor_run_windows_debug", "ampcor_profile", "ampcor_profile_report",
                  "ampcor_set_precision", "ampcor_precision_stats", "ampcor_set_fft_radices",
-                 "ampcor_set_option"):
+                 "ampcor_set_option", "ampcor_profile_only"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib

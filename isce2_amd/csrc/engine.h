@@ -64,8 +64,9 @@ struct Engine {
     struct ProfAcc { long launches = 0; double ms = 0.0; long windows = 0; };
     std::map<std::string, ProfAcc> prof_acc;
     hipEvent_t take_event();
+    std::string prof_only;   // time only this kernel when not empty
     template <class F> void timed(const char *name, int windows, hipStream_t st, F &&launch) {
-        if (!profiling) { launch(); return; }
+        if (!profiling || (!prof_only.empty() && prof_only != name)) { launch(); return; }
         ProfRec r{name, take_event(), take_event(), windows};
         (void)hipEventRecord(r.a, st);
         launch();

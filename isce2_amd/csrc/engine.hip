@@ -700,6 +700,11 @@ int ampcor_profile(ampcor_ctx *ctx, int32_t enable) {
     });
 }
 
+int ampcor_profile_only(ampcor_ctx *ctx, const char *name) {
+    if (!ctx) return AMPCOR_E_ARG;
+    return guarded(ctx, [&] { ctx->eng.prof_only = name ? name : ""; });
+}
+
 int ampcor_profile_report(ampcor_ctx *ctx, char *buf, int64_t len) {
     if (!ctx || !buf || len <= 0) return AMPCOR_E_ARG;
     return guarded(ctx, [&] {

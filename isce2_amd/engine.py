@@ -152,6 +152,10 @@ class Engine:
         """Reset and enable (or disable) per-kernel HIP event timing."""
         check(self.lib.ampcor_profile(self._h, int(bool(enable))), self._h)
 
+    def profile_only(self, name: str | None):
+        """Time only kernel ``name`` (None: every kernel)."""
+        check(self.lib.ampcor_profile_only(self._h, (name or "").encode()), self._h)
+
     def profile_report(self) -> dict:
         """{kernel: {"launches", "ms", "windows"}} accumulated since profile(True)."""
         buf = C.create_string_buffer(1 << 16)
