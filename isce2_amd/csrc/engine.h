@@ -108,7 +108,6 @@ struct Engine {
         hipStream_t stream;
         float *margin_coarse = nullptr, *margin_fine = nullptr;   // debug: top-1 minus top-2
         bool nosync = false;                                       // AMPCOR_FLAG_NOSYNC
-        int32_t reserve = 0;       // staged path: size its buffers for at least this many windows
     };
     // precision: 0 FP32 only; 1 FP32 + FP64 re-run of windows whose argmax
     // margins fall below the thresholds; 2 FP64 staged pipeline for every window
@@ -125,12 +124,6 @@ struct Engine {
     void run_staged(const RunArgs &a);
     template <class T> void run_staged_t(const RunArgs &a);
     void rerun_fp64(const RunArgs &a, const float *mc, const float *mf);
-    // FP64 re-run of windows [w0, w0 + n) of a call, queued on the side stream
-    // once event `ready` (the chunk's margins) has completed; later chunks keep
-    // running on the caller's stream meanwhile
-    void rerun_fp64_chunk(const RunArgs &a, const float *mc, const float *mf, int w0, int n, hipEvent_t ready);
-    hipStream_t side = nullptr, side_copy = nullptr;   // default-priority, non-blocking
-    std::vector<float> h_mc, h_mf;
     void run_fused(const RunArgs &a);
 };
 

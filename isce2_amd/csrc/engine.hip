@@ -56,8 +56,6 @@ void Engine::prof_collect() {
 Engine::~Engine() {
     for (auto &r : prof_pending) { (void)hipEventDestroy(r.a); (void)hipEventDestroy(r.b); }
     for (auto e : event_pool) (void)hipEventDestroy(e);
-    if (side) (void)hipStreamDestroy(side);
-    if (side_copy) (void)hipStreamDestroy(side_copy);
 }
 
 FftDesc make_schedule(int n) {
@@ -248,7 +246,7 @@ void Engine::run_staged_t(const RunArgs &a) {
     // correlation transform sizes (5-smooth, >= the search chips)
     const int L1h = sz.fft_corr_raw_h, L1w = sz.fft_corr_raw_w, L2h = sz.fft_corr_os_h, L2w = sz.fft_corr_os_w;
     const int Z = sz.zoom, So = sz.surf_out;
-    const int nb = std::max(std::min(chunk, a.n), std::min(chunk, int(a.reserve)));
+    const int nb = std::min(chunk, a.n);
 
     cx<T> *c_ref = buf<cx<T>>(tn + "c_ref", size_t(nb) * W_h * W_w);
     T *r_ref = buf<T>(tn + "r_ref", size_t(nb) * W_h * W_w);
@@ -489,65 +487,6 @@ void Engine::rerun_fp64(const RunArgs &a, const float *mc, const float *mf) {
     };
     auto put_i = [&](const int32_t *src, int32_t *dst) {
         if (dst) hipLaunchKernelGGL(k_put<int32_t>, dim3(idivup(m * 2, 256)), dim3(256), 0, a.stream, src, d_idx, dst, m, 2);
-    };
-    put_f(o_off, a.offset, 2);
-    put_f(o_snr, a.snr, 1);
-    put_f(o_cov, a.cov, 3);
-    put_f(o_val, a.maxval, 1);
-    put_i(sub.maxloc, a.maxloc);
-    put_i(sub.start, a.start);
-    put_i(sub.shift, a.shift);
-    put_i(sub.zoom_idx, a.zoom_idx);
-    HIP_OK(hipGetLastError());
-}
-
-void Engine::rerun_fp64_chunk(const RunArgs &a, const float *mc, const float *mf, int w0, int n,
-                              hipEvent_t ready) {
-    constexpr int kRerunChunk = 1024;   // FP64 staged buffers are sized once for this many windows
-    if (!side) HIP_OK(hipStreamCreateWithFlags(&side, hipStreamNonBlocking));
-    if (!side_copy) HIP_OK(hipStreamCreateWithFlags(&side_copy, hipStreamNonBlocking));
-    if (int(h_mc.size()) < a.n) { h_mc.resize(a.n); h_mf.resize(a.n); }
-    // margins of this chunk: wait for the chunk only, copy on the copy stream
-    HIP_OK(hipStreamWaitEvent(side_copy, ready, 0));
-    HIP_OK(hipMemcpyAsync(h_mc.data() + w0, mc + w0, sizeof(float) * n, hipMemcpyDeviceToHost, side_copy));
-    HIP_OK(hipMemcpyAsync(h_mf.data() + w0, mf + w0, sizeof(float) * n, hipMemcpyDeviceToHost, side_copy));
-    HIP_OK(hipStreamSynchronize(side_copy));
-    std::vector<int32_t> idx;
-    for (int i = w0; i < w0 + n; ++i)
-        if (!(h_mc[i] >= tau_coarse) || !(h_mf[i] >= tau_fine)) idx.push_back(i);
-    prec_checked += n;
-    prec_rerun += long(idx.size());
-    if (idx.empty()) return;
-    const int m = int(idx.size());
-    // scratch of the whole call, so no buffer grows while work is queued
-    int32_t *d_idx = buf<int32_t>("p_idx", a.n) + w0;
-    int32_t *st4 = buf<int32_t>("p_starts", size_t(4) * a.n) + size_t(4) * w0;
-    float *o_off = buf<float>("p_off", size_t(2) * a.n) + size_t(2) * w0;
-    float *o_snr = buf<float>("p_snr", a.n) + w0;
-    float *o_cov = buf<float>("p_cov", size_t(3) * a.n) + size_t(3) * w0;
-    float *o_val = buf<float>("p_val", a.n) + w0;
-    int32_t *o_i = buf<int32_t>("p_ints", size_t(8) * a.n) + size_t(8) * w0;
-    HIP_OK(hipStreamWaitEvent(side, ready, 0));
-    HIP_OK(hipMemcpyAsync(d_idx, idx.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, side));
-    HIP_OK(hipStreamSynchronize(side));   // idx is a host temporary; the copy is small
-    const int32_t *srcs[4] = {a.ref_d, a.ref_a, a.sec_d, a.sec_a};
-    for (int k = 0; k < 4; ++k)
-        hipLaunchKernelGGL(k_take_i32, dim3(idivup(m, 256)), dim3(256), 0, side, srcs[k], d_idx, st4 + k * m, m, 1);
-    RunArgs sub = a;
-    sub.ref_d = st4; sub.ref_a = st4 + m; sub.sec_d = st4 + 2 * m; sub.sec_a = st4 + 3 * m;
-    sub.n = m;
-    sub.margin_coarse = sub.margin_fine = nullptr;
-    sub.offset = o_off; sub.snr = o_snr; sub.cov = o_cov; sub.maxval = o_val;
-    sub.maxloc = o_i; sub.start = o_i + 2 * m; sub.shift = o_i + 4 * m; sub.zoom_idx = o_i + 6 * m;
-    sub.chunk = kRerunChunk;
-    sub.reserve = kRerunChunk;
-    sub.stream = side;
-    run_staged_t<double>(sub);
-    auto put_f = [&](const float *src, float *dst, int width) {
-        if (dst) hipLaunchKernelGGL(k_put<float>, dim3(idivup(m * width, 256)), dim3(256), 0, side, src, d_idx, dst, m, width);
-    };
-    auto put_i = [&](const int32_t *src, int32_t *dst) {
-        if (dst) hipLaunchKernelGGL(k_put<int32_t>, dim3(idivup(m * 2, 256)), dim3(256), 0, side, src, d_idx, dst, m, 2);
     };
     put_f(o_off, a.offset, 2);
     put_f(o_snr, a.snr, 1);

@@ -586,9 +586,6 @@ void Engine::run_fused(const RunArgs &a) {
     const size_t lds_or = os_lds_bytes(W), lds_oz = os_lds_bytes(Zc);
     const size_t lds_f = FineLds(P, P - T2 + 1, T2).total;
 
-    const bool rerun = precision == 1 && !a.nosync;
-    std::vector<std::pair<int, int>> chunks;
-    std::vector<hipEvent_t> done;
     for (int w0 = 0; w0 < a.n; w0 += nb) {
         const int n = std::min(nb, a.n - w0);
         timed("k_coarse", n, st, [&] {
@@ -628,26 +625,8 @@ void Engine::run_fused(const RunArgs &a) {
             });
         }
         HIP_OK(hipGetLastError());
-        if (rerun) {
-            hipEvent_t e = take_event();
-            HIP_OK(hipEventRecord(e, st));
-            chunks.emplace_back(w0, n);
-            done.push_back(e);
-        }
     }
-    if (rerun) {
-        // chunk by chunk: select near ties as soon as a chunk's margins exist and
-        // queue their FP64 re-run on the side stream while later chunks compute
-        for (size_t k = 0; k < chunks.size(); ++k)
-            rerun_fp64_chunk(a, mc, mf, chunks[k].first, chunks[k].second, done[k]);
-        if (side) {
-            hipEvent_t joined = take_event();
-            HIP_OK(hipEventRecord(joined, side));
-            HIP_OK(hipStreamWaitEvent(st, joined, 0));
-            event_pool.push_back(joined);
-        }
-        for (hipEvent_t e : done) event_pool.push_back(e);
-    }
+    if (precision == 1 && !a.nosync) rerun_fp64(a, mc, mf);
 }
 
 }  // namespace ampcor
