@@ -222,13 +222,42 @@ void fft_cols(const FftPlan &pl, cx<T> *data, int nimg, int w, int elems, hipStr
     hipLaunchKernelGGL((k_fft_cols<T, DIR>), dim3(idivup(w, c), nimg), dim3(256), lds, st, data, w, c, d);
 }
 
+template <class T, int DIR>
+void fft_rows_sel(const FftPlan &pl, cx<T> *data, int nimg, int H, int sel, int split, int off1, int elems,
+                  hipStream_t st) {
+    const FftDesc &d = std::is_same<T, float>::value ? pl.desc32 : pl.desc64;
+    const int g = std::max(1, elems / pl.n);
+    const size_t lds = 2 * size_t(g) * pl.n * sizeof(cx<T>);
+    hipLaunchKernelGGL((k_fft_rows_sel<T, DIR>), dim3(idivup(nimg * sel, g)), dim3(256), lds, st, data, nimg, H, sel,
+                       split, off1, g, d);
+}
+
+template <class T>
+int fft_elems(const Engine &e) { return sizeof(T) == 8 ? std::max(1024, e.fft_block_elems / 2) : e.fft_block_elems; }
+
+// inverse 2D FFT of a quadrant-padded spectrum (k_quadpad of an h-row chip into
+// H rows): the row pass skips the all-zero rows
+template <class T, int DIR>
+void fft2_padded(Engine &e, cx<T> *data, int nimg, int H, int W, int h, hipStream_t st) {
+    const int h2 = h / 2;
+    fft_rows_sel<T, DIR>(e.plan(W), data, nimg, H, 2 * h2, h2, H - h2, fft_elems<T>(e), st);
+    fft_cols<T, DIR>(e.plan(H), data, nimg, W, fft_elems<T>(e), st);
+}
+
+// 2D FFT of which only rows [0, keep) of the result are used: columns first,
+// then only those rows
+template <class T, int DIR>
+void fft2_rows_kept(Engine &e, cx<T> *data, int nimg, int h, int w, int keep, hipStream_t st) {
+    fft_cols<T, DIR>(e.plan(h), data, nimg, w, fft_elems<T>(e), st);
+    fft_rows_sel<T, DIR>(e.plan(w), data, nimg, h, keep, keep, 0, fft_elems<T>(e), st);
+}
+
 // 2D FFT of nimg images h x w (rows then columns), in place
 template <class T, int DIR>
 void fft2(Engine &e, cx<T> *data, int nimg, int h, int w, hipStream_t st) {
-    // double elements take twice the LDS: halve the element budget
-    const int elems = sizeof(T) == 8 ? std::max(1024, e.fft_block_elems / 2) : e.fft_block_elems;
-    fft_rows<T, DIR>(e.plan(w), data, int64_t(nimg) * h, elems, st);
-    fft_cols<T, DIR>(e.plan(h), data, nimg, w, elems, st);
+    // double elements take twice the LDS: half the element budget (fft_elems)
+    fft_rows<T, DIR>(e.plan(w), data, int64_t(nimg) * h, fft_elems<T>(e), st);
+    fft_cols<T, DIR>(e.plan(h), data, nimg, w, fft_elems<T>(e), st);
 }
 
 inline dim3 grid2(int per_img, int nimg) { return dim3(std::min(idivup(per_img, 256), 64), nimg); }
@@ -314,7 +343,7 @@ void Engine::run_staged_t(const RunArgs &a) {
         fft2<T, -1>(*this, zc, n, L1h, L1w, st);
         hipLaunchKernelGGL(k_spec_product<T>, grid2(L1h * L1w, n), dim3(256), 0, st, zc, L1h, L1w,
                            T(1.0 / (double(L1h) * L1w)));
-        fft2<T, +1>(*this, zc, n, L1h, L1w, st);
+        fft2_rows_kept<T, +1>(*this, zc, n, L1h, L1w, sz.raw_corr_h, st);
         hipLaunchKernelGGL(k_crop_real<T>, grid2(sz.raw_corr_h * sz.raw_corr_w, n), dim3(256), 0, st, zc, L1h, L1w,
                            corr, sz.raw_corr_h, sz.raw_corr_w);
         // normalize (cuAmpcorChunk.cpp:60)
@@ -332,7 +361,7 @@ void Engine::run_staged_t(const RunArgs &a) {
         fft2<T, +1>(*this, c_ref, n, W_h, W_w, st);
         hipLaunchKernelGGL(k_quadpad<T>, grid2(sz.os_win_h * sz.os_win_w, n), dim3(256), 0, st, c_ref, W_h, W_w,
                            c_ref_os, sz.os_win_h, sz.os_win_w);
-        fft2<T, -1>(*this, c_ref_os, n, sz.os_win_h, sz.os_win_w, st);
+        fft2_padded<T, -1>(*this, c_ref_os, n, sz.os_win_h, sz.os_win_w, W_h, st);
         hipLaunchKernelGGL(k_abs<T>, dim3(idivup(n * sz.os_win_h * sz.os_win_w, 256)), dim3(256), 0, st, c_ref_os,
                            r_ref_os, int64_t(n) * sz.os_win_h * sz.os_win_w);
         hipLaunchKernelGGL(k_sub_mean<T>, dim3(n), dim3(per_window_threads(sz.os_win_h * sz.os_win_w)), 0, st,
@@ -346,7 +375,7 @@ void Engine::run_staged_t(const RunArgs &a) {
         fft2<T, +1>(*this, c_zoom, n, sz.zoom_in_h, sz.zoom_in_w, st);
         hipLaunchKernelGGL(k_quadpad<T>, grid2(OSh * OSw, n), dim3(256), 0, st, c_zoom, sz.zoom_in_h, sz.zoom_in_w,
                            c_sec_os, OSh, OSw);
-        fft2<T, -1>(*this, c_sec_os, n, OSh, OSw, st);
+        fft2_padded<T, -1>(*this, c_sec_os, n, OSh, OSw, sz.zoom_in_h, st);
         hipLaunchKernelGGL(k_abs<T>, dim3(idivup(n * OSh * OSw, 256)), dim3(256), 0, st, c_sec_os, r_sec_os,
                            int64_t(n) * OSh * OSw);
         // oversampled correlation + normalize (cuAmpcorChunk.cpp:148-161)
@@ -355,7 +384,7 @@ void Engine::run_staged_t(const RunArgs &a) {
         fft2<T, -1>(*this, zc, n, L2h, L2w, st);
         hipLaunchKernelGGL(k_spec_product<T>, grid2(L2h * L2w, n), dim3(256), 0, st, zc, L2h, L2w,
                            T(1.0 / (double(L2h) * L2w)));
-        fft2<T, +1>(*this, zc, n, L2h, L2w, st);
+        fft2_rows_kept<T, +1>(*this, zc, n, L2h, L2w, sz.os_corr_h, st);
         hipLaunchKernelGGL(k_crop_real<T>, grid2(sz.os_corr_h * sz.os_corr_w, n), dim3(256), 0, st, zc, L2h, L2w,
                            corr_os, sz.os_corr_h, sz.os_corr_w);
         normalize(corr_os, sz.os_corr_h, sz.os_corr_w, r_ref_os, sz.os_win_h, sz.os_win_w, r_sec_os, OSh, OSw,
@@ -380,7 +409,7 @@ void Engine::run_staged_t(const RunArgs &a) {
                                    sz.os_corr_w, c_surf_in, Z);
                 fft2<T, +1>(*this, c_surf_in, n, Z, Z, st);
                 hipLaunchKernelGGL(k_quadpad<T>, grid2(So * So, n), dim3(256), 0, st, c_surf_in, Z, Z, c_surf, So, So);
-                fft2<T, -1>(*this, c_surf, n, So, So, st);
+                fft2_padded<T, -1>(*this, c_surf, n, So, So, Z, st);
                 hipLaunchKernelGGL(k_real<T>, dim3(idivup(n * So * So, 256)), dim3(256), 0, st, c_surf, r_surf,
                                    int64_t(n) * So * So);
             }

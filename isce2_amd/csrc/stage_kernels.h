@@ -27,6 +27,35 @@ __global__ void __launch_bounds__(256) k_fft_rows(cx<T> *data, int nrows, int g,
     for (int i = threadIdx.x; i < tot; i += blockDim.x) base[i] = res[i];
 }
 
+// rows of length d.n selected per image: of the `sel` selected rows of each
+// [H][d.n] image, rows s < split are row s and the others row off1 + s - split
+// (the non-zero rows of a quadrant-padded spectrum, or the leading rows an
+// inverse correlation keeps); `g` selected rows per block
+template <class T, int DIR>
+__global__ void __launch_bounds__(256) k_fft_rows_sel(cx<T> *data, int nimg, int H, int sel, int split, int off1,
+                                                      int g, FftDesc d) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *a = reinterpret_cast<cx<T> *>(smem_raw);
+    cx<T> *b = a + g * d.n;
+    const int t0 = blockIdx.x * g;
+    const int nr = min(g, nimg * sel - t0);
+    auto row = [&](int k) {
+        const int t = t0 + k, img = t / sel, sr = t - img * sel;
+        const int r = sr < split ? sr : off1 + (sr - split);
+        return data + (size_t(img) * H + r) * d.n;
+    };
+    for (int i = threadIdx.x; i < nr * d.n; i += blockDim.x) {
+        const int k = i / d.n, c = i - k * d.n;
+        a[i] = row(k)[c];
+    }
+    __syncthreads();
+    cx<T> *res = fft_lds<T, DIR>(a, b, d, nr, 1, d.n);
+    for (int i = threadIdx.x; i < nr * d.n; i += blockDim.x) {
+        const int k = i / d.n, c = i - k * d.n;
+        row(k)[c] = res[i];
+    }
+}
+
 // columns: images [nimg][h][w] with h = d.n; `c` columns per block
 template <class T, int DIR>
 __global__ void __launch_bounds__(256) k_fft_cols(cx<T> *data, int w, int c, FftDesc d) {
