@@ -28,10 +28,10 @@ MFLOP_PER_WINDOW = 12.364
 KERNEL_MFLOP = {"k_coarse": 1.235, "k_oversample_ref": 1.434, "k_oversample_search": 1.817, "k_fine": 7.878}
 FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per window of each fused kernel, from rocprofv3 FETCH_SIZE + WRITE_SIZE (KiB per dispatch of a
-# 2048-window launch) scaled by a 1 GiB device copy's counts (FETCH x2, WRITE x1); re-measure after kernel changes
-HBM_BYTES_PER_WINDOW = {k: v * 1024 / 2048 for k, v in {
-    "k_fine": 148940 * 2 + 192, "k_coarse": 133033 * 2 + 49616,
-    "k_oversample_ref": 81388 * 2 + 131136, "k_oversample_search": 85076 * 2 + 165888}.items()}
+# 7812-window launch) scaled by a 1 GiB device copy's counts (FETCH x2, WRITE x1); re-measure after kernel changes
+HBM_BYTES_PER_WINDOW = {k: v * 1024 / 7812 for k, v in {
+    "k_fine": 595099.1 * 2 + 768.1, "k_coarse": 545475.3 * 2 + 198464.4,
+    "k_oversample_ref": 326390.4 * 2 + 524544.0, "k_oversample_search": 341513.1 * 2 + 663552.0}.items()}
 HBM_PEAK_GBS = 8000.0
 
 
