@@ -70,7 +70,17 @@ __global__ void __launch_bounds__(NT, (K >= 9 ? 2 : 4)) k_coarse(CoarseArgs a, c
     // template's FP64 sum and sum of squares, gathers unrolled for memory parallelism
     const int64_t rd = a.ref_d[win], ra = a.ref_a[win];
     const int64_t sd = a.sec_d[win], sa = a.sec_a[win];
-#pragma unroll 2
+    // template pixels are requested first and held raw in registers, so that
+    // their loads overlap the search chip's
+    constexpr int TI = W * W / NT;
+    static_assert(W * W % NT == 0, "template pixels divide evenly over the workgroup");
+    float2 traw[TI];
+#pragma unroll
+    for (int k = 0; k < TI; ++k) {
+        const int i = threadIdx.x + k * NT, r = i / W, x = i - r * W;
+        traw[k] = pixel_raw(a.ref, a.mode, (rd + r) * a.ref_pitch + ra + x);
+    }
+#pragma unroll 8
     for (int i = threadIdx.x; i < nS * L; i += NT) {
         const int j = i / L, x = i - j * L;
         cf v = {0.f, 0.f};
@@ -81,10 +91,9 @@ __global__ void __launch_bounds__(NT, (K >= 9 ? 2 : 4)) k_coarse(CoarseArgs a, c
         S[j * P + x] = v;
     }
     double sum = 0.0, sum2 = 0.0;
-#pragma unroll 2
-    for (int i = threadIdx.x; i < W * W; i += NT) {
-        const int r = i / W, x = i - r * W;
-        const double v = pixel_amp(a.ref, a.mode, (rd + r) * a.ref_pitch + ra + x);
+#pragma unroll
+    for (int k = 0; k < TI; ++k) {
+        const double v = raw_amp(traw[k], a.mode);
         sum += v;
         sum2 += v * v;
     }
@@ -108,7 +117,7 @@ __global__ void __launch_bounds__(NT, (K >= 9 ? 2 : 4)) k_coarse(CoarseArgs a, c
     #pragma unroll 1
     for (int c0 = 0; c0 < W; c0 += TC) {
         // template rows [c0, c0 + TC) minus the FP64 mean, packed in pairs
-#pragma unroll 2
+#pragma unroll
         for (int i = threadIdx.x; i < nTc * L; i += NT) {
             const int j = i / L, x = i - j * L;
             cf v = {0.f, 0.f};
@@ -358,14 +367,32 @@ __global__ void __launch_bounds__(NTH) k_fine(FineArgs a, const cf *tw, int w0) 
     const float *tp = a.tamp + size_t(win) * T * T;
     const float *sg = a.samp + size_t(win) * P * P;
     const double mean = a.tmean[2 * win], t2 = a.tmean[2 * win + 1];
-#pragma unroll 4
-    for (int i = threadIdx.x; i < nS * L; i += NTH) {
-        const int j = i / L, x = i - j * L;
+    // the search chip's loads are all issued before any of its values is stored
+    constexpr int SN = nS * L, SI = (SN + NTH - 1) / NTH;
+    constexpr int TN = (T / 2) * L, TI = (TN + NTH - 1) / NTH;
+    cf sv[SI], tv[TI];
+#pragma unroll
+    for (int k = 0; k < SI; ++k) {
+        const int i = threadIdx.x + k * NTH, j = i / L, x = i - j * L;
         // planar oversampled chip: row 2j + p, column x -> plane 2p + (x & 1), (j, x >> 1)
         const float *s0 = sg + size_t(x & 1) * (P / 2) * (P / 2) + j * (P / 2) + (x >> 1);
-        S[j * PP + x] = {s0[0], s0[size_t(2) * (P / 2) * (P / 2)]};
+        if (SN % NTH == 0 || i < SN) sv[k] = {s0[0], s0[size_t(2) * (P / 2) * (P / 2)]};
+    }
+#pragma unroll
+    for (int k = 0; k < SI; ++k) {
+        const int i = threadIdx.x + k * NTH, j = i / L, x = i - j * L;
+        if (SN % NTH == 0 || i < SN) S[j * PP + x] = sv[k];
     }
     __syncthreads();
+    // the template's loads go out once the search chip has arrived (they would
+    // only delay it) and stay in flight through the window sums
+#pragma unroll
+    for (int k = 0; k < TI; ++k) {
+        const int i = threadIdx.x + k * NTH, j = i / L, x = i - j * L;
+        const float *t0 = tp + size_t(x & 1) * (T / 2) * (T / 2) + j * (T / 2) + (x >> 1);
+        tv[k] = {0.f, 0.f};
+        if ((TN % NTH == 0 || i < TN) && x < T) tv[k] = {t0[0], t0[size_t(2) * (T / 2) * (T / 2)]};
+    }
     window_norms<(P + 63) / 64, NTH>(
         [&](int r, int c) -> double {
             const cf v = S[(r >> 1) * PP + c];
@@ -374,16 +401,11 @@ __global__ void __launch_bounds__(NTH) k_fine(FineArgs a, const cf *tw, int w0) 
         P, T, T, NL, NL, t2, nrm, reinterpret_cast<double *>(Tb));   // template not loaded yet
     __syncthreads();
     // template rows minus the FP64 mean, row pairs packed, zero beyond column T
-#pragma unroll 3
-    for (int i = threadIdx.x; i < (T / 2) * L; i += NTH) {
-        const int j = i / L, x = i - j * L;
-        cf v = {0.f, 0.f};
-        if (x < T) {
-            const float *t0 = tp + size_t(x & 1) * (T / 2) * (T / 2) + j * (T / 2) + (x >> 1);
-            v.x = float(double(t0[0]) - mean);
-            v.y = float(double(t0[size_t(2) * (T / 2) * (T / 2)]) - mean);
-        }
-        Tb[j * PP + x] = v;
+#pragma unroll
+    for (int k = 0; k < TI; ++k) {
+        const int i = threadIdx.x + k * NTH, j = i / L, x = i - j * L;
+        if (TN % NTH == 0 || i < TN)
+            Tb[j * PP + x] = x < T ? cf{float(double(tv[k].x) - mean), float(double(tv[k].y) - mean)} : cf{0.f, 0.f};
     }
     __syncthreads();
     if (lay.tchunk == nS * PP * 8) {

@@ -25,6 +25,16 @@ __device__ __forceinline__ float pixel_amp(const void *img, int mode, int64_t of
     return sqrtf(z.x * z.x + z.y * z.y);
 }
 
+// one pixel as loaded (real images: (value, 0)); raw_amp gives what pixel_amp gives
+__device__ __forceinline__ float2 pixel_raw(const void *img, int mode, int64_t off) {
+    if (mode == 2) return {static_cast<const float *>(img)[off], 0.f};
+    return static_cast<const float2 *>(img)[off];
+}
+
+__device__ __forceinline__ float raw_amp(float2 z, int mode) {
+    return mode == 2 ? fabsf(z.x) : sqrtf(z.x * z.x + z.y * z.y);
+}
+
 __device__ __forceinline__ cf pixel_cx(const void *img, int mode, int64_t off) {
     if (mode == 2) return {static_cast<const float *>(img)[off], 0.f};
     const float2 z = static_cast<const float2 *>(img)[off];
