@@ -477,31 +477,69 @@ __global__ void __launch_bounds__(NTH) k_fine(FineArgs a, const cf *tw, int w0) 
         for (int c = 0; c < 16; ++c) G[k1 * SO + 16 * c + dd] = x[c];
     }
     __syncthreads();
-    // out[16aa+b][q] = Re DFT16_aa( G[k1][q] e^{-2 pi i b ks1/256} ); running argmax
-    // each thread takes one column q of G and sweeps its b values
-    static_assert(NTH % SO == 0, "whole surface columns per thread");
-    Top2<float> top;
-    {
-        const int qq = threadIdx.x & (SO - 1);
-        #pragma unroll 1
-        for (int b = threadIdx.x / SO; b < 16; b += NTH / SO) {
-            // e^{-2 pi i b k/256} for k = 1..8 (negative bins use the conjugate);
-            // b is uniform over the wave, so these are scalar loads into SGPRs
-            const int bu = __builtin_amdgcn_readfirstlane(b);
-            cf w[9];
+    // Column pruning. Every value of surface column q is at most
+    // B_q = sum_k1 |G[k1][q]|, and the surface passes through the trimmed
+    // samples, so its maximum is at least their largest value amax. A column with
+    // B_q (1 + 2^-10) + 2^-11 |a|max < amax can hold neither the argmax nor any
+    // value within 2^-12 |a|max of it (the slack covers FP32 rounding many times
+    // over): it is skipped, which leaves the argmax, the tie rule and every
+    // margin below 2^-12 |a|max unchanged. NaN bounds keep their column.
+    static_assert(NTH >= SO, "one thread per surface column for the bounds");
+    float amax = -FLT_MAX, amag = 0.f;
+    for (int i = threadIdx.x & 63; i < Z * Z; i += 64) {
+        const float v = corr[(i / Z) * NL + (i % Z)];
+        amax = fmaxf(amax, v);
+        amag = fmaxf(amag, fabsf(v));
+    }
 #pragma unroll
-            for (int k = 1; k <= 8; ++k) w[k] = a.tw256[(bu * k) & 255];
-            cf x[16];
-            x[0] = G[qq];
+    for (int o = 32; o > 0; o >>= 1) {
+        amax = fmaxf(amax, __shfl_xor(amax, o));
+        amag = fmaxf(amag, __shfl_xor(amag, o));
+    }
+    uint64_t *masks = reinterpret_cast<uint64_t *>(red + 48);   // ballots of the SO/64 bound waves
+    int *cols = reinterpret_cast<int *>(nrm);                    // swept columns (window sums are spent)
+    bool keep = false;
+    if (threadIdx.x < SO) {
+        float bq = 0.f;
 #pragma unroll
-            for (int k1 = 1; k1 < 16; ++k1) {
-                const cf t = k1 < 8 ? w[k1] : conj(w[16 - k1]);
-                x[k1] = cmul(G[k1 * SO + qq], t);
-            }
-            dft16<float, -1>(x);
-#pragma unroll
-            for (int aa = 0; aa < 16; ++aa) top.add_idx(x[aa].x, uint32_t((16 * aa + b) * SO + qq));
+        for (int k1 = 0; k1 < 16; ++k1) {
+            const cf g = G[k1 * SO + threadIdx.x];
+            bq += sqrtf(g.x * g.x + g.y * g.y);
         }
+        keep = !(bq * (1.0f + 0x1p-10f) + 0x1p-11f * amag < amax);
+    }
+    const uint64_t wm = __ballot(keep);
+    if (threadIdx.x < SO && (threadIdx.x & 63) == 0) masks[threadIdx.x >> 6] = wm;
+    __syncthreads();
+    int ncols = 0, before = 0;
+#pragma unroll
+    for (int w = 0; w < SO / 64; ++w) {
+        const int c = __popcll(masks[w]);
+        if (w < (threadIdx.x >> 6)) before += c;
+        ncols += c;
+    }
+    if (keep) cols[before + __popcll(wm & ((uint64_t(1) << (threadIdx.x & 63)) - 1))] = threadIdx.x;
+    __syncthreads();
+    // out[16aa+b][q] = Re DFT16_aa( G[k1][q] e^{-2 pi i b ks1/256} ) over the kept
+    // columns q; task t = (b, column list entry); running argmax
+    Top2<float> top;
+    #pragma unroll 1
+    for (int t = threadIdx.x; t < 16 * ncols; t += NTH) {
+        const int b = t / ncols, qq = cols[t - b * ncols];
+        // e^{-2 pi i b k/256} for k = 1..8 (negative bins use the conjugate)
+        cf w[9];
+#pragma unroll
+        for (int k = 1; k <= 8; ++k) w[k] = tws[(b * k) & 255];
+        cf x[16];
+        x[0] = G[qq];
+#pragma unroll
+        for (int k1 = 1; k1 < 16; ++k1) {
+            const cf tk = k1 < 8 ? w[k1] : conj(w[16 - k1]);
+            x[k1] = cmul(G[k1 * SO + qq], tk);
+        }
+        dft16<float, -1>(x);
+#pragma unroll
+        for (int aa = 0; aa < 16; ++aa) top.add_idx(x[aa].x, uint32_t((16 * aa + b) * SO + qq));
     }
     // red: [0, 16) block sums, then NTH/64 top-1 values, runner-up values and keys
     top.reduce<NTH>(reinterpret_cast<float *>(red + 16), reinterpret_cast<float *>(red + 16) + 16,

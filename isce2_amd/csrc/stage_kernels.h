@@ -665,24 +665,60 @@ __global__ void __launch_bounds__(256) k_surface(const T *corr, int ch, int cw, 
         for (int cc = 0; cc < 16; ++cc) G[k1 * 256 + 16 * cc + dd] = x[cc];
     }
     __syncthreads();
-    Top2<T> top;
+    // Column pruning: every value of column q is at most B_q = sum_k1 |G[k1][q]|
+    // and the surface passes through the 16x16 samples, so its maximum is at
+    // least their largest value amax. Columns with B_q (1 + 2^-10) + 2^-11 |a|max
+    // < amax hold neither the argmax nor any value within 2^-12 |a|max of it, and
+    // are skipped (argmax, tie rule and margins below that are unchanged; NaN
+    // bounds keep their column).
+    __shared__ uint64_t masks[4];
+    __shared__ int cols[256];
+    T amax = -INFINITY, amag = T(0);
+    for (int i = threadIdx.x & 63; i < 256; i += 64) {
+        const T v = c[(i >> 4) * cw + (i & 15)];
+        amax = fmax(amax, v);
+        amag = fmax(amag, fabs(v));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        amax = fmax(amax, __shfl_xor(amax, o));
+        amag = fmax(amag, __shfl_xor(amag, o));
+    }
+    bool keep;
     {
-        const int qq = threadIdx.x;     // one surface column per thread, all 16 values of b
-        cx<T> g[16];
+        T bq = T(0);
 #pragma unroll
-        for (int k1 = 0; k1 < 16; ++k1) g[k1] = G[k1 * 256 + qq];
-        #pragma unroll 1
-        for (int b = 0; b < 16; ++b) {
-            cx<T> x[16];
-#pragma unroll
-            for (int k1 = 0; k1 < 16; ++k1) {
-                const int ks = k1 < 8 ? k1 : k1 - 16;
-                x[k1] = cmul(g[k1], tws[(b * ks + 4096) & 255]);
-            }
-            dft16<T, -1>(x);
-#pragma unroll
-            for (int aa = 0; aa < 16; ++aa) top.add_idx(x[aa].x, uint32_t((16 * aa + b) * 256 + qq));
+        for (int k1 = 0; k1 < 16; ++k1) {
+            const cx<T> g = G[k1 * 256 + threadIdx.x];
+            bq += sqrt(g.x * g.x + g.y * g.y);
         }
+        keep = !(bq * T(1.0 + 0x1p-10) + T(0x1p-11) * amag < amax);
+    }
+    const uint64_t wm = __ballot(keep);
+    if ((threadIdx.x & 63) == 0) masks[threadIdx.x >> 6] = wm;
+    __syncthreads();
+    int ncols = 0, before = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+        const int n = __popcll(masks[w]);
+        if (w < (threadIdx.x >> 6)) before += n;
+        ncols += n;
+    }
+    if (keep) cols[before + __popcll(wm & ((uint64_t(1) << (threadIdx.x & 63)) - 1))] = threadIdx.x;
+    __syncthreads();
+    Top2<T> top;
+    #pragma unroll 1
+    for (int t = threadIdx.x; t < 16 * ncols; t += 256) {
+        const int b = t / ncols, qq = cols[t - b * ncols];
+        cx<T> x[16];
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) {
+            const int ks = k1 < 8 ? k1 : k1 - 16;
+            x[k1] = cmul(G[k1 * 256 + qq], tws[(b * ks + 4096) & 255]);
+        }
+        dft16<T, -1>(x);
+#pragma unroll
+        for (int aa = 0; aa < 16; ++aa) top.add_idx(x[aa].x, uint32_t((16 * aa + b) * 256 + qq));
     }
     top.template reduce<256>(shv, shv2, shk);
     if (threadIdx.x == 0) {
