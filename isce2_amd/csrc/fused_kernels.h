@@ -64,10 +64,36 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
 // NB row blocks summed by all threads at once (`part`: 2*NB*SW doubles of
 // scratch); each wave then slides them down to its first lag row, and walks
 // its contiguous lag rows with the horizontal window sums taken as differences
-// of wave prefix scans (lanes own columns, CPL per lane). One barrier, which
-// every thread of the workgroup must reach.
+// of wave prefix scans (lanes own columns, CPL per lane; the scans run on DPP
+// row shifts and row broadcasts, so they are VALU work with no LDS round trip).
+// One barrier, which every thread of the workgroup must reach.
 // ---------------------------------------------------------------------------
 constexpr int NORM_BLOCKS = 4;
+
+// DPP move of an FP64 value (both halves); lanes the control leaves out read 0
+template <int CTRL, int ROWMASK>
+__device__ __forceinline__ double dpp_f64(double x) {
+    const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, ROWMASK, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(x), CTRL, ROWMASK, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
+// inclusive wave64 prefix sum: row_shr 1, 2, 4, 8 within rows of 16 lanes, then
+// row_bcast:15 into rows 1 and 3 and row_bcast:31 into rows 2 and 3
+__device__ __forceinline__ double wave_scan_f64(double x) {
+    x += dpp_f64<0x111, 0xf>(x);
+    x += dpp_f64<0x112, 0xf>(x);
+    x += dpp_f64<0x114, 0xf>(x);
+    x += dpp_f64<0x118, 0xf>(x);
+    x += dpp_f64<0x142, 0xa>(x);
+    x += dpp_f64<0x143, 0xc>(x);
+    return x;
+}
+
+__device__ __forceinline__ double lane63_f64(double x) {
+    return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(x), 63),
+                            __builtin_amdgcn_readlane(__double2loint(x), 63));
+}
 
 template <int CPL, int NTH = NT, class Amp>
 __device__ void window_norms(Amp amp, int SW, int TH, int TW, int NLh, int NLw, double t2, float *nrm,
@@ -109,12 +135,22 @@ __device__ void window_norms(Amp amp, int SW, int TH, int TW, int NLh, int NLw, 
                 g1[m] += part[b * SW + c];
                 g2[m] += part[(NB + b) * SW + c];
             }
-            #pragma unroll 1
-            for (int u = 0; u < u_beg; ++u) {
+            // slide down to u_beg (two independent chains)
+            double d1 = 0.0, d2 = 0.0, e1 = 0.0, e2 = 0.0;
+            int u = 0;
+#pragma unroll 2
+            for (; u + 1 < u_beg; u += 2) {
                 const double vn = amp(u + TH, c), vo = amp(u, c);
-                g1[m] += vn - vo;
-                g2[m] += vn * vn - vo * vo;
+                const double wn = amp(u + 1 + TH, c), wo = amp(u + 1, c);
+                d1 += vn - vo; d2 += vn * vn - vo * vo;
+                e1 += wn - wo; e2 += wn * wn - wo * wo;
             }
+            if (u < u_beg) {
+                const double vn = amp(u + TH, c), vo = amp(u, c);
+                d1 += vn - vo; d2 += vn * vn - vo * vo;
+            }
+            g1[m] += d1 + e1;
+            g2[m] += d2 + e2;
         }
     }
     const double inv_n = 1.0 / double(TH * TW);
@@ -124,16 +160,10 @@ __device__ void window_norms(Amp amp, int SW, int TH, int TW, int NLh, int NLw, 
         double c1 = 0.0, c2 = 0.0;
 #pragma unroll
         for (int m = 0; m < CPL; ++m) {
-            double x1 = g1[m], x2 = g2[m];
-#pragma unroll
-            for (int d = 1; d < 64; d <<= 1) {
-                const double y1 = __shfl_up(x1, d), y2 = __shfl_up(x2, d);
-                if (lane >= d) { x1 += y1; x2 += y2; }
-            }
-            x1 += c1; x2 += c2;
+            const double x1 = wave_scan_f64(g1[m]) + c1, x2 = wave_scan_f64(g2[m]) + c2;
             p1[m] = x1; p2[m] = x2;
-            c1 = __shfl(x1, 63);
-            c2 = __shfl(x2, 63);
+            c1 = lane63_f64(x1);
+            c2 = lane63_f64(x2);
         }
         #pragma unroll 1
         for (int k = 0; k * 64 < NLw; ++k) {
