@@ -64,7 +64,7 @@ __global__ void __launch_bounds__(NT, (K >= 9 ? 2 : 4)) k_coarse(CoarseArgs a, c
     float *nrm = reinterpret_cast<float *>(smem + lay.nrm);
     float *corr = reinterpret_cast<float *>(S + lay.corr_off);
     double *red = reinterpret_cast<double *>(smem + lay.red);
-    const int win = w0 + blockIdx.x;
+    const int win = w0 + xcd_window(blockIdx.x, gridDim.x);
 
     // search amplitudes, row pairs packed as (row 2j, row 2j+1), and the
     // template's FP64 sum and sum of squares, gathers unrolled for memory parallelism
@@ -198,7 +198,7 @@ __global__ void __launch_bounds__(NT) k_oversample(OsArgs a, const cf *tw, int w
     cx<double> *rowf = reinterpret_cast<cx<double> *>(smem + al16(N * PC * 8 + N * 8));   // e^{i r phy}
     cx<double> *colf = rowf + N;                                                         // e^{i c phx}
     double *red = reinterpret_cast<double *>(colf + N);
-    const int win = w0 + blockIdx.x;
+    const int win = w0 + xcd_window(blockIdx.x, gridDim.x);
     int64_t r0 = a.sd[win], c0 = a.sa[win];
     if (a.start) { r0 += a.start[2 * win]; c0 += a.start[2 * win + 1]; }
     auto gather = [&]() {
@@ -363,7 +363,7 @@ __global__ void __launch_bounds__(NTH) k_fine(FineArgs a, const cf *tw, int w0) 
     float *nrm = reinterpret_cast<float *>(smem + lay.nrm);
     float *corr = reinterpret_cast<float *>(smem + lay.corr);
     double *red = reinterpret_cast<double *>(smem + lay.red);
-    const int win = w0 + blockIdx.x;
+    const int win = w0 + xcd_window(blockIdx.x, gridDim.x);
     const float *tp = a.tamp + size_t(win) * T * T;
     const float *sg = a.samp + size_t(win) * P * P;
     const double mean = a.tmean[2 * win], t2 = a.tmean[2 * win + 1];

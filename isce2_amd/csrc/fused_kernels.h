@@ -42,6 +42,15 @@ __device__ __forceinline__ cf pixel_cx(const void *img, int mode, int64_t off) {
     return {z.x, z.y};
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (block b runs on XCD
+// b % 8), each with its own L2. Giving every XCD a contiguous run of windows
+// keeps neighbouring windows, whose chips overlap in the image, in one L2.
+__device__ __forceinline__ int xcd_window(int b, int n) {
+    const int x = b & 7, j = b >> 3;
+    const int q = n >> 3, r = n & 7;     // XCD x runs q + (x < r) of the n blocks
+    return x * q + min(x, r) + j;
+}
+
 // FP64 workgroup sum over NTH threads; `sh` holds NTH/64 doubles
 template <int NTH = NT>
 __device__ __forceinline__ double block_sum(double v, double *sh) {
