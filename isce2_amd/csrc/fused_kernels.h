@@ -267,6 +267,7 @@ __device__ __forceinline__ int row_slot(int r, int P) { return (r >> 1) * P + (r
 template <int H, int K>
 __device__ __forceinline__ void mac_rows(cf (&q)[K], const cf *S, int Sp, const cf *Tsp, int Tp, int t_off,
                                          int i_beg, int i_end, int u0, int SH, int f) {
+    const bool dc = f == 0;
     #pragma unroll 1
     for (int i0 = i_beg; i0 < i_end; i0 += 8) {
         cf sw[K + 7];
@@ -279,19 +280,14 @@ __device__ __forceinline__ void mac_rows(cf (&q)[K], const cf *S, int Sp, const 
         for (int i = 0; i < 8; ++i) {
             if (i0 + i < i_end) {
                 const cf tv = Tsp[row_slot<H>(i0 + i - t_off, Tp) + f];
-                if (f == 0) {
+                // one branch-free form for every lane: conj(t) s, or for slot 0 the
+                // component-wise product (t.x s.x, t.y s.y)
+                const float c0 = tv.x, c1 = dc ? 0.f : tv.y, c2 = dc ? tv.y : tv.x, c3 = dc ? 0.f : -tv.y;
 #pragma unroll
-                    for (int k = 0; k < K; ++k) {
-                        q[k].x += tv.x * sw[i + k].x;
-                        q[k].y += tv.y * sw[i + k].y;
-                    }
-                } else {
-#pragma unroll
-                    for (int k = 0; k < K; ++k) {
-                        const cf s = sw[i + k];
-                        q[k].x += tv.x * s.x + tv.y * s.y;
-                        q[k].y += tv.x * s.y - tv.y * s.x;
-                    }
+                for (int k = 0; k < K; ++k) {
+                    const cf s = sw[i + k];
+                    q[k].x += c0 * s.x + c1 * s.y;
+                    q[k].y += c2 * s.y + c3 * s.x;
                 }
             }
         }
