@@ -55,6 +55,16 @@ _STR_PROPS = {
     "snrImageName": "snr.snr", "covImageName": "cov.cov",
 }
 _ALIASES = {"grossOffsetDownStatic": 0, "grossOffsetAcrossStatic": 0}
+
+
+def _to_device(img, cplx, dev):
+    """Raster (memory-mapped, possibly read-only or strided) -> device tensor,
+    staged through one pinned host buffer."""
+    host = torch.empty(img.shape, dtype=torch.complex64 if cplx else torch.float32, pin_memory=True)
+    host.numpy()[...] = img
+    out = host.to(dev, non_blocking=True)
+    torch.cuda.current_stream(dev).synchronize()
+    return out
 _MODES = {"auto": MODE_AUTO, "staged": MODE_STAGED, "fused": MODE_FUSED}
 
 
@@ -216,9 +226,8 @@ class PyCuAmpcor:
             raise raster.RasterError("reference and secondary rasters must both be complex or both real")
         dev = torch.device("cuda", v["deviceID"])
         t0 = time.perf_counter()
-        dt = np.complex64 if cplx else np.float32
-        ref = torch.from_numpy(np.ascontiguousarray(ref_np, dtype=dt)).to(dev, non_blocking=False)
-        sec = torch.from_numpy(np.ascontiguousarray(sec_np, dtype=dt)).to(dev, non_blocking=False)
+        ref = _to_device(ref_np, cplx, dev)
+        sec = _to_device(sec_np, cplx, dev)
         out = self.run_arrays(ref, sec)
         torch.cuda.synchronize(dev)
         st["seconds"] = time.perf_counter() - t0

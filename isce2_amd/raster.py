@@ -35,13 +35,7 @@ class RasterError(IOError):
     pass
 
 
-def _from_vrt(path: Path):
-    root = ET.parse(path).getroot()
-    w, h = int(root.get("rasterXSize")), int(root.get("rasterYSize"))
-    bands = root.findall("VRTRasterBand")
-    if not bands:
-        raise RasterError(f"{path}: no VRTRasterBand")
-    b = bands[0]
+def _vrt_band(path: Path, b, w: int, h: int):
     if b.get("subClass") != "VRTRawRasterBand":
         raise RasterError(f"{path}: only VRTRawRasterBand is supported")
     dt = _GDAL_TYPES.get(b.get("dataType"))
@@ -64,11 +58,23 @@ def _from_vrt(path: Path):
     need = (h - 1) * line_off + (w - 1) * pix_off + item
     if raw.size * item < need:
         raise RasterError(f"{fn}: file smaller than the VRT geometry")
-    arr = np.lib.stride_tricks.as_strided(raw, shape=(h, w), strides=strides, writeable=False)
-    return arr
+    return np.lib.stride_tricks.as_strided(raw, shape=(h, w), strides=strides, writeable=False)
 
 
-def _from_isce_xml(path: Path):
+def _vrt_bands(path: Path):
+    root = ET.parse(path).getroot()
+    w, h = int(root.get("rasterXSize")), int(root.get("rasterYSize"))
+    bands = root.findall("VRTRasterBand")
+    if not bands:
+        raise RasterError(f"{path}: no VRTRasterBand")
+    return [_vrt_band(path, b, w, h) for b in bands]
+
+
+def _from_vrt(path: Path):
+    return _vrt_bands(path)[0]
+
+
+def _isce_xml_bands(path: Path):
     root = ET.parse(path).getroot()
     props = {}
     for p in root.iter("property"):
@@ -80,15 +86,42 @@ def _from_isce_xml(path: Path):
     dt = _ISCE_TYPES.get(props.get("data_type", "CFLOAT").upper())
     if dt is None:
         raise RasterError(f"{path}: unsupported data_type")
-    bands = int(props.get("number_bands", "1"))
-    if bands != 1:
-        raise RasterError(f"{path}: only single-band images are supported")
+    nb = int(props.get("number_bands", "1"))
     fn = props.get("file_name", str(path)[:-4])
-    if not os.path.isabs(fn):
+    if not os.path.isabs(fn) or not os.path.exists(fn):
         fn = str(path.parent / os.path.basename(fn))
     order = props.get("byte_order", "l").lower()
     dtype = np.dtype(dt).newbyteorder("<" if order.startswith("l") else ">")
-    return np.memmap(fn, dtype=dtype, mode="r", shape=(h, w))
+    scheme = props.get("scheme", "BIP").upper()
+    if scheme == "BIL":
+        a = np.memmap(fn, dtype=dtype, mode="r", shape=(h, nb, w))
+        return [a[:, i, :] for i in range(nb)]
+    if scheme == "BSQ":
+        a = np.memmap(fn, dtype=dtype, mode="r", shape=(nb, h, w))
+        return [a[i] for i in range(nb)]
+    a = np.memmap(fn, dtype=dtype, mode="r", shape=(h, w, nb))
+    return [a[:, :, i] for i in range(nb)]
+
+
+def _from_isce_xml(path: Path):
+    bands = _isce_xml_bands(path)
+    if len(bands) != 1:
+        raise RasterError(f"{path}: only single-band images are supported")
+    return bands[0]
+
+
+def open_bands(name: str):
+    """Every band of a multi-band raster (``.vrt`` or ISCE ``.xml`` sidecar, BIP/BIL/BSQ)
+    as read-only 2-D arrays; single-band formats give a one-element list."""
+    p = Path(name)
+    if p.suffix == ".vrt" and p.exists():
+        return _vrt_bands(p)
+    if p.suffix == ".xml" and p.exists():
+        return _isce_xml_bands(p)
+    for side in (Path(str(p) + ".vrt"), Path(str(p) + ".xml")):
+        if side.exists():
+            return open_bands(str(side))
+    return [open_raster(name)]
 
 
 def open_raster(name: str, height: int | None = None, width: int | None = None):
@@ -142,7 +175,7 @@ def write_isce_xml(name: str, width: int, length: int, bands: int, data_type: st
 def write_vrt(name: str, width: int, length: int, bands: int, data_type: str = "Float32",
               scheme: str = "BIP") -> None:
     """GDAL VRTRawRasterBand sidecar (isceobj/Image/Image.py:461-579 layout rules)."""
-    item = {"Float32": 4, "CFloat32": 8}[data_type]
+    item = {"Byte": 1, "Int16": 2, "Int32": 4, "Float32": 4, "Float64": 8, "CFloat32": 8, "CFloat64": 16}[data_type]
     root = ET.Element("VRTDataset", rasterXSize=str(width), rasterYSize=str(length))
     for b in range(bands):
         if scheme == "BIP":
