@@ -35,6 +35,27 @@ HBM_BYTES_PER_WINDOW = {k: v * 1024 / 7812 for k, v in {
 HBM_PEAK_GBS = 8000.0
 
 
+def bench_offset_qa(gpu_out, ref_d, ref_a, W, skip):
+    """Host quality control of the field's offsets: SNR cull + offoutliers, and the
+    iterative affine fit (L2 on the whole field; L1 on a 4096-offset subsample)."""
+    from isce2_amd.offset_qa import OffsetField, fitoff, offoutliers
+    off = gpu_out["offset"]
+    f = OffsetField(ref_a + W // 2, off[:, 1], ref_d + W // 2, off[:, 0], gpu_out["snr"])
+    t0 = time.perf_counter()
+    oo = offoutliers(f, distance=2.0, snr_threshold=2.0)
+    t1 = time.perf_counter()
+    l2 = fitoff(f, nsig=1.5, maxrms=0.08, minpoint=50, l1norm=False)
+    t2 = time.perf_counter()
+    sub = f.take(np.random.default_rng(3).choice(len(f), size=min(4096, len(f)), replace=False))
+    l1 = fitoff(sub, l1norm=True)
+    t3 = time.perf_counter()
+    return {"offsets": len(f), "offoutliers_ms": round((t1 - t0) * 1e3, 2), "offoutliers_kept": int(len(oo["index"])),
+            "fitoff_l2_ms": round((t2 - t1) * 1e3, 2), "fitoff_l2_status": l2["status"],
+            "fitoff_l2_kept": int(len(l2["index"])), "fitoff_l1_4096_ms": round((t3 - t2) * 1e3, 2),
+            "fitoff_l1_status": l1["status"], "note": "host numpy/scipy; the field is warped, so the affine fit "
+            "culls toward the best affine subset"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -220,6 +241,7 @@ def main():
         truth = np.stack([dd[cr, cc], da[cr, cc]], axis=1)
         accuracy["median_abs_err_vs_true_field_px"] = float(np.median(np.abs(g_off - truth)))
 
+    qa = bench_offset_qa(gpu_out, ref_d, ref_a, W, SKIP)
     uneven = None if args.no_uneven else bench_uneven(dev, args.uneven_pairs, max(1, args.steps // 2))
     serving = None if args.no_serve else bench_serving(dev, ref_np, sec_np, args.serve_jobs, args.serve_rate)
     stress = None if args.no_stress else bench_precision_stress(dev)
@@ -238,6 +260,7 @@ def main():
                       "rerun_fraction": round(rerun / max(1, checked), 5)},
         "roofline": roof, "cpu_baseline": cpu, "kernels_untimed_pass": kernels, "plain_staged": plain, "accuracy": accuracy,
         "host_resident_inputs": pcie, "uneven_batch": uneven, "serving": serving, "precision_stress": stress,
+        "offset_qa": qa,
         "scene_gen_s": round(gen_s, 1),
     }
     print(json.dumps(line), flush=True)
