@@ -351,7 +351,7 @@ void Engine::run_staged_t(const RunArgs &a) {
         // argmax, covariance, SNR, zoom start/shift (cuAmpcorChunk.cpp:69-106)
         hipLaunchKernelGGL(k_maxloc<T>, dim3(n), dim3(256), 0, st, corr, sz.raw_corr_h, sz.raw_corr_w, loc, val,
                            a.margin_coarse ? a.margin_coarse + w0 : nullptr);
-        hipLaunchKernelGGL(k_stats<T>, dim3(idivup(n, 64)), dim3(64), 0, st, corr, sz.raw_corr_h, sz.raw_corr_w, n,
+        hipLaunchKernelGGL(k_stats<T>, dim3(n), dim3(64), 0, st, corr, sz.raw_corr_h, sz.raw_corr_w, n,
                            loc, val, W_h * W_w, sz.corr_raw_zoom_h, sz.corr_raw_zoom_w, p.hs_down, p.hs_across,
                            sz.half_zoom, a.cov + 3 * size_t(w0), a.snr + w0, start, shift);
         // reference oversampling (cuAmpcorChunk.cpp:115-126)

@@ -151,11 +151,12 @@ __global__ void __launch_bounds__(NT, (K >= 9 ? 2 : 4)) k_coarse(CoarseArgs a, c
     const Top2<float> top = block_argmax(corr, NLh * NLw, red);
     const int idx = top.index();
     const float val = top.v1;
-    if (threadIdx.x == 0) {
-        if (a.margin) a.margin[win] = top.margin();
-        const int px = idx / NLw, py = idx - px * NLw;
+    const int px = idx / NLw, py = idx - px * NLw;
+    if (threadIdx.x < 64)      // wave 0: SNR patch sum over its lanes, covariance on lane 0
         window_stats(corr, NLh, NLw, px, py, double(val), W * W, a.patch_h, a.patch_w, a.cov + 3 * win,
                      a.snr + win);
+    if (threadIdx.x == 0) {
+        if (a.margin) a.margin[win] = top.margin();
         int s0, h0, s1, h1;
         adjust_axis(px, a.hs_d, a.hz, s0, h0);
         adjust_axis(py, a.hs_a, a.hz, s1, h1);

@@ -403,9 +403,30 @@ __device__ __forceinline__ void adjust_axis(int maxloc, int old_range, int new_r
     else if (start > rbound) { shift = start - rbound; start = rbound; }
 }
 
+// Covariance and SNR of one window's normalized surface, computed by one wave:
+// lanes share the SNR patch sum, lane 0 writes both results.
 template <class T>
 __device__ void window_stats(const T *c, int nx, int ny, int px, int py, double peak, int template_size,
                              int patch_h, int patch_w, float *cov3, float *snr) {
+    const int lane = threadIdx.x & 63;
+    // SNR over the patch centred on the peak (offset -size/2), zero outside
+    double sum = 0.0;
+    int count = 0;
+    for (int k = lane; k < patch_h * patch_w; k += 64) {
+        const int i = k / patch_w, j = k - i * patch_w;
+        const int x = px + i - patch_h / 2, y = py + j - patch_w / 2;
+        if (x >= 0 && x < nx && y >= 0 && y < ny) {
+            const double v = double(c[x * ny + y]);
+            sum += v * v;
+            ++count;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        sum += __shfl_xor(sum, o);
+        count += __shfl_xor(count, o);
+    }
+    if (lane != 0) return;
     // covariance
     double cxx = 99.0, cyy = 99.0, cxy = 0.0;
     if (!(px - 1 < 0 || py - 1 < 0 || px + 1 >= nx || py + 1 >= ny)) {
@@ -425,38 +446,27 @@ __device__ void window_stats(const T *c, int nx, int ny, int px, int py, double 
         }
     }
     cov3[0] = float(cxx); cov3[1] = float(cyy); cov3[2] = float(cxy);
-    // SNR over the patch centred on the peak (offset -size/2), zero outside
-    double sum = 0.0;
-    int count = 0;
-    for (int i = 0; i < patch_h; ++i) {
-        const int x = px + i - patch_h / 2;
-        if (x < 0 || x >= nx) continue;
-        for (int j = 0; j < patch_w; ++j) {
-            const int y = py + j - patch_w / 2;
-            if (y < 0 || y >= ny) continue;
-            const double v = double(c[x * ny + y]);
-            sum += v * v;
-            ++count;
-        }
-    }
     const double mean = (sum - peak * peak) / double(count - 1);
     *snr = float(peak * peak / mean);
 }
 
+// one 64-thread block (one wave) per window
 template <class T>
-__global__ void k_stats(const T *corr, int nx, int ny, int n, const int32_t *loc, const T *val, int template_size,
-                        int patch_h, int patch_w, int hs_down, int hs_across, int hz,
-                        float *cov, float *snr, int32_t *start, int32_t *shift) {
-    const int w = blockIdx.x * blockDim.x + threadIdx.x;
-    if (w >= n) return;
+__global__ void __launch_bounds__(64) k_stats(const T *corr, int nx, int ny, int n, const int32_t *loc, const T *val,
+                                              int template_size, int patch_h, int patch_w, int hs_down,
+                                              int hs_across, int hz, float *cov, float *snr, int32_t *start,
+                                              int32_t *shift) {
+    const int w = blockIdx.x;
     const int px = loc[2 * w], py = loc[2 * w + 1];
     window_stats(corr + size_t(w) * nx * ny, nx, ny, px, py, double(val[w]), template_size, patch_h, patch_w,
                  cov + 3 * w, snr + w);
-    int s0, h0, s1, h1;
-    adjust_axis(px, hs_down, hz, s0, h0);
-    adjust_axis(py, hs_across, hz, s1, h1);
-    start[2 * w] = s0; start[2 * w + 1] = s1;
-    shift[2 * w] = h0; shift[2 * w + 1] = h1;
+    if (threadIdx.x == 0) {
+        int s0, h0, s1, h1;
+        adjust_axis(px, hs_down, hz, s0, h0);
+        adjust_axis(py, hs_across, hz, s1, h1);
+        start[2 * w] = s0; start[2 * w + 1] = s1;
+        shift[2 * w] = h0; shift[2 * w + 1] = h1;
+    }
 }
 
 // ---------------------------------------------------------------------------
