@@ -108,6 +108,8 @@ struct Engine {
         hipStream_t stream;
         float *margin_coarse = nullptr, *margin_fine = nullptr;   // debug: top-1 minus top-2
         bool nosync = false;                                       // AMPCOR_FLAG_NOSYNC
+        // staged pipeline from the zoom stage on: per-window coarse zoom start and shift
+        const int32_t *fine_start = nullptr, *fine_shift = nullptr;
     };
     // precision: 0 FP32 only; 1 FP32 + FP64 re-run of windows whose argmax
     // margins fall below the thresholds; 2 FP64 staged pipeline for every window
@@ -123,7 +125,8 @@ struct Engine {
     void run(const RunArgs &a);
     void run_staged(const RunArgs &a);
     template <class T> void run_staged_t(const RunArgs &a);
-    void rerun_fp64(const RunArgs &a, const float *mc, const float *mf);
+    void rerun_fp64(const RunArgs &a, const float *mc, const float *mf, const int32_t *start32 = nullptr,
+                    const int32_t *shift32 = nullptr);
     void run_fused(const RunArgs &a);
 };
 

@@ -324,36 +324,46 @@ void Engine::run_staged_t(const RunArgs &a) {
     T *r_adj = buf<T>(tn + "r_adj", size_t(nb) * Z * Z);
     if (p.surf_method) ensure_sinc_table();
 
+    // fine-only mode: the coarse stage's zoom start and shift are given (a
+    // re-run of windows whose coarse peak is not in doubt)
+    const bool fine_only = a.fine_start != nullptr;
     for (int w0 = 0; w0 < a.n; w0 += nb) {
         const int n = std::min(nb, a.n - w0);
         // chip load + amplitude + mean (cuAmpcorChunk.cpp:18-39)
         hipLaunchKernelGGL(k_gather<T>, grid2(W_h * W_w, n), dim3(256), 0, st, a.ref, a.ref_pitch,
                            a.ref_d + w0, a.ref_a + w0, W_h, W_w, mode, c_ref);
-        hipLaunchKernelGGL(k_abs<T>, dim3(idivup(n * W_h * W_w, 256)), dim3(256), 0, st, c_ref, r_ref,
-                           int64_t(n) * W_h * W_w);
-        hipLaunchKernelGGL(k_sub_mean<T>, dim3(n), dim3(per_window_threads(W_h * W_w)), 0, st, r_ref, W_h * W_w,
-                           ssd_raw);
         hipLaunchKernelGGL(k_gather<T>, grid2(S_h * S_w, n), dim3(256), 0, st, a.sec, a.sec_pitch,
                            a.sec_d + w0, a.sec_a + w0, S_h, S_w, mode, c_sec);
-        hipLaunchKernelGGL(k_abs<T>, dim3(idivup(n * S_h * S_w, 256)), dim3(256), 0, st, c_sec, r_sec,
-                           int64_t(n) * S_h * S_w);
-        // raw correlation (cuAmpcorChunk.cpp:48-52)
-        hipLaunchKernelGGL(k_pack_corr<T>, grid2(L1h * L1w, n), dim3(256), 0, st, r_ref, W_h, W_w, r_sec, S_h, S_w,
-                           zc, L1h, L1w);
-        fft2<T, -1>(*this, zc, n, L1h, L1w, st);
-        hipLaunchKernelGGL(k_spec_product<T>, grid2(L1h * L1w, n), dim3(256), 0, st, zc, L1h, L1w,
-                           T(1.0 / (double(L1h) * L1w)));
-        fft2_rows_kept<T, +1>(*this, zc, n, L1h, L1w, sz.raw_corr_h, st);
-        hipLaunchKernelGGL(k_crop_real<T>, grid2(sz.raw_corr_h * sz.raw_corr_w, n), dim3(256), 0, st, zc, L1h, L1w,
-                           corr, sz.raw_corr_h, sz.raw_corr_w);
-        // normalize (cuAmpcorChunk.cpp:60)
-        normalize(corr, sz.raw_corr_h, sz.raw_corr_w, r_ref, W_h, W_w, r_sec, S_h, S_w, ssd_raw, n);
-        // argmax, covariance, SNR, zoom start/shift (cuAmpcorChunk.cpp:69-106)
-        hipLaunchKernelGGL(k_maxloc<T>, dim3(n), dim3(256), 0, st, corr, sz.raw_corr_h, sz.raw_corr_w, loc, val,
-                           a.margin_coarse ? a.margin_coarse + w0 : nullptr);
-        hipLaunchKernelGGL(k_stats<T>, dim3(n), dim3(64), 0, st, corr, sz.raw_corr_h, sz.raw_corr_w, n,
-                           loc, val, W_h * W_w, sz.corr_raw_zoom_h, sz.corr_raw_zoom_w, p.hs_down, p.hs_across,
-                           sz.half_zoom, a.cov + 3 * size_t(w0), a.snr + w0, start, shift);
+        if (fine_only) {
+            HIP_OK(hipMemcpyAsync(start, a.fine_start + 2 * size_t(w0), sizeof(int32_t) * 2 * n,
+                                  hipMemcpyDeviceToDevice, st));
+            HIP_OK(hipMemcpyAsync(shift, a.fine_shift + 2 * size_t(w0), sizeof(int32_t) * 2 * n,
+                                  hipMemcpyDeviceToDevice, st));
+        } else {
+            hipLaunchKernelGGL(k_abs<T>, dim3(idivup(n * W_h * W_w, 256)), dim3(256), 0, st, c_ref, r_ref,
+                               int64_t(n) * W_h * W_w);
+            hipLaunchKernelGGL(k_sub_mean<T>, dim3(n), dim3(per_window_threads(W_h * W_w)), 0, st, r_ref, W_h * W_w,
+                               ssd_raw);
+            hipLaunchKernelGGL(k_abs<T>, dim3(idivup(n * S_h * S_w, 256)), dim3(256), 0, st, c_sec, r_sec,
+                               int64_t(n) * S_h * S_w);
+            // raw correlation (cuAmpcorChunk.cpp:48-52)
+            hipLaunchKernelGGL(k_pack_corr<T>, grid2(L1h * L1w, n), dim3(256), 0, st, r_ref, W_h, W_w, r_sec, S_h, S_w,
+                               zc, L1h, L1w);
+            fft2<T, -1>(*this, zc, n, L1h, L1w, st);
+            hipLaunchKernelGGL(k_spec_product<T>, grid2(L1h * L1w, n), dim3(256), 0, st, zc, L1h, L1w,
+                               T(1.0 / (double(L1h) * L1w)));
+            fft2_rows_kept<T, +1>(*this, zc, n, L1h, L1w, sz.raw_corr_h, st);
+            hipLaunchKernelGGL(k_crop_real<T>, grid2(sz.raw_corr_h * sz.raw_corr_w, n), dim3(256), 0, st, zc, L1h, L1w,
+                               corr, sz.raw_corr_h, sz.raw_corr_w);
+            // normalize (cuAmpcorChunk.cpp:60)
+            normalize(corr, sz.raw_corr_h, sz.raw_corr_w, r_ref, W_h, W_w, r_sec, S_h, S_w, ssd_raw, n);
+            // argmax, covariance, SNR, zoom start/shift (cuAmpcorChunk.cpp:69-106)
+            hipLaunchKernelGGL(k_maxloc<T>, dim3(n), dim3(256), 0, st, corr, sz.raw_corr_h, sz.raw_corr_w, loc, val,
+                               a.margin_coarse ? a.margin_coarse + w0 : nullptr);
+            hipLaunchKernelGGL(k_stats<T>, dim3(n), dim3(64), 0, st, corr, sz.raw_corr_h, sz.raw_corr_w, n,
+                               loc, val, W_h * W_w, sz.corr_raw_zoom_h, sz.corr_raw_zoom_w, p.hs_down, p.hs_across,
+                               sz.half_zoom, a.cov + 3 * size_t(w0), a.snr + w0, start, shift);
+        }
         // reference oversampling (cuAmpcorChunk.cpp:115-126)
         if (p.deramp == 1)
             hipLaunchKernelGGL(k_deramp<T>, dim3(n), dim3(per_window_threads(W_h * W_w)), size_t(W_h + W_w) * 16, st,
@@ -417,11 +427,11 @@ void Engine::run_staged_t(const RunArgs &a) {
             hipLaunchKernelGGL(k_subpixel, dim3(idivup(n, 64)), dim3(64), 0, st, start, zidx, n, ratio, p.hs_down,
                                p.hs_across, a.offset + 2 * size_t(w0));
         }
-        if (a.maxloc) HIP_OK(hipMemcpyAsync(a.maxloc + 2 * size_t(w0), loc, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToDevice, st));
+        if (a.maxloc && !fine_only) HIP_OK(hipMemcpyAsync(a.maxloc + 2 * size_t(w0), loc, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToDevice, st));
         if (a.start) HIP_OK(hipMemcpyAsync(a.start + 2 * size_t(w0), start, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToDevice, st));
         if (a.shift) HIP_OK(hipMemcpyAsync(a.shift + 2 * size_t(w0), shift, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToDevice, st));
         if (a.zoom_idx) HIP_OK(hipMemcpyAsync(a.zoom_idx + 2 * size_t(w0), zidx, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToDevice, st));
-        if (a.maxval) hipLaunchKernelGGL(k_to_float<T>, dim3(idivup(n, 256)), dim3(256), 0, st, val, a.maxval + w0, n);
+        if (a.maxval && !fine_only) hipLaunchKernelGGL(k_to_float<T>, dim3(idivup(n, 256)), dim3(256), 0, st, val, a.maxval + w0, n);
         HIP_OK(hipGetLastError());
     }
 }
@@ -481,51 +491,71 @@ __global__ void k_put(const U *src, const int32_t *idx, U *dst, int n, int width
 // fine argmax margin (top-1 minus top-2 of the normalized surface) is below
 // its threshold, and overwrite that window's outputs. FP32 rounding can only
 // reorder near-equal candidates, so this makes the integer peaks those of an
-// FP64 evaluation.
-void Engine::rerun_fp64(const RunArgs &a, const float *mc, const float *mf) {
+// FP64 evaluation. When the coarse zoom start and shift of every window are
+// given (start32, shift32) and only a window's fine margin is small, only its
+// fine stage is re-run, from that start: its coarse peak, SNR and covariance
+// stand as computed.
+void Engine::rerun_fp64(const RunArgs &a, const float *mc, const float *mf, const int32_t *start32,
+                        const int32_t *shift32) {
     std::vector<float> hc(a.n), hf(a.n);
     HIP_OK(hipMemcpyAsync(hc.data(), mc, sizeof(float) * a.n, hipMemcpyDeviceToHost, a.stream));
     HIP_OK(hipMemcpyAsync(hf.data(), mf, sizeof(float) * a.n, hipMemcpyDeviceToHost, a.stream));
     HIP_OK(hipStreamSynchronize(a.stream));
-    std::vector<int32_t> idx;
-    for (int i = 0; i < a.n; ++i)
-        if (!(hc[i] >= tau_coarse) || !(hf[i] >= tau_fine)) idx.push_back(i);
+    std::vector<int32_t> full, fine;
+    for (int i = 0; i < a.n; ++i) {
+        const bool c = !(hc[i] >= tau_coarse), f = !(hf[i] >= tau_fine);
+        if (c || (f && !start32)) full.push_back(i);
+        else if (f) fine.push_back(i);
+    }
     prec_checked += a.n;
-    prec_rerun += long(idx.size());
-    if (idx.empty()) return;
-    const int m = int(idx.size());
-    int32_t *d_idx = buf<int32_t>("p_idx", m);
-    HIP_OK(hipMemcpyAsync(d_idx, idx.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, a.stream));
-    int32_t *st4 = buf<int32_t>("p_starts", size_t(4) * m);
-    const int32_t *srcs[4] = {a.ref_d, a.ref_a, a.sec_d, a.sec_a};
-    for (int k = 0; k < 4; ++k)
-        hipLaunchKernelGGL(k_take_i32, dim3(idivup(m, 256)), dim3(256), 0, a.stream, srcs[k], d_idx, st4 + k * m, m, 1);
-    float *o_off = buf<float>("p_off", size_t(2) * m), *o_snr = buf<float>("p_snr", m);
-    float *o_cov = buf<float>("p_cov", size_t(3) * m), *o_val = buf<float>("p_val", m);
-    int32_t *o_i = buf<int32_t>("p_ints", size_t(8) * m);
-    RunArgs sub = a;
-    sub.ref_d = st4; sub.ref_a = st4 + m; sub.sec_d = st4 + 2 * m; sub.sec_a = st4 + 3 * m;
-    sub.n = m;
-    sub.margin_coarse = sub.margin_fine = nullptr;
-    sub.offset = o_off; sub.snr = o_snr; sub.cov = o_cov; sub.maxval = o_val;
-    sub.maxloc = o_i; sub.start = o_i + 2 * m; sub.shift = o_i + 4 * m; sub.zoom_idx = o_i + 6 * m;
-    sub.chunk = 0;
-    run_staged_t<double>(sub);
-    auto put_f = [&](const float *src, float *dst, int width) {
-        if (dst) hipLaunchKernelGGL(k_put<float>, dim3(idivup(m * width, 256)), dim3(256), 0, a.stream, src, d_idx, dst, m, width);
+    prec_rerun += long(full.size() + fine.size());
+    auto subset = [&](const std::vector<int32_t> &idx, bool fine_only) {
+        if (idx.empty()) return;
+        const int m = int(idx.size());
+        const std::string pre = fine_only ? "q_" : "p_";
+        int32_t *d_idx = buf<int32_t>(pre + "idx", m);
+        HIP_OK(hipMemcpyAsync(d_idx, idx.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, a.stream));
+        int32_t *st4 = buf<int32_t>(pre + "starts", size_t(8) * m);
+        const int32_t *srcs[4] = {a.ref_d, a.ref_a, a.sec_d, a.sec_a};
+        for (int k = 0; k < 4; ++k)
+            hipLaunchKernelGGL(k_take_i32, dim3(idivup(m, 256)), dim3(256), 0, a.stream, srcs[k], d_idx, st4 + k * m, m, 1);
+        float *o_off = buf<float>(pre + "off", size_t(2) * m), *o_snr = buf<float>(pre + "snr", m);
+        float *o_cov = buf<float>(pre + "cov", size_t(3) * m), *o_val = buf<float>(pre + "val", m);
+        int32_t *o_i = buf<int32_t>(pre + "ints", size_t(8) * m);
+        RunArgs sub = a;
+        sub.ref_d = st4; sub.ref_a = st4 + m; sub.sec_d = st4 + 2 * m; sub.sec_a = st4 + 3 * m;
+        sub.n = m;
+        sub.margin_coarse = sub.margin_fine = nullptr;
+        sub.offset = o_off; sub.snr = o_snr; sub.cov = o_cov; sub.maxval = o_val;
+        sub.maxloc = o_i; sub.start = o_i + 2 * m; sub.shift = o_i + 4 * m; sub.zoom_idx = o_i + 6 * m;
+        sub.chunk = 0;
+        if (fine_only) {
+            hipLaunchKernelGGL(k_take_i32, dim3(idivup(2 * m, 256)), dim3(256), 0, a.stream, start32, d_idx, st4 + 4 * m, m, 2);
+            hipLaunchKernelGGL(k_take_i32, dim3(idivup(2 * m, 256)), dim3(256), 0, a.stream, shift32, d_idx, st4 + 6 * m, m, 2);
+            sub.fine_start = st4 + 4 * m;
+            sub.fine_shift = st4 + 6 * m;
+        }
+        run_staged_t<double>(sub);
+        auto put_f = [&](const float *src, float *dst, int width) {
+            if (dst) hipLaunchKernelGGL(k_put<float>, dim3(idivup(m * width, 256)), dim3(256), 0, a.stream, src, d_idx, dst, m, width);
+        };
+        auto put_i = [&](const int32_t *src, int32_t *dst) {
+            if (dst) hipLaunchKernelGGL(k_put<int32_t>, dim3(idivup(m * 2, 256)), dim3(256), 0, a.stream, src, d_idx, dst, m, 2);
+        };
+        put_f(o_off, a.offset, 2);
+        put_i(sub.zoom_idx, a.zoom_idx);
+        if (!fine_only) {
+            put_f(o_snr, a.snr, 1);
+            put_f(o_cov, a.cov, 3);
+            put_f(o_val, a.maxval, 1);
+            put_i(sub.maxloc, a.maxloc);
+            put_i(sub.start, a.start);
+            put_i(sub.shift, a.shift);
+        }
+        HIP_OK(hipGetLastError());
     };
-    auto put_i = [&](const int32_t *src, int32_t *dst) {
-        if (dst) hipLaunchKernelGGL(k_put<int32_t>, dim3(idivup(m * 2, 256)), dim3(256), 0, a.stream, src, d_idx, dst, m, 2);
-    };
-    put_f(o_off, a.offset, 2);
-    put_f(o_snr, a.snr, 1);
-    put_f(o_cov, a.cov, 3);
-    put_f(o_val, a.maxval, 1);
-    put_i(sub.maxloc, a.maxloc);
-    put_i(sub.start, a.start);
-    put_i(sub.shift, a.shift);
-    put_i(sub.zoom_idx, a.zoom_idx);
-    HIP_OK(hipGetLastError());
+    subset(full, false);
+    subset(fine, true);
 }
 
 void Engine::run(const RunArgs &in) {

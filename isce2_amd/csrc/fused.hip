@@ -687,7 +687,7 @@ void Engine::run_fused(const RunArgs &a) {
         }
         HIP_OK(hipGetLastError());
     }
-    if (precision == 1 && !a.nosync) rerun_fp64(a, mc, mf);
+    if (precision == 1 && !a.nosync) rerun_fp64(a, mc, mf, start, shift);
 }
 
 }  // namespace ampcor
