@@ -153,6 +153,13 @@ class Autotuner:
                 fe = min(times, key=times.get)
                 eng.set_option("fft_block_elems", fe)
                 best.options["fft_block_elems"] = fe
+                times = {}
+                for cw in (0, 8, 16):
+                    eng.set_option("fft_cols_width", cw)
+                    times[cw] = self._time(eng, sub, MODE_STAGED, min(best.chunk, m), m)
+                cw = min(times, key=times.get)
+                eng.set_option("fft_cols_width", cw)
+                best.options["fft_cols_width"] = cw
                 for n_fft in staged_lengths(sizes, params):
                     orders = factorizations(n_fft)
                     if len(orders) < 2:

@@ -119,6 +119,8 @@ struct Engine {
     int fine_threads = 1024;
     // complex elements per workgroup of the staged FFT kernels (float; double uses half)
     int fft_block_elems = 4096;
+    // columns per workgroup of the FP32 staged column FFTs (0: from fft_block_elems)
+    int fft_cols_width = 8;
     int precision = 1;
     float tau_coarse = 2e-6f, tau_fine = 2e-6f;
     long long prec_checked = 0, prec_rerun = 0;

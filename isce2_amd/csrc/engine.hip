@@ -215,11 +215,15 @@ void fft_rows(const FftPlan &pl, cx<T> *data, int64_t nrows, int elems, hipStrea
 }
 
 template <class T, int DIR>
-void fft_cols(const FftPlan &pl, cx<T> *data, int nimg, int w, int elems, hipStream_t st) {
+void fft_cols(const FftPlan &pl, cx<T> *data, int nimg, int w, int elems, hipStream_t st, int width = 0) {
     const FftDesc &d = std::is_same<T, float>::value ? pl.desc32 : pl.desc64;
-    const int c = std::max(1, std::min(w, std::min(16, elems / pl.n)));
+    // columns per block: from the element budget, or a fixed width (whole
+    // 64/128-byte column segments per row) when the LDS for it is available
+    int c = std::max(1, std::min(w, std::min(16, elems / pl.n)));
+    if (width > 0 && 2 * size_t(width) * pl.n * sizeof(cx<T>) <= 160 * 1024) c = std::min(w, width);
     const size_t lds = 2 * size_t(c) * pl.n * sizeof(cx<T>);
-    hipLaunchKernelGGL((k_fft_cols<T, DIR>), dim3(idivup(w, c), nimg), dim3(256), lds, st, data, w, c, d);
+    const int threads = c >= 16 ? 512 : 256;
+    hipLaunchKernelGGL((k_fft_cols<T, DIR>), dim3(idivup(w, c), nimg), dim3(threads), lds, st, data, w, c, d);
 }
 
 template <class T, int DIR>
@@ -235,20 +239,24 @@ void fft_rows_sel(const FftPlan &pl, cx<T> *data, int nimg, int H, int sel, int 
 template <class T>
 int fft_elems(const Engine &e) { return sizeof(T) == 8 ? std::max(1024, e.fft_block_elems / 2) : e.fft_block_elems; }
 
+// fixed column-block width for the FP32 column FFTs (the FP64 ones use the element budget)
+template <class T>
+int cols_width(const Engine &e) { return sizeof(T) == 4 ? e.fft_cols_width : 0; }
+
 // inverse 2D FFT of a quadrant-padded spectrum (k_quadpad of an h-row chip into
 // H rows): the row pass skips the all-zero rows
 template <class T, int DIR>
 void fft2_padded(Engine &e, cx<T> *data, int nimg, int H, int W, int h, hipStream_t st) {
     const int h2 = h / 2;
     fft_rows_sel<T, DIR>(e.plan(W), data, nimg, H, 2 * h2, h2, H - h2, fft_elems<T>(e), st);
-    fft_cols<T, DIR>(e.plan(H), data, nimg, W, fft_elems<T>(e), st);
+    fft_cols<T, DIR>(e.plan(H), data, nimg, W, fft_elems<T>(e), st, cols_width<T>(e));
 }
 
 // 2D FFT of which only rows [0, keep) of the result are used: columns first,
 // then only those rows
 template <class T, int DIR>
 void fft2_rows_kept(Engine &e, cx<T> *data, int nimg, int h, int w, int keep, hipStream_t st) {
-    fft_cols<T, DIR>(e.plan(h), data, nimg, w, fft_elems<T>(e), st);
+    fft_cols<T, DIR>(e.plan(h), data, nimg, w, fft_elems<T>(e), st, cols_width<T>(e));
     fft_rows_sel<T, DIR>(e.plan(w), data, nimg, h, keep, keep, 0, fft_elems<T>(e), st);
 }
 
@@ -257,7 +265,7 @@ template <class T, int DIR>
 void fft2(Engine &e, cx<T> *data, int nimg, int h, int w, hipStream_t st) {
     // double elements take twice the LDS: half the element budget (fft_elems)
     fft_rows<T, DIR>(e.plan(w), data, int64_t(nimg) * h, fft_elems<T>(e), st);
-    fft_cols<T, DIR>(e.plan(h), data, nimg, w, fft_elems<T>(e), st);
+    fft_cols<T, DIR>(e.plan(h), data, nimg, w, fft_elems<T>(e), st, cols_width<T>(e));
 }
 
 inline dim3 grid2(int per_img, int nimg) { return dim3(std::min(idivup(per_img, 256), 64), nimg); }
@@ -715,6 +723,10 @@ int ampcor_set_option(ampcor_ctx *ctx, const char *name, int64_t value) {
             if (value != 1024 && value != 2048 && value != 4096 && value != 8192)
                 throw ampcor::Error(AMPCOR_E_ARG, "fft_block_elems must be 1024, 2048, 4096 or 8192");
             ctx->eng.fft_block_elems = int(value);
+        } else if (k == "fft_cols_width") {
+            if (value != 0 && value != 8 && value != 16)
+                throw ampcor::Error(AMPCOR_E_ARG, "fft_cols_width must be 0, 8 or 16");
+            ctx->eng.fft_cols_width = int(value);
         } else if (k == "fine_threads") {
             if (value != 512 && value != 1024) throw ampcor::Error(AMPCOR_E_ARG, "fine_threads must be 512 or 1024");
             ctx->eng.fine_threads = int(value);

@@ -58,7 +58,7 @@ __global__ void __launch_bounds__(256) k_fft_rows_sel(cx<T> *data, int nimg, int
 
 // columns: images [nimg][h][w] with h = d.n; `c` columns per block
 template <class T, int DIR>
-__global__ void __launch_bounds__(256) k_fft_cols(cx<T> *data, int w, int c, FftDesc d) {
+__global__ void __launch_bounds__(512) k_fft_cols(cx<T> *data, int w, int c, FftDesc d) {
     extern __shared__ unsigned char smem_raw[];
     cx<T> *a = reinterpret_cast<cx<T> *>(smem_raw);
     cx<T> *b = a + c * d.n;

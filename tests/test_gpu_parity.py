@@ -172,7 +172,7 @@ def test_staged_chunk_invariance(eng, scene1):
 
 
 def test_launch_options_do_not_change_results(eng, scene1):
-    """fine_threads (fused) and fft_block_elems (staged) change only launch shapes."""
+    """fine_threads (fused), fft_block_elems and fft_cols_width (staged) change only launch shapes."""
     ref, sec = scene1
     cfg = O.AmpcorConfig(hs_down=20, hs_across=20)
     starts = _grid(5, 5, 20, 80, 20)
@@ -183,14 +183,14 @@ def test_launch_options_do_not_change_results(eng, scene1):
     st = [torch.as_tensor(np.asarray(s, np.int32)).to(dev) for s in starts]
     res = {}
     for mode, name, vals in ((MODE_FUSED, "fine_threads", (512, 1024)),
-                             (MODE_STAGED, "fft_block_elems", (1024, 2048, 4096, 8192))):
+                             (MODE_STAGED, "fft_block_elems", (1024, 2048, 4096, 8192)),
+                             (MODE_STAGED, "fft_cols_width", (0, 16, 8))):
         for v in vals:
             eng.set_option(name, v)
             r = eng.run_windows(rt, stt, *st, mode=mode, debug=True)
             torch.cuda.synchronize()
             res[(name, v)] = {k: t.cpu().numpy() for k, t in r.items()}
-        eng.set_option(name, vals[-1] if name == "fft_block_elems" else 1024)
-    eng.set_option("fft_block_elems", 4096)
+        eng.set_option(name, {"fft_block_elems": 4096, "fine_threads": 1024, "fft_cols_width": 8}[name])
     base_f, base_s = res[("fine_threads", 1024)], res[("fft_block_elems", 4096)]
     for (name, v), r in res.items():
         base = base_f if name == "fine_threads" else base_s
