@@ -76,7 +76,7 @@ template <int R1, int R2>
 __host__ __device__ __forceinline__ int dr_pos(int k) { return k / R1 + R2 * (k % R1); }
 
 template <class T, int DIR, int R1, int R2, int NTH>
-__device__ void fft_nat2dr(cx<T> *buf, int nsig, int ss, int es, const cx<T> *tw) {
+__device__ __forceinline__ void fft_nat2dr(cx<T> *buf, int nsig, int ss, int es, const cx<T> *tw) {
     constexpr int n = R1 * R2;
     #pragma unroll 1
     for (int t = threadIdx.x; t < nsig * R2; t += NTH) {
@@ -109,7 +109,7 @@ __device__ void fft_nat2dr(cx<T> *buf, int nsig, int ss, int es, const cx<T> *tw
 }
 
 template <class T, int DIR, int R1, int R2, int NTH>
-__device__ void fft_dr2nat(cx<T> *buf, int nsig, int ss, int es, const cx<T> *tw) {
+__device__ __forceinline__ void fft_dr2nat(cx<T> *buf, int nsig, int ss, int es, const cx<T> *tw) {
     #pragma unroll 1
     for (int t = threadIdx.x; t < nsig * R1; t += NTH) {
         const int s = t % nsig, k1 = t / nsig;

@@ -105,7 +105,7 @@ __device__ __forceinline__ double lane63_f64(double x) {
 }
 
 template <int CPL, int NTH = NT, class Amp>
-__device__ void window_norms(Amp amp, int SW, int TH, int TW, int NLh, int NLw, double t2, float *nrm,
+__device__ __forceinline__ void window_norms(Amp amp, int SW, int TH, int TW, int NLh, int NLw, double t2, float *nrm,
                              double *part) {
     constexpr int NB = NORM_BLOCKS;
     const int rows_per = (TH + NB - 1) / NB;
@@ -214,7 +214,7 @@ __device__ void window_norms(Amp amp, int SW, int TH, int TW, int NLh, int NLw, 
 // reads of signal j (bins k and L-k) must all happen before any write to j.
 // ---------------------------------------------------------------------------
 template <int R1, int R2, int NTH = NT>
-__device__ void split_pairs_inplace(cf *buf, int nsig, int P) {
+__device__ __forceinline__ void split_pairs_inplace(cf *buf, int nsig, int P) {
     constexpr int L = R1 * R2, H = L / 2, U = 4;
     constexpr int SIG_PER_ROUND = (NTH * U) / H;
     static_assert(SIG_PER_ROUND >= 1, "a round must hold at least one signal");
@@ -298,7 +298,7 @@ __device__ __forceinline__ void mac_rows(cf (&q)[K], const cf *S, int Sp, const 
 // for v < NLw: lag pairs (2p, 2p+1) share one complex transform placed in Y
 // (npair signals, pitch P, digit-reversed input order).
 template <int R1, int R2, int NTH = NT>
-__device__ void inverse_lags(const cf *Qs, int NLh, int NLw, cf *Y, int P, const cx<float> *tw, const float *nrm,
+__device__ __forceinline__ void inverse_lags(const cf *Qs, int NLh, int NLw, cf *Y, int P, const cx<float> *tw, const float *nrm,
                              float *corr) {
     constexpr int L = R1 * R2, H = L / 2;
     const int npair = (NLh + 1) / 2;
