@@ -183,13 +183,13 @@ struct OsArgs {
     int deramp;
     const cf *hsh;                    // [N] e^{-i pi k_s/N} / N
     float *out;                       // [n][4][N][N]: pixel (2r+p, 2c+q) in plane 2p+q at (r, c)
-    double *mean;                     // [n][2]: mean and sum of squared deviations of |.|, or null
+    double *mean;                     // [n][2]: mean and sum of squared deviations of |.| (MEAN kernels)
 };
 
 // chip (pitch N+1), shift table, FP64 deramp row/column phase factors, reduction scratch
 __host__ __device__ inline int os_lds_bytes(int N) { return al16(N * (N + 1) * 8 + N * 8) + 2 * N * 16 + 32 * 8; }
 
-template <int R1, int R2>
+template <int R1, int R2, bool MEAN>
 __global__ void __launch_bounds__(NT) k_oversample(OsArgs a, const cf *tw, int w0) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int N = R1 * R2, PC = N + 1;
@@ -258,8 +258,10 @@ __global__ void __launch_bounds__(NT) k_oversample(OsArgs a, const cf *tw, int w
             const cf v = C[r * PC + c];
             const float amp = sqrtf(v.x * v.x + v.y * v.y);
             out[size_t(2 * p + q) * N * N + r * N + c] = amp;     // whole lines per phase
-            sum += amp;
-            sum2 += double(amp) * amp;
+            if (MEAN) {
+                sum += amp;
+                sum2 += double(amp) * amp;
+            }
         }
     };
     auto interp_cols = [&]() {          // C <- C at half-row positions (every column)
@@ -297,7 +299,7 @@ __global__ void __launch_bounds__(NT) k_oversample(OsArgs a, const cf *tw, int w
     __syncthreads();
     interp_cols();
     emit(1, 1);
-    if (a.mean) {
+    if (MEAN) {
         // FP64 mean and sum of squared deviations (sum2 - sum^2 / count)
         sum = block_sum(sum, red);
         sum2 = block_sum(sum2, red);
@@ -663,10 +665,10 @@ void Engine::run_fused(const RunArgs &a) {
                     p.hs_across, 1.0f / float(p.surf_ovs * p.raw_ovs), tw256, zidx, a.offset, mf};
         if (W == 64) {
             timed("k_oversample_ref", n, st, [&] {
-                hipLaunchKernelGGL((k_oversample<8, 8>), dim3(n), dim3(NT), lds_or, st, oref, twW, w0);
+                hipLaunchKernelGGL((k_oversample<8, 8, true>), dim3(n), dim3(NT), lds_or, st, oref, twW, w0);
             });
             timed("k_oversample_search", n, st, [&] {
-                hipLaunchKernelGGL((k_oversample<8, 9>), dim3(n), dim3(NT), lds_oz, st, osec, twZ, w0);
+                hipLaunchKernelGGL((k_oversample<8, 9, false>), dim3(n), dim3(NT), lds_oz, st, osec, twZ, w0);
             });
             timed("k_fine", n, st, [&] {
                 if (fine_threads == 1024)
@@ -676,10 +678,10 @@ void Engine::run_fused(const RunArgs &a) {
             });
         } else {
             timed("k_oversample_ref", n, st, [&] {
-                hipLaunchKernelGGL((k_oversample<8, 4>), dim3(n), dim3(NT), lds_or, st, oref, twW, w0);
+                hipLaunchKernelGGL((k_oversample<8, 4, true>), dim3(n), dim3(NT), lds_or, st, oref, twW, w0);
             });
             timed("k_oversample_search", n, st, [&] {
-                hipLaunchKernelGGL((k_oversample<8, 5>), dim3(n), dim3(NT), lds_oz, st, osec, twZ, w0);
+                hipLaunchKernelGGL((k_oversample<8, 5, false>), dim3(n), dim3(NT), lds_oz, st, osec, twZ, w0);
             });
             timed("k_fine", n, st, [&] {
                 hipLaunchKernelGGL((k_fine<64, 16, 5, 2, NT>), dim3(n), dim3(NT), lds_f, st, fa, tw2, w0);
