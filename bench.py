@@ -30,8 +30,8 @@ FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per window of each fused kernel, from rocprofv3 FETCH_SIZE + WRITE_SIZE (KiB per dispatch of a
 # 7812-window launch) scaled by a 1 GiB device copy's counts (FETCH x2, WRITE x1); re-measure after kernel changes
 HBM_BYTES_PER_WINDOW = {k: v * 1024 / 7812 for k, v in {
-    "k_fine": 595099.1 * 2 + 768.1, "k_coarse": 545475.3 * 2 + 198464.4,
-    "k_oversample_ref": 326390.4 * 2 + 524544.0, "k_oversample_search": 341513.1 * 2 + 663552.0}.items()}
+    "k_fine": 594726.9 * 2 + 161.6, "k_coarse": 149361.2 * 2 + 384.0,
+    "k_oversample_ref": 122693.8 * 2 + 524416.1, "k_oversample_search": 143811.6 * 2 + 663552.0}.items()}
 HBM_PEAK_GBS = 8000.0
 
 
