@@ -324,6 +324,13 @@ void Engine::run_staged_t(const RunArgs &a) {
     cx<T> *c_sec_os = buf<cx<T>>(tn + "c_sec_os", size_t(nb) * OSh * OSw);
     T *r_sec_os = buf<T>(tn + "r_sec_os", size_t(nb) * OSh * OSw);
     T *corr_os = buf<T>(tn + "corr_os", size_t(nb) * sz.os_corr_h * sz.os_corr_w);
+    // the oversampled correlation has few lag rows (17 at raw ovs 2, zoom 16): row-spectrum path
+    const bool row_corr = sz.os_corr_h <= 18 &&
+                          size_t((sz.os_win_h + 1) / 2 + (OSh + 1) / 2) * L2w <= size_t(L2h) * L2w;
+    const int H1os = L2w / 2 + 1;
+    cx<T> *os_tspec = row_corr ? buf<cx<T>>(tn + "os_tspec", size_t(nb) * sz.os_win_h * H1os) : nullptr;
+    cx<T> *os_sspec = row_corr ? buf<cx<T>>(tn + "os_sspec", size_t(nb) * OSh * H1os) : nullptr;
+    cx<T> *os_q = row_corr ? buf<cx<T>>(tn + "os_q", size_t(nb) * sz.os_corr_h * L2w) : nullptr;
     // the 16-sparse surface kernel needs no 256x256 buffers
     const bool fast_surface = !p.surf_method && Z == 16 && So == 256;
     cx<T> *c_surf_in = fast_surface ? nullptr : buf<cx<T>>(tn + "c_surf_in", size_t(nb) * Z * Z);
@@ -397,14 +404,32 @@ void Engine::run_staged_t(const RunArgs &a) {
         hipLaunchKernelGGL(k_abs<T>, dim3(idivup(n * OSh * OSw, 256)), dim3(256), 0, st, c_sec_os, r_sec_os,
                            int64_t(n) * OSh * OSw);
         // oversampled correlation + normalize (cuAmpcorChunk.cpp:148-161)
-        hipLaunchKernelGGL(k_pack_corr<T>, grid2(L2h * L2w, n), dim3(256), 0, st, r_ref_os, sz.os_win_h,
-                           sz.os_win_w, r_sec_os, OSh, OSw, zc, L2h, L2w);
-        fft2<T, -1>(*this, zc, n, L2h, L2w, st);
-        hipLaunchKernelGGL(k_spec_product<T>, grid2(L2h * L2w, n), dim3(256), 0, st, zc, L2h, L2w,
-                           T(1.0 / (double(L2h) * L2w)));
-        fft2_rows_kept<T, +1>(*this, zc, n, L2h, L2w, sz.os_corr_h, st);
-        hipLaunchKernelGGL(k_crop_real<T>, grid2(sz.os_corr_h * sz.os_corr_w, n), dim3(256), 0, st, zc, L2h, L2w,
-                           corr_os, sz.os_corr_h, sz.os_corr_w);
+        if (row_corr) {
+            // few lag rows: row spectra + per-bin sums over template rows + NL row IFFTs
+            const int th = sz.os_win_h, NL = sz.os_corr_h, L = L2w, H1 = L / 2 + 1;
+            const int tpairs = (th + 1) / 2, spairs = (OSh + 1) / 2;
+            cx<T> *zt = zc, *zs = zc + size_t(n) * tpairs * L;
+            hipLaunchKernelGGL(k_pack_rowpairs<T>, grid2(tpairs * L, n), dim3(256), 0, st, r_ref_os, th,
+                               sz.os_win_w, L, zt);
+            hipLaunchKernelGGL(k_pack_rowpairs<T>, grid2(spairs * L, n), dim3(256), 0, st, r_sec_os, OSh, OSw, L, zs);
+            fft_rows<T, -1>(plan(L), zc, int64_t(n) * (tpairs + spairs), fft_elems<T>(*this), st);
+            hipLaunchKernelGGL(k_split_rowpairs<T>, grid2(tpairs * H1, n), dim3(256), 0, st, zt, th, L, os_tspec);
+            hipLaunchKernelGGL(k_split_rowpairs<T>, grid2(spairs * H1, n), dim3(256), 0, st, zs, OSh, L, os_sspec);
+            hipLaunchKernelGGL((k_mac_lags<T, 9>), dim3(idivup(H1, 64), idivup(NL, 9), n), dim3(64), 0, st, os_tspec,
+                               th, os_sspec, OSh, NL, L, os_q, T(1.0 / double(L)));
+            fft_rows<T, +1>(plan(L), os_q, int64_t(n) * NL, fft_elems<T>(*this), st);
+            hipLaunchKernelGGL(k_crop_real<T>, grid2(NL * sz.os_corr_w, n), dim3(256), 0, st, os_q, NL, L, corr_os,
+                               NL, sz.os_corr_w);
+        } else {
+            hipLaunchKernelGGL(k_pack_corr<T>, grid2(L2h * L2w, n), dim3(256), 0, st, r_ref_os, sz.os_win_h,
+                               sz.os_win_w, r_sec_os, OSh, OSw, zc, L2h, L2w);
+            fft2<T, -1>(*this, zc, n, L2h, L2w, st);
+            hipLaunchKernelGGL(k_spec_product<T>, grid2(L2h * L2w, n), dim3(256), 0, st, zc, L2h, L2w,
+                               T(1.0 / (double(L2h) * L2w)));
+            fft2_rows_kept<T, +1>(*this, zc, n, L2h, L2w, sz.os_corr_h, st);
+            hipLaunchKernelGGL(k_crop_real<T>, grid2(sz.os_corr_h * sz.os_corr_w, n), dim3(256), 0, st, zc, L2h, L2w,
+                               corr_os, sz.os_corr_h, sz.os_corr_w);
+        }
         normalize(corr_os, sz.os_corr_h, sz.os_corr_w, r_ref_os, sz.os_win_h, sz.os_win_w, r_sec_os, OSh, OSw,
                   ssd_os, n);
         // trim + surface oversampling + argmax + sub-pixel offset (cuAmpcorChunk.cpp:169-207)

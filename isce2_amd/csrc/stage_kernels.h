@@ -189,6 +189,80 @@ __global__ void k_spec_product(cx<T> *z, int Lh, int Lw, T scale) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Row-spectrum correlation for a few lag rows (the oversampled correlation):
+//   c[u][v] = (1/L) IFFT_L( sum_t conj(T_t) S_{t+u} )[v],   0 <= u < NL,
+// equal to the zero-padded 2-D correlation (cuCorrFrequency.cu:63-79) at every
+// valid lag for any row length L >= the search width. Rows are packed in pairs
+// into complex rows (k_pack_rowpairs), transformed along rows, split into
+// per-row half spectra (k_split_rowpairs), summed over template rows per bin
+// and lag (k_mac_lags), and only the NL lag rows go back through a row IFFT.
+// ---------------------------------------------------------------------------
+
+// rows 2j and 2j+1 of each h x w image -> complex row j of length L (zero beyond w, h)
+template <class T>
+__global__ void k_pack_rowpairs(const T *img, int h, int w, int L, cx<T> *out) {
+    const int win = blockIdx.y, np = (h + 1) / 2;
+    const T *a = img + size_t(win) * h * w;
+    cx<T> *o = out + size_t(win) * np * L;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < np * L; i += gridDim.x * blockDim.x) {
+        const int j = i / L, x = i - j * L, r = 2 * j;
+        o[i] = {x < w ? a[r * w + x] : T(0), (x < w && r + 1 < h) ? a[(r + 1) * w + x] : T(0)};
+    }
+}
+
+// Z_j = FFT(row 2j + i row 2j+1) -> spectra of rows 2j and 2j+1 at bins 0..L/2:
+// A = (Z[k] + conj Z[-k]) / 2, B = (Z[k] - conj Z[-k]) / (2i)
+template <class T>
+__global__ void k_split_rowpairs(const cx<T> *z, int h, int L, cx<T> *spec) {
+    const int win = blockIdx.y, np = (h + 1) / 2, H1 = L / 2 + 1;
+    const cx<T> *Z = z + size_t(win) * np * L;
+    cx<T> *sp = spec + size_t(win) * h * H1;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < np * H1; i += gridDim.x * blockDim.x) {
+        const int j = i / H1, k = i - j * H1;
+        const cx<T> a = Z[size_t(j) * L + k], bc = conj(Z[size_t(j) * L + (k ? L - k : 0)]);
+        sp[size_t(2 * j) * H1 + k] = (a + bc) * T(0.5);
+        if (2 * j + 1 < h) {
+            const cx<T> d = (a - bc) * T(0.5);
+            sp[size_t(2 * j + 1) * H1 + k] = {d.y, -d.x};
+        }
+    }
+}
+
+// q[u][k] = scale * sum_t conj(T_t[k]) S_{t+u}[k] for lags u0 <= u < u0 + K (u < nl),
+// written as full Hermitian rows of length L for the inverse row FFT. One
+// thread per (bin, lag group, window); it walks the search rows once, keeping
+// the template rows of its K lags in a sliding register window.
+template <class T, int K>
+__global__ void __launch_bounds__(64) k_mac_lags(const cx<T> *tspec, int th, const cx<T> *sspec, int sh, int nl,
+                                                 int L, cx<T> *q, T scale) {
+    const int H1 = L / 2 + 1, k = blockIdx.x * blockDim.x + threadIdx.x, u0 = blockIdx.y * K, win = blockIdx.z;
+    if (k >= H1) return;
+    const cx<T> *ts = tspec + size_t(win) * th * H1 + k;
+    const cx<T> *ss = sspec + size_t(win) * sh * H1 + k;
+    cx<T> acc[K], tw[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) acc[j] = tw[j] = cx<T>{T(0), T(0)};
+    const int r_end = min(sh, u0 + K - 1 + th);
+    #pragma unroll 2
+    for (int r = u0; r < r_end; ++r) {
+#pragma unroll
+        for (int j = K - 1; j > 0; --j) tw[j] = tw[j - 1];     // tw[j] = T_{r - u0 - j}
+        tw[0] = r - u0 < th ? ts[size_t(r - u0) * H1] : cx<T>{T(0), T(0)};
+        const cx<T> sv = ss[size_t(r) * H1];
+#pragma unroll
+        for (int j = 0; j < K; ++j) acc[j] = acc[j] + cmulc(tw[j], sv);
+    }
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        if (u0 + j >= nl) break;
+        cx<T> *row = q + (size_t(win) * nl + u0 + j) * L;
+        const cx<T> v = acc[j] * scale;
+        row[k] = v;
+        if (k && L - k != k) row[L - k] = conj(v);
+    }
+}
+
 // real part of the top-left oh x ow block
 template <class T>
 __global__ void k_crop_real(const cx<T> *z, int Lh, int Lw, T *out, int oh, int ow) {
