@@ -324,13 +324,37 @@ void Engine::run_staged_t(const RunArgs &a) {
     cx<T> *c_sec_os = buf<cx<T>>(tn + "c_sec_os", size_t(nb) * OSh * OSw);
     T *r_sec_os = buf<T>(tn + "r_sec_os", size_t(nb) * OSh * OSw);
     T *corr_os = buf<T>(tn + "corr_os", size_t(nb) * sz.os_corr_h * sz.os_corr_w);
-    // the oversampled correlation has few lag rows (17 at raw ovs 2, zoom 16): row-spectrum path
-    const bool row_corr = sz.os_corr_h <= 18 &&
-                          size_t((sz.os_win_h + 1) / 2 + (OSh + 1) / 2) * L2w <= size_t(L2h) * L2w;
-    const int H1os = L2w / 2 + 1;
-    cx<T> *os_tspec = row_corr ? buf<cx<T>>(tn + "os_tspec", size_t(nb) * sz.os_win_h * H1os) : nullptr;
-    cx<T> *os_sspec = row_corr ? buf<cx<T>>(tn + "os_sspec", size_t(nb) * OSh * H1os) : nullptr;
-    cx<T> *os_q = row_corr ? buf<cx<T>>(tn + "os_q", size_t(nb) * sz.os_corr_h * L2w) : nullptr;
+    // Correlations through row spectra (k_pack_rowpairs .. k_mac_lags): row FFTs of the
+    // template and search rows, per-bin sums over template rows for each lag row, and
+    // row IFFTs of the lag rows only; used when the packed rows fit the 2-D buffer
+    auto fits_rows = [&](int th, int sh, int Lh) { return (th + 1) / 2 + (sh + 1) / 2 <= Lh; };
+    const bool row_raw = fits_rows(W_h, S_h, L1h), row_os = fits_rows(sz.os_win_h, OSh, L2h);
+    size_t spec_elems = 0, q_elems = 0;
+    if (row_raw) {
+        spec_elems = std::max(spec_elems, size_t(W_h + S_h) * (L1w / 2 + 1));
+        q_elems = std::max(q_elems, size_t(sz.raw_corr_h) * L1w);
+    }
+    if (row_os) {
+        spec_elems = std::max(spec_elems, size_t(sz.os_win_h + OSh) * (L2w / 2 + 1));
+        q_elems = std::max(q_elems, size_t(sz.os_corr_h) * L2w);
+    }
+    cx<T> *rspec = spec_elems ? buf<cx<T>>(tn + "rspec", size_t(nb) * spec_elems) : nullptr;
+    cx<T> *rq = q_elems ? buf<cx<T>>(tn + "rq", size_t(nb) * q_elems) : nullptr;
+    auto row_correlate = [&](const T *tm, int th, int tw, const T *sr, int sh, int sw, int L, int NL, int NLw,
+                             T *out, int n) {
+        const int H1 = L / 2 + 1, tpairs = (th + 1) / 2, spairs = (sh + 1) / 2;
+        cx<T> *zt = zc, *zs = zc + size_t(n) * tpairs * L;
+        cx<T> *tspec = rspec, *sspec = rspec + size_t(n) * th * H1;
+        hipLaunchKernelGGL(k_pack_rowpairs<T>, grid2(tpairs * L, n), dim3(256), 0, st, tm, th, tw, L, zt);
+        hipLaunchKernelGGL(k_pack_rowpairs<T>, grid2(spairs * L, n), dim3(256), 0, st, sr, sh, sw, L, zs);
+        fft_rows<T, -1>(plan(L), zc, int64_t(n) * (tpairs + spairs), fft_elems<T>(*this), st);
+        hipLaunchKernelGGL(k_split_rowpairs<T>, grid2(tpairs * H1, n), dim3(256), 0, st, zt, th, L, tspec);
+        hipLaunchKernelGGL(k_split_rowpairs<T>, grid2(spairs * H1, n), dim3(256), 0, st, zs, sh, L, sspec);
+        hipLaunchKernelGGL((k_mac_lags<T, 9>), dim3(idivup(H1, 64), idivup(NL, 9), n), dim3(64), 0, st, tspec, th,
+                           sspec, sh, NL, L, rq, T(1.0 / double(L)));
+        fft_rows<T, +1>(plan(L), rq, int64_t(n) * NL, fft_elems<T>(*this), st);
+        hipLaunchKernelGGL(k_crop_real<T>, grid2(NL * NLw, n), dim3(256), 0, st, rq, NL, L, out, NL, NLw);
+    };
     // the 16-sparse surface kernel needs no 256x256 buffers
     const bool fast_surface = !p.surf_method && Z == 16 && So == 256;
     cx<T> *c_surf_in = fast_surface ? nullptr : buf<cx<T>>(tn + "c_surf_in", size_t(nb) * Z * Z);
@@ -362,14 +386,18 @@ void Engine::run_staged_t(const RunArgs &a) {
             hipLaunchKernelGGL(k_abs<T>, dim3(idivup(n * S_h * S_w, 256)), dim3(256), 0, st, c_sec, r_sec,
                                int64_t(n) * S_h * S_w);
             // raw correlation (cuAmpcorChunk.cpp:48-52)
-            hipLaunchKernelGGL(k_pack_corr<T>, grid2(L1h * L1w, n), dim3(256), 0, st, r_ref, W_h, W_w, r_sec, S_h, S_w,
-                               zc, L1h, L1w);
-            fft2<T, -1>(*this, zc, n, L1h, L1w, st);
-            hipLaunchKernelGGL(k_spec_product<T>, grid2(L1h * L1w, n), dim3(256), 0, st, zc, L1h, L1w,
-                               T(1.0 / (double(L1h) * L1w)));
-            fft2_rows_kept<T, +1>(*this, zc, n, L1h, L1w, sz.raw_corr_h, st);
-            hipLaunchKernelGGL(k_crop_real<T>, grid2(sz.raw_corr_h * sz.raw_corr_w, n), dim3(256), 0, st, zc, L1h, L1w,
-                               corr, sz.raw_corr_h, sz.raw_corr_w);
+            if (row_raw) {
+                row_correlate(r_ref, W_h, W_w, r_sec, S_h, S_w, L1w, sz.raw_corr_h, sz.raw_corr_w, corr, n);
+            } else {
+                hipLaunchKernelGGL(k_pack_corr<T>, grid2(L1h * L1w, n), dim3(256), 0, st, r_ref, W_h, W_w, r_sec,
+                                   S_h, S_w, zc, L1h, L1w);
+                fft2<T, -1>(*this, zc, n, L1h, L1w, st);
+                hipLaunchKernelGGL(k_spec_product<T>, grid2(L1h * L1w, n), dim3(256), 0, st, zc, L1h, L1w,
+                                   T(1.0 / (double(L1h) * L1w)));
+                fft2_rows_kept<T, +1>(*this, zc, n, L1h, L1w, sz.raw_corr_h, st);
+                hipLaunchKernelGGL(k_crop_real<T>, grid2(sz.raw_corr_h * sz.raw_corr_w, n), dim3(256), 0, st, zc,
+                                   L1h, L1w, corr, sz.raw_corr_h, sz.raw_corr_w);
+            }
             // normalize (cuAmpcorChunk.cpp:60)
             normalize(corr, sz.raw_corr_h, sz.raw_corr_w, r_ref, W_h, W_w, r_sec, S_h, S_w, ssd_raw, n);
             // argmax, covariance, SNR, zoom start/shift (cuAmpcorChunk.cpp:69-106)
@@ -404,22 +432,9 @@ void Engine::run_staged_t(const RunArgs &a) {
         hipLaunchKernelGGL(k_abs<T>, dim3(idivup(n * OSh * OSw, 256)), dim3(256), 0, st, c_sec_os, r_sec_os,
                            int64_t(n) * OSh * OSw);
         // oversampled correlation + normalize (cuAmpcorChunk.cpp:148-161)
-        if (row_corr) {
-            // few lag rows: row spectra + per-bin sums over template rows + NL row IFFTs
-            const int th = sz.os_win_h, NL = sz.os_corr_h, L = L2w, H1 = L / 2 + 1;
-            const int tpairs = (th + 1) / 2, spairs = (OSh + 1) / 2;
-            cx<T> *zt = zc, *zs = zc + size_t(n) * tpairs * L;
-            hipLaunchKernelGGL(k_pack_rowpairs<T>, grid2(tpairs * L, n), dim3(256), 0, st, r_ref_os, th,
-                               sz.os_win_w, L, zt);
-            hipLaunchKernelGGL(k_pack_rowpairs<T>, grid2(spairs * L, n), dim3(256), 0, st, r_sec_os, OSh, OSw, L, zs);
-            fft_rows<T, -1>(plan(L), zc, int64_t(n) * (tpairs + spairs), fft_elems<T>(*this), st);
-            hipLaunchKernelGGL(k_split_rowpairs<T>, grid2(tpairs * H1, n), dim3(256), 0, st, zt, th, L, os_tspec);
-            hipLaunchKernelGGL(k_split_rowpairs<T>, grid2(spairs * H1, n), dim3(256), 0, st, zs, OSh, L, os_sspec);
-            hipLaunchKernelGGL((k_mac_lags<T, 9>), dim3(idivup(H1, 64), idivup(NL, 9), n), dim3(64), 0, st, os_tspec,
-                               th, os_sspec, OSh, NL, L, os_q, T(1.0 / double(L)));
-            fft_rows<T, +1>(plan(L), os_q, int64_t(n) * NL, fft_elems<T>(*this), st);
-            hipLaunchKernelGGL(k_crop_real<T>, grid2(NL * sz.os_corr_w, n), dim3(256), 0, st, os_q, NL, L, corr_os,
-                               NL, sz.os_corr_w);
+        if (row_os) {
+            row_correlate(r_ref_os, sz.os_win_h, sz.os_win_w, r_sec_os, OSh, OSw, L2w, sz.os_corr_h, sz.os_corr_w,
+                          corr_os, n);
         } else {
             hipLaunchKernelGGL(k_pack_corr<T>, grid2(L2h * L2w, n), dim3(256), 0, st, r_ref_os, sz.os_win_h,
                                sz.os_win_w, r_sec_os, OSh, OSw, zc, L2h, L2w);
