@@ -214,13 +214,20 @@ void fft_rows(const FftPlan &pl, cx<T> *data, int64_t nrows, int elems, hipStrea
     }
 }
 
+// columns per block of a column FFT of length pl.n over w columns: from the
+// element budget, or a fixed width (whole 64/128-byte column segments per row)
+// when the LDS for it is available
+template <class T>
+int cols_per_block(const FftPlan &pl, int w, int elems, int width) {
+    int c = std::max(1, std::min(w, std::min(16, elems / pl.n)));
+    if (width > 0 && 2 * size_t(width) * pl.n * sizeof(cx<T>) <= 160 * 1024) c = std::min(w, width);
+    return c;
+}
+
 template <class T, int DIR>
 void fft_cols(const FftPlan &pl, cx<T> *data, int nimg, int w, int elems, hipStream_t st, int width = 0) {
     const FftDesc &d = std::is_same<T, float>::value ? pl.desc32 : pl.desc64;
-    // columns per block: from the element budget, or a fixed width (whole
-    // 64/128-byte column segments per row) when the LDS for it is available
-    int c = std::max(1, std::min(w, std::min(16, elems / pl.n)));
-    if (width > 0 && 2 * size_t(width) * pl.n * sizeof(cx<T>) <= 160 * 1024) c = std::min(w, width);
+    const int c = cols_per_block<T>(pl, w, elems, width);
     const size_t lds = 2 * size_t(c) * pl.n * sizeof(cx<T>);
     const int threads = c >= 16 ? 512 : 256;
     hipLaunchKernelGGL((k_fft_cols<T, DIR>), dim3(idivup(w, c), nimg), dim3(threads), lds, st, data, w, c, d);
@@ -243,13 +250,22 @@ int fft_elems(const Engine &e) { return sizeof(T) == 8 ? std::max(1024, e.fft_bl
 template <class T>
 int cols_width(const Engine &e) { return sizeof(T) == 4 ? e.fft_cols_width : 0; }
 
-// inverse 2D FFT of a quadrant-padded spectrum (k_quadpad of an h-row chip into
-// H rows): the row pass skips the all-zero rows
-template <class T, int DIR>
-void fft2_padded(Engine &e, cx<T> *data, int nimg, int H, int W, int h, hipStream_t st) {
-    const int h2 = h / 2;
-    fft_rows_sel<T, DIR>(e.plan(W), data, nimg, H, 2 * h2, h2, H - h2, fft_elems<T>(e), st);
-    fft_cols<T, DIR>(e.plan(H), data, nimg, W, fft_elems<T>(e), st, cols_width<T>(e));
+// Inverse 2D FFT of the quadrant padding of h x w spectra `in` into H x W,
+// keeping the real image `out` only (OUT 1 |z|, OUT 2 Re z); `work` holds the
+// H x W complex row-pass result, of which the all-zero rows are neither written
+// nor read
+template <class T, int DIR, int OUT>
+void fft2_quad(Engine &e, const cx<T> *in, int nimg, int h, int w, cx<T> *work, int H, int W, T *out,
+               hipStream_t st) {
+    const FftPlan &pr = e.plan(W), &pc = e.plan(H);
+    const bool f32 = std::is_same<T, float>::value;
+    const int elems = fft_elems<T>(e), sel = 2 * (h / 2);
+    const int g = std::max(1, elems / W);
+    hipLaunchKernelGGL((k_fft_rows_quad<T, DIR>), dim3(idivup(nimg * sel, g)), dim3(256),
+                       2 * size_t(g) * W * sizeof(cx<T>), st, in, nimg, h, w, work, H, g, f32 ? pr.desc32 : pr.desc64);
+    const int c = cols_per_block<T>(pc, W, elems, cols_width<T>(e));
+    hipLaunchKernelGGL((k_fft_cols_quad<T, DIR, OUT>), dim3(idivup(W, c), nimg), dim3(c >= 16 ? 512 : 256),
+                       2 * size_t(c) * H * sizeof(cx<T>), st, work, W, c, h / 2, f32 ? pc.desc32 : pc.desc64, out);
 }
 
 // 2D FFT of which only rows [0, keep) of the result are used: columns first,
@@ -370,21 +386,17 @@ void Engine::run_staged_t(const RunArgs &a) {
         const int n = std::min(nb, a.n - w0);
         // chip load + amplitude + mean (cuAmpcorChunk.cpp:18-39)
         hipLaunchKernelGGL(k_gather<T>, grid2(W_h * W_w, n), dim3(256), 0, st, a.ref, a.ref_pitch,
-                           a.ref_d + w0, a.ref_a + w0, W_h, W_w, mode, c_ref);
+                           a.ref_d + w0, a.ref_a + w0, W_h, W_w, mode, c_ref, fine_only ? nullptr : r_ref);
         hipLaunchKernelGGL(k_gather<T>, grid2(S_h * S_w, n), dim3(256), 0, st, a.sec, a.sec_pitch,
-                           a.sec_d + w0, a.sec_a + w0, S_h, S_w, mode, c_sec);
+                           a.sec_d + w0, a.sec_a + w0, S_h, S_w, mode, c_sec, fine_only ? nullptr : r_sec);
         if (fine_only) {
             HIP_OK(hipMemcpyAsync(start, a.fine_start + 2 * size_t(w0), sizeof(int32_t) * 2 * n,
                                   hipMemcpyDeviceToDevice, st));
             HIP_OK(hipMemcpyAsync(shift, a.fine_shift + 2 * size_t(w0), sizeof(int32_t) * 2 * n,
                                   hipMemcpyDeviceToDevice, st));
         } else {
-            hipLaunchKernelGGL(k_abs<T>, dim3(idivup(n * W_h * W_w, 256)), dim3(256), 0, st, c_ref, r_ref,
-                               int64_t(n) * W_h * W_w);
             hipLaunchKernelGGL(k_sub_mean<T>, dim3(n), dim3(per_window_threads(W_h * W_w)), 0, st, r_ref, W_h * W_w,
                                ssd_raw);
-            hipLaunchKernelGGL(k_abs<T>, dim3(idivup(n * S_h * S_w, 256)), dim3(256), 0, st, c_sec, r_sec,
-                               int64_t(n) * S_h * S_w);
             // raw correlation (cuAmpcorChunk.cpp:48-52)
             if (row_raw) {
                 row_correlate(r_ref, W_h, W_w, r_sec, S_h, S_w, L1w, sz.raw_corr_h, sz.raw_corr_w, corr, n);
@@ -412,11 +424,7 @@ void Engine::run_staged_t(const RunArgs &a) {
             hipLaunchKernelGGL(k_deramp<T>, dim3(n), dim3(per_window_threads(W_h * W_w)), size_t(W_h + W_w) * 16, st,
                                c_ref, W_h, W_w);
         fft2<T, +1>(*this, c_ref, n, W_h, W_w, st);
-        hipLaunchKernelGGL(k_quadpad<T>, grid2(sz.os_win_h * sz.os_win_w, n), dim3(256), 0, st, c_ref, W_h, W_w,
-                           c_ref_os, sz.os_win_h, sz.os_win_w);
-        fft2_padded<T, -1>(*this, c_ref_os, n, sz.os_win_h, sz.os_win_w, W_h, st);
-        hipLaunchKernelGGL(k_abs<T>, dim3(idivup(n * sz.os_win_h * sz.os_win_w, 256)), dim3(256), 0, st, c_ref_os,
-                           r_ref_os, int64_t(n) * sz.os_win_h * sz.os_win_w);
+        fft2_quad<T, -1, 1>(*this, c_ref, n, W_h, W_w, c_ref_os, sz.os_win_h, sz.os_win_w, r_ref_os, st);
         hipLaunchKernelGGL(k_sub_mean<T>, dim3(n), dim3(per_window_threads(sz.os_win_h * sz.os_win_w)), 0, st,
                            r_ref_os, sz.os_win_h * sz.os_win_w, ssd_os);
         // secondary zoom-in crop and oversampling (cuAmpcorChunk.cpp:134-137)
@@ -426,11 +434,7 @@ void Engine::run_staged_t(const RunArgs &a) {
             hipLaunchKernelGGL(k_deramp<T>, dim3(n), dim3(per_window_threads(sz.zoom_in_h * sz.zoom_in_w)),
                                size_t(sz.zoom_in_h + sz.zoom_in_w) * 16, st, c_zoom, sz.zoom_in_h, sz.zoom_in_w);
         fft2<T, +1>(*this, c_zoom, n, sz.zoom_in_h, sz.zoom_in_w, st);
-        hipLaunchKernelGGL(k_quadpad<T>, grid2(OSh * OSw, n), dim3(256), 0, st, c_zoom, sz.zoom_in_h, sz.zoom_in_w,
-                           c_sec_os, OSh, OSw);
-        fft2_padded<T, -1>(*this, c_sec_os, n, OSh, OSw, sz.zoom_in_h, st);
-        hipLaunchKernelGGL(k_abs<T>, dim3(idivup(n * OSh * OSw, 256)), dim3(256), 0, st, c_sec_os, r_sec_os,
-                           int64_t(n) * OSh * OSw);
+        fft2_quad<T, -1, 1>(*this, c_zoom, n, sz.zoom_in_h, sz.zoom_in_w, c_sec_os, OSh, OSw, r_sec_os, st);
         // oversampled correlation + normalize (cuAmpcorChunk.cpp:148-161)
         if (row_os) {
             row_correlate(r_ref_os, sz.os_win_h, sz.os_win_w, r_sec_os, OSh, OSw, L2w, sz.os_corr_h, sz.os_corr_w,
@@ -466,10 +470,7 @@ void Engine::run_staged_t(const RunArgs &a) {
                 hipLaunchKernelGGL(k_trim_complex<T>, grid2(Z * Z, n), dim3(256), 0, st, corr_os, sz.os_corr_h,
                                    sz.os_corr_w, c_surf_in, Z);
                 fft2<T, +1>(*this, c_surf_in, n, Z, Z, st);
-                hipLaunchKernelGGL(k_quadpad<T>, grid2(So * So, n), dim3(256), 0, st, c_surf_in, Z, Z, c_surf, So, So);
-                fft2_padded<T, -1>(*this, c_surf, n, So, So, Z, st);
-                hipLaunchKernelGGL(k_real<T>, dim3(idivup(n * So * So, 256)), dim3(256), 0, st, c_surf, r_surf,
-                                   int64_t(n) * So * So);
+                fft2_quad<T, -1, 2>(*this, c_surf_in, n, Z, Z, c_surf, So, So, r_surf, st);
             }
             hipLaunchKernelGGL(k_maxloc<T>, dim3(n), dim3(256), 0, st, r_surf, So, So, zidx, (T *)nullptr, mf);
             hipLaunchKernelGGL(k_subpixel, dim3(idivup(n, 64)), dim3(64), 0, st, start, zidx, n, ratio, p.hs_down,

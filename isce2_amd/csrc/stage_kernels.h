@@ -79,12 +79,76 @@ __global__ void __launch_bounds__(512) k_fft_cols(cx<T> *data, int w, int c, Fft
     }
 }
 
+// Inverse 2-D FFT of the quadrant padding of h x w spectra into H x W
+// (cuArraysPadding.cu:223-266 followed by the oversampler's second transform,
+// cuOverSampler.cpp:72-78 and :141-148), in two passes that never store the
+// padding. Row pass: each of the 2*(h/2) non-zero padded rows is built in LDS
+// from the small spectrum (scaled by 1/(h*w), zeros between the quadrants) and
+// transformed; it lands at its padded row of `out` ([nimg][H][d.n]), whose
+// all-zero rows are left unwritten. `g` rows per block.
+template <class T, int DIR>
+__global__ void __launch_bounds__(256) k_fft_rows_quad(const cx<T> *in, int nimg, int h, int w, cx<T> *out, int H,
+                                                       int g, FftDesc d) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *a = reinterpret_cast<cx<T> *>(smem_raw);
+    cx<T> *b = a + g * d.n;
+    const int W = d.n, h2 = h / 2, w2 = w / 2, sel = 2 * h2;
+    const T f = T(1) / T(h * w);
+    const int t0 = blockIdx.x * g;
+    const int nr = min(g, nimg * sel - t0);
+    for (int i = threadIdx.x; i < nr * W; i += blockDim.x) {
+        const int k = i / W, q = i - k * W;
+        const int t = t0 + k, img = t / sel, s = t - img * sel;
+        const int sr = s < h2 ? s : h - sel + s;   // rows [0, h2) and [h - h2, h) of the spectrum
+        const int sc = q < w2 ? q : (q >= W - w2 ? q - W + w : -1);
+        a[i] = sc >= 0 ? in[(size_t(img) * h + sr) * w + sc] * f : cx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+    cx<T> *res = fft_lds<T, DIR>(a, b, d, nr, 1, W);
+    for (int i = threadIdx.x; i < nr * W; i += blockDim.x) {
+        const int k = i / W, q = i - k * W;
+        const int t = t0 + k, img = t / sel, s = t - img * sel;
+        const int p = s < h2 ? s : H - sel + s;    // padded rows [0, h2) and [H - h2, H)
+        out[(size_t(img) * H + p) * W + q] = res[i];
+    }
+}
+
+// Column pass of the same inverse over [nimg][d.n][w] images whose rows
+// [h2, d.n - h2) are zero and not read. Only a real image is kept: |z| (OUT 1,
+// the oversampled chips' amplitude, cuArraysCopy.cu:510-529) or Re z (OUT 2,
+// the oversampled surface, cuArraysCopy.cu:326-339), written to `out`
+// ([nimg][d.n][w]). `c` columns per block.
+template <class T, int DIR, int OUT>
+__global__ void __launch_bounds__(512) k_fft_cols_quad(const cx<T> *data, int w, int c, int h2, FftDesc d, T *out) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *a = reinterpret_cast<cx<T> *>(smem_raw);
+    cx<T> *b = a + c * d.n;
+    const int h = d.n;
+    const int col0 = blockIdx.x * c;
+    const int nc = min(c, w - col0);
+    const cx<T> *img = data + size_t(blockIdx.y) * h * w + col0;
+    const int tot = h * nc;
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+        const int r = i / nc, cc = i - r * nc;
+        a[r * c + cc] = (r < h2 || r >= h - h2) ? img[size_t(r) * w + cc] : cx<T>{T(0), T(0)};
+    }
+    __syncthreads();
+    const cx<T> *res = fft_lds<T, DIR>(a, b, d, nc, c, 1);
+    T *o = out + size_t(blockIdx.y) * h * w + col0;
+    for (int i = threadIdx.x; i < tot; i += blockDim.x) {
+        const int r = i / nc, cc = i - r * nc;
+        const cx<T> v = res[r * c + cc];
+        o[size_t(r) * w + cc] = OUT == 1 ? amag(v) : v.x;
+    }
+}
+
 // ---------------------------------------------------------------------------
-// chip gather (cuArraysCopy.cu:70-165): mode 0 complex copy, 1 (|z|,0), 2 real -> (x,0)
+// chip gather (cuArraysCopy.cu:70-165): mode 0 complex copy, 1 (|z|,0), 2 real -> (x,0);
+// with `amp`, also the amplitude of the gathered chip (cuArraysAbs, cuArraysCopy.cu:510-529)
 // ---------------------------------------------------------------------------
 template <class T>
 __global__ void k_gather(const void *img, int64_t pitch, const int32_t *sd, const int32_t *sa,
-                         int h, int w, int mode, cx<T> *out) {
+                         int h, int w, int mode, cx<T> *out, T *amp) {
     const int win = blockIdx.y;
     const int tot = h * w;
     const int64_t r0 = sd[win], c0 = sa[win];
@@ -101,14 +165,8 @@ __global__ void k_gather(const void *img, int64_t pitch, const int32_t *sd, cons
             if (mode == 1) v = {T(sqrtf(z.x * z.x + z.y * z.y)), T(0)};
         }
         o[i] = v;
+        if (amp) amp[size_t(win) * tot + i] = amag(v);
     }
-}
-
-// |z| (cuArraysCopy.cu:510-529)
-template <class T>
-__global__ void k_abs(const cx<T> *in, T *out, int64_t total) {
-    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < total; i += int64_t(gridDim.x) * blockDim.x)
-        out[i] = amag(in[i]);
 }
 
 // block-wide FP64 sum (256 threads)
@@ -589,23 +647,6 @@ __global__ void __launch_bounds__(1024) k_deramp(cx<T> *chips, int h, int w) {
     }
 }
 
-// quadrant padding h x w -> H x W with scale 1/(h*w) (cuArraysPadding.cu:223-266)
-template <class T>
-__global__ void k_quadpad(const cx<T> *in, int h, int w, cx<T> *out, int H, int W) {
-    const int win = blockIdx.y;
-    const int h2 = h / 2, w2 = w / 2;
-    const T f = T(1) / T(h * w);
-    const cx<T> *a = in + size_t(win) * h * w;
-    cx<T> *o = out + size_t(win) * H * W;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < H * W; i += gridDim.x * blockDim.x) {
-        const int p = i / W, q = i - p * W;
-        int sr = -1, sc = -1;
-        if (p < h2) sr = p; else if (p >= H - h2) sr = p - H + h;
-        if (q < w2) sc = q; else if (q >= W - w2) sc = q - W + w;
-        o[i] = (sr >= 0 && sc >= 0) ? a[sr * w + sc] * f : cx<T>{T(0), T(0)};
-    }
-}
-
 // per-window crop at a varying start (cuArraysCopy.cu:209-224)
 template <class T>
 __global__ void k_crop_start(const cx<T> *in, int ih, int iw, const int32_t *start, cx<T> *out, int oh, int ow) {
@@ -634,13 +675,6 @@ __global__ void k_trim_real(const T *in, int ih, int iw, T *out, int z) {
         const int r = i / z, c = i - r * z;
         out[size_t(win) * z * z + i] = in[size_t(win) * ih * iw + r * iw + c];
     }
-}
-
-// real part of a complex array
-template <class T>
-__global__ void k_real(const cx<T> *in, T *out, int64_t total) {
-    for (int64_t i = blockIdx.x * int64_t(blockDim.x) + threadIdx.x; i < total; i += int64_t(gridDim.x) * blockDim.x)
-        out[i] = in[i].x;
 }
 
 // windowed-sinc surface oversampling (cuSincOverSampler.cu:84-195): zero output,
