@@ -305,8 +305,6 @@ void Engine::run_staged_t(const RunArgs &a) {
     T *r_ref = buf<T>(tn + "r_ref", size_t(nb) * W_h * W_w);
     cx<T> *c_sec = buf<cx<T>>(tn + "c_sec", size_t(nb) * S_h * S_w);
     T *r_sec = buf<T>(tn + "r_sec", size_t(nb) * S_h * S_w);
-    const size_t zmax = std::max(size_t(L1h) * L1w, size_t(L2h) * L2w);
-    cx<T> *zc = buf<cx<T>>(tn + "zc", size_t(nb) * zmax);
     auto norm_lds = [](int sw) { return size_t(4 * sw + 2) * sizeof(double); };
     // block size of the one-block-per-window reductions
     auto per_window_threads = [](int size) { return size >= 16384 ? 1024 : 256; };
@@ -340,11 +338,16 @@ void Engine::run_staged_t(const RunArgs &a) {
     cx<T> *c_sec_os = buf<cx<T>>(tn + "c_sec_os", size_t(nb) * OSh * OSw);
     T *r_sec_os = buf<T>(tn + "r_sec_os", size_t(nb) * OSh * OSw);
     T *corr_os = buf<T>(tn + "corr_os", size_t(nb) * sz.os_corr_h * sz.os_corr_w);
-    // Correlations through row spectra (k_pack_rowpairs .. k_mac_lags): row FFTs of the
-    // template and search rows, per-bin sums over template rows for each lag row, and
-    // row IFFTs of the lag rows only; used when the packed rows fit the 2-D buffer
+    // Correlations through row spectra (k_rowspec, k_mac_lags, k_lagrows_real): row
+    // FFTs of the template and search rows, per-bin sums over template rows for each
+    // lag row, and row IFFTs of the lag rows only. The padded 2-D FFT route remains
+    // for shapes whose row pairs outnumber the transform rows.
     auto fits_rows = [&](int th, int sh, int Lh) { return (th + 1) / 2 + (sh + 1) / 2 <= Lh; };
     const bool row_raw = fits_rows(W_h, S_h, L1h), row_os = fits_rows(sz.os_win_h, OSh, L2h);
+    size_t zmax = 0;
+    if (!row_raw) zmax = size_t(L1h) * L1w;
+    if (!row_os) zmax = std::max(zmax, size_t(L2h) * L2w);
+    cx<T> *zc = zmax ? buf<cx<T>>(tn + "zc", size_t(nb) * zmax) : nullptr;
     size_t spec_elems = 0, q_elems = 0;
     if (row_raw) {
         spec_elems = std::max(spec_elems, size_t(W_h + S_h) * (L1w / 2 + 1));
@@ -358,18 +361,19 @@ void Engine::run_staged_t(const RunArgs &a) {
     cx<T> *rq = q_elems ? buf<cx<T>>(tn + "rq", size_t(nb) * q_elems) : nullptr;
     auto row_correlate = [&](const T *tm, int th, int tw, const T *sr, int sh, int sw, int L, int NL, int NLw,
                              T *out, int n) {
-        const int H1 = L / 2 + 1, tpairs = (th + 1) / 2, spairs = (sh + 1) / 2;
-        cx<T> *zt = zc, *zs = zc + size_t(n) * tpairs * L;
+        const int H1 = L / 2 + 1;
+        const FftPlan &pl = plan(L);
+        const FftDesc &d = sizeof(T) == 4 ? pl.desc32 : pl.desc64;
+        const int g = std::max(1, fft_elems<T>(*this) / L);
+        const size_t lds = 2 * size_t(g) * L * sizeof(cx<T>);
         cx<T> *tspec = rspec, *sspec = rspec + size_t(n) * th * H1;
-        hipLaunchKernelGGL(k_pack_rowpairs<T>, grid2(tpairs * L, n), dim3(256), 0, st, tm, th, tw, L, zt);
-        hipLaunchKernelGGL(k_pack_rowpairs<T>, grid2(spairs * L, n), dim3(256), 0, st, sr, sh, sw, L, zs);
-        fft_rows<T, -1>(plan(L), zc, int64_t(n) * (tpairs + spairs), fft_elems<T>(*this), st);
-        hipLaunchKernelGGL(k_split_rowpairs<T>, grid2(tpairs * H1, n), dim3(256), 0, st, zt, th, L, tspec);
-        hipLaunchKernelGGL(k_split_rowpairs<T>, grid2(spairs * H1, n), dim3(256), 0, st, zs, sh, L, sspec);
+        hipLaunchKernelGGL(k_rowspec<T>, dim3(idivup(n * ((th + 1) / 2), g)), dim3(256), lds, st, tm, n, th, tw, g, d,
+                           tspec);
+        hipLaunchKernelGGL(k_rowspec<T>, dim3(idivup(n * ((sh + 1) / 2), g)), dim3(256), lds, st, sr, n, sh, sw, g, d,
+                           sspec);
         hipLaunchKernelGGL((k_mac_lags<T, 9>), dim3(idivup(H1, 64), idivup(NL, 9), n), dim3(64), 0, st, tspec, th,
                            sspec, sh, NL, L, rq, T(1.0 / double(L)));
-        fft_rows<T, +1>(plan(L), rq, int64_t(n) * NL, fft_elems<T>(*this), st);
-        hipLaunchKernelGGL(k_crop_real<T>, grid2(NL * NLw, n), dim3(256), 0, st, rq, NL, L, out, NL, NLw);
+        hipLaunchKernelGGL(k_lagrows_real<T>, dim3(idivup(n * NL, g)), dim3(256), lds, st, rq, n * NL, g, d, NLw, out);
     };
     // the 16-sparse surface kernel needs no 256x256 buffers
     const bool fast_surface = !p.surf_method && Z == 16 && So == 256;

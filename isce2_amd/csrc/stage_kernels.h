@@ -248,42 +248,64 @@ __global__ void k_spec_product(cx<T> *z, int Lh, int Lw, T scale) {
 }
 
 // ---------------------------------------------------------------------------
-// Row-spectrum correlation for a few lag rows (the oversampled correlation):
+// Row-spectrum correlation over NL lag rows:
 //   c[u][v] = (1/L) IFFT_L( sum_t conj(T_t) S_{t+u} )[v],   0 <= u < NL,
 // equal to the zero-padded 2-D correlation (cuCorrFrequency.cu:63-79) at every
-// valid lag for any row length L >= the search width. Rows are packed in pairs
-// into complex rows (k_pack_rowpairs), transformed along rows, split into
-// per-row half spectra (k_split_rowpairs), summed over template rows per bin
-// and lag (k_mac_lags), and only the NL lag rows go back through a row IFFT.
+// valid lag for any row length L >= the search width. Row half spectra come
+// from k_rowspec, are summed over template rows per bin and lag (k_mac_lags),
+// and only the NL lag rows go back through a row IFFT (k_lagrows_real).
 // ---------------------------------------------------------------------------
 
-// rows 2j and 2j+1 of each h x w image -> complex row j of length L (zero beyond w, h)
+// Half spectra of the rows of [nimg][h][w] real images ([nimg][h][L/2+1]):
+// rows 2j and 2j+1 are loaded as one complex row z = r_2j + i r_2j+1 of length
+// L = d.n (zero beyond w and h), transformed in LDS, and split with
+// A = (Z[k] + conj Z[-k]) / 2, B = (Z[k] - conj Z[-k]) / (2i). `g` row pairs per block.
 template <class T>
-__global__ void k_pack_rowpairs(const T *img, int h, int w, int L, cx<T> *out) {
-    const int win = blockIdx.y, np = (h + 1) / 2;
-    const T *a = img + size_t(win) * h * w;
-    cx<T> *o = out + size_t(win) * np * L;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < np * L; i += gridDim.x * blockDim.x) {
-        const int j = i / L, x = i - j * L, r = 2 * j;
-        o[i] = {x < w ? a[r * w + x] : T(0), (x < w && r + 1 < h) ? a[(r + 1) * w + x] : T(0)};
+__global__ void __launch_bounds__(256) k_rowspec(const T *img, int nimg, int h, int w, int g, FftDesc d,
+                                                 cx<T> *spec) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *a = reinterpret_cast<cx<T> *>(smem_raw);
+    cx<T> *b = a + g * d.n;
+    const int L = d.n, np = (h + 1) / 2, H1 = L / 2 + 1;
+    const int t0 = blockIdx.x * g;
+    const int nr = min(g, nimg * np - t0);
+    for (int i = threadIdx.x; i < nr * L; i += blockDim.x) {
+        const int k = i / L, x = i - k * L;
+        const int t = t0 + k, win = t / np, r = 2 * (t - win * np);
+        const T *im = img + size_t(win) * h * w;
+        a[i] = {x < w ? im[r * w + x] : T(0), (x < w && r + 1 < h) ? im[(r + 1) * w + x] : T(0)};
+    }
+    __syncthreads();
+    const cx<T> *Z = fft_lds<T, -1>(a, b, d, nr, 1, L);
+    for (int i = threadIdx.x; i < nr * H1; i += blockDim.x) {
+        const int k = i / H1, kk = i - k * H1;
+        const int t = t0 + k, win = t / np, j = t - win * np;
+        const cx<T> za = Z[k * L + kk], bc = conj(Z[k * L + (kk ? L - kk : 0)]);
+        cx<T> *sp = spec + size_t(win) * h * H1;
+        sp[size_t(2 * j) * H1 + kk] = (za + bc) * T(0.5);
+        if (2 * j + 1 < h) {
+            const cx<T> dd = (za - bc) * T(0.5);
+            sp[size_t(2 * j + 1) * H1 + kk] = {dd.y, -dd.x};
+        }
     }
 }
 
-// Z_j = FFT(row 2j + i row 2j+1) -> spectra of rows 2j and 2j+1 at bins 0..L/2:
-// A = (Z[k] + conj Z[-k]) / 2, B = (Z[k] - conj Z[-k]) / (2i)
+// the inverse row FFT of the lag rows and k_crop_real in one pass: rows t of
+// length d.n (t = window * nl + lag) -> out[t][v] = Re, v < nlw; `g` rows per block
 template <class T>
-__global__ void k_split_rowpairs(const cx<T> *z, int h, int L, cx<T> *spec) {
-    const int win = blockIdx.y, np = (h + 1) / 2, H1 = L / 2 + 1;
-    const cx<T> *Z = z + size_t(win) * np * L;
-    cx<T> *sp = spec + size_t(win) * h * H1;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < np * H1; i += gridDim.x * blockDim.x) {
-        const int j = i / H1, k = i - j * H1;
-        const cx<T> a = Z[size_t(j) * L + k], bc = conj(Z[size_t(j) * L + (k ? L - k : 0)]);
-        sp[size_t(2 * j) * H1 + k] = (a + bc) * T(0.5);
-        if (2 * j + 1 < h) {
-            const cx<T> d = (a - bc) * T(0.5);
-            sp[size_t(2 * j + 1) * H1 + k] = {d.y, -d.x};
-        }
+__global__ void __launch_bounds__(256) k_lagrows_real(const cx<T> *q, int nrows, int g, FftDesc d, int nlw, T *out) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *a = reinterpret_cast<cx<T> *>(smem_raw);
+    cx<T> *b = a + g * d.n;
+    const int r0 = blockIdx.x * g;
+    const int nr = min(g, nrows - r0);
+    const cx<T> *base = q + size_t(r0) * d.n;
+    for (int i = threadIdx.x; i < nr * d.n; i += blockDim.x) a[i] = base[i];
+    __syncthreads();
+    const cx<T> *res = fft_lds<T, 1>(a, b, d, nr, 1, d.n);
+    for (int i = threadIdx.x; i < nr * nlw; i += blockDim.x) {
+        const int k = i / nlw, v = i - k * nlw;
+        out[size_t(r0 + k) * nlw + v] = res[k * d.n + v].x;
     }
 }
 
