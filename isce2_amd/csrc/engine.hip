@@ -284,6 +284,18 @@ void fft2(Engine &e, cx<T> *data, int nimg, int h, int w, hipStream_t st) {
     fft_cols<T, DIR>(e.plan(h), data, nimg, w, fft_elems<T>(e), st, cols_width<T>(e));
 }
 
+// forward 2D FFT of h x w chips read from [nimg][ih][iw] sources, cropped at
+// `start` and deramped with `tab` when given (k_fft_rows_chip), into dst
+template <class T, int DIR>
+void fft2_chip(Engine &e, const cx<T> *src, int ih, int iw, const int32_t *start, const cx<double> *tab, cx<T> *dst,
+               int nimg, int h, int w, hipStream_t st) {
+    const FftPlan &pl = e.plan(w);
+    const int g = std::max(1, fft_elems<T>(e) / w);
+    hipLaunchKernelGGL((k_fft_rows_chip<T, DIR>), dim3(idivup(nimg * h, g)), dim3(256), 2 * size_t(g) * w * sizeof(cx<T>),
+                       st, src, ih, iw, start, tab, dst, nimg, h, g, sizeof(T) == 4 ? pl.desc32 : pl.desc64);
+    fft_cols<T, DIR>(e.plan(h), dst, nimg, w, fft_elems<T>(e), st, cols_width<T>(e));
+}
+
 inline dim3 grid2(int per_img, int nimg) { return dim3(std::min(idivup(per_img, 256), 64), nimg); }
 }  // namespace
 
@@ -334,6 +346,9 @@ void Engine::run_staged_t(const RunArgs &a) {
     cx<T> *c_ref_os = buf<cx<T>>(tn + "c_ref_os", size_t(nb) * sz.os_win_h * sz.os_win_w);
     T *r_ref_os = buf<T>(tn + "r_ref_os", size_t(nb) * sz.os_win_h * sz.os_win_w);
     cx<T> *c_zoom = buf<cx<T>>(tn + "c_zoom", size_t(nb) * sz.zoom_in_h * sz.zoom_in_w);
+    // deramp rotation tables (row then column factors per chip)
+    cx<double> *dtab = p.deramp == 1 ? buf<cx<double>>(tn + "dtab", size_t(nb) * (std::max(W_h + W_w,
+                                                                 sz.zoom_in_h + sz.zoom_in_w))) : nullptr;
     const int OSh = sz.os_search_h, OSw = sz.os_search_w;   // oversampled zoom chip (fixed by the interpolation)
     cx<T> *c_sec_os = buf<cx<T>>(tn + "c_sec_os", size_t(nb) * OSh * OSw);
     T *r_sec_os = buf<T>(tn + "r_sec_os", size_t(nb) * OSh * OSw);
@@ -425,19 +440,18 @@ void Engine::run_staged_t(const RunArgs &a) {
         }
         // reference oversampling (cuAmpcorChunk.cpp:115-126)
         if (p.deramp == 1)
-            hipLaunchKernelGGL(k_deramp<T>, dim3(n), dim3(per_window_threads(W_h * W_w)), size_t(W_h + W_w) * 16, st,
-                               c_ref, W_h, W_w);
-        fft2<T, +1>(*this, c_ref, n, W_h, W_w, st);
+            hipLaunchKernelGGL(k_deramp_phase<T>, dim3(n), dim3(per_window_threads(W_h * W_w)), 0, st, c_ref, W_h, W_w,
+                               (const int32_t *)nullptr, W_h, W_w, dtab);
+        fft2_chip<T, +1>(*this, c_ref, W_h, W_w, nullptr, p.deramp == 1 ? dtab : nullptr, c_ref, n, W_h, W_w, st);
         fft2_quad<T, -1, 1>(*this, c_ref, n, W_h, W_w, c_ref_os, sz.os_win_h, sz.os_win_w, r_ref_os, st);
         hipLaunchKernelGGL(k_sub_mean<T>, dim3(n), dim3(per_window_threads(sz.os_win_h * sz.os_win_w)), 0, st,
                            r_ref_os, sz.os_win_h * sz.os_win_w, ssd_os);
         // secondary zoom-in crop and oversampling (cuAmpcorChunk.cpp:134-137)
-        hipLaunchKernelGGL(k_crop_start<T>, grid2(sz.zoom_in_h * sz.zoom_in_w, n), dim3(256), 0, st, c_sec, S_h,
-                           S_w, start, c_zoom, sz.zoom_in_h, sz.zoom_in_w);
         if (p.deramp == 1)
-            hipLaunchKernelGGL(k_deramp<T>, dim3(n), dim3(per_window_threads(sz.zoom_in_h * sz.zoom_in_w)),
-                               size_t(sz.zoom_in_h + sz.zoom_in_w) * 16, st, c_zoom, sz.zoom_in_h, sz.zoom_in_w);
-        fft2<T, +1>(*this, c_zoom, n, sz.zoom_in_h, sz.zoom_in_w, st);
+            hipLaunchKernelGGL(k_deramp_phase<T>, dim3(n), dim3(per_window_threads(sz.zoom_in_h * sz.zoom_in_w)), 0,
+                               st, c_sec, S_h, S_w, start, sz.zoom_in_h, sz.zoom_in_w, dtab);
+        fft2_chip<T, +1>(*this, c_sec, S_h, S_w, start, p.deramp == 1 ? dtab : nullptr, c_zoom, n, sz.zoom_in_h,
+                         sz.zoom_in_w, st);
         fft2_quad<T, -1, 1>(*this, c_zoom, n, sz.zoom_in_h, sz.zoom_in_w, c_sec_os, OSh, OSw, r_sec_os, st);
         // oversampled correlation + normalize (cuAmpcorChunk.cpp:148-161)
         if (row_os) {

@@ -625,26 +625,37 @@ __global__ void __launch_bounds__(64) k_stats(const T *corr, int nx, int ny, int
 
 // ---------------------------------------------------------------------------
 // deramp method 1 (cuDeramp.cu:343-390) with float2.h's operator*= (imaginary
-// part reuses the updated real part); sums, phases and rotation in FP64
+// part reuses the updated real part); sums, phases and rotation in FP64. The
+// phase is separable, c*phx + r*phy: k_deramp_phase writes one FP64 table per
+// chip, e^{i r phy} for its h rows then e^{i c phx} for its w columns, and the
+// forward row FFT applies the rotation as it loads the chip (deramp_apply).
 // ---------------------------------------------------------------------------
+// chip win of [nimg][ih][iw] images, cropped at `start` (row, column) when given
 template <class T>
-__global__ void __launch_bounds__(1024) k_deramp(cx<T> *chips, int h, int w) {
+__device__ __forceinline__ const cx<T> *crop_origin(const cx<T> *src, int ih, int iw, const int32_t *start, int win) {
+    const cx<T> *o = src + size_t(win) * ih * iw;
+    return start ? o + start[2 * win] * iw + start[2 * win + 1] : o;
+}
+
+template <class T>
+__global__ void __launch_bounds__(1024) k_deramp_phase(const cx<T> *src, int ih, int iw, const int32_t *start, int h,
+                                                       int w, cx<double> *tab) {
     __shared__ double sh[16];
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    cx<double> *rowf = reinterpret_cast<cx<double> *>(smem_raw);   // e^{i r phy}
-    cx<double> *colf = rowf + h;                                   // e^{i c phx}
-    cx<T> *z = chips + size_t(blockIdx.x) * h * w;
+    const cx<T> *z = crop_origin(src, ih, iw, start, blockIdx.x);
     double ayr = 0, ayi = 0, axr = 0, axi = 0;
     for (int i = threadIdx.x; i < h * w; i += blockDim.x) {
         const int r = i / w, c = i - r * w;
-        const double zr = double(z[i].x), zi = double(z[i].y);
+        const cx<T> v = z[r * iw + c];
+        const double zr = double(v.x), zi = double(v.y);
         if (c < w - 1) {   // z * conj(z_right)
-            const double br = double(z[i + 1].x), bi = double(z[i + 1].y);
+            const cx<T> u = z[r * iw + c + 1];
+            const double br = double(u.x), bi = double(u.y);
             ayr += zr * br + zi * bi;
             ayi += zi * br - zr * bi;
         }
         if (r < h - 1) {   // z * conj(z_below)
-            const double br = double(z[i + w].x), bi = double(z[i + w].y);
+            const cx<T> u = z[(r + 1) * iw + c];
+            const double br = double(u.x), bi = double(u.y);
             axr += zr * br + zi * bi;
             axi += zi * br - zr * bi;
         }
@@ -655,32 +666,49 @@ __global__ void __launch_bounds__(1024) k_deramp(cx<T> *chips, int h, int w) {
     axi = block_sum256(axi, sh);
     const double phy = atan2(ayi, ayr);
     const double phx = atan2(axi, axr);
-    // the phase c*phx + r*phy is separable: one FP64 sincos per row and per column
+    cx<double> *rowf = tab + size_t(blockIdx.x) * (h + w), *colf = rowf + h;
     for (int r = threadIdx.x; r < h; r += blockDim.x) sincos(double(r) * phy, &rowf[r].y, &rowf[r].x);
     for (int c = threadIdx.x; c < w; c += blockDim.x) sincos(double(c) * phx, &colf[c].y, &colf[c].x);
-    __syncthreads();
-    for (int i = threadIdx.x; i < h * w; i += blockDim.x) {
-        const int r = i / w, c = i - r * w;
-        const cx<double> f = cmul(colf[c], rowf[r]);
-        const double x = double(z[i].x), y = double(z[i].y);
-        const double nx = x * f.x - y * f.y;
-        const double ny = y * f.x + nx * f.y;
-        z[i] = {T(nx), T(ny)};
-    }
 }
 
-// per-window crop at a varying start (cuArraysCopy.cu:209-224)
 template <class T>
-__global__ void k_crop_start(const cx<T> *in, int ih, int iw, const int32_t *start, cx<T> *out, int oh, int ow) {
-    const int win = blockIdx.y;
-    const int r0 = start[2 * win], c0 = start[2 * win + 1];
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < oh * ow; i += gridDim.x * blockDim.x) {
-        const int r = i / ow, c = i - r * ow;
-        out[size_t(win) * oh * ow + i] = in[size_t(win) * ih * iw + (r0 + r) * iw + c0 + c];
-    }
+__device__ __forceinline__ cx<T> deramp_apply(cx<T> v, cx<double> rf, cx<double> cf) {
+    const cx<double> f = cmul(cf, rf);
+    const double x = double(v.x), y = double(v.y);
+    const double nx = x * f.x - y * f.y;
+    const double ny = y * f.x + nx * f.y;
+    return {T(nx), T(ny)};
 }
 
-// top-left zoom x zoom of the oversampled surface -> complex (cuAmpcorChunk.cpp:169, cuOverSampler.cpp:143)
+// forward row FFT of h x d.n chips read from [nimg][ih][iw] sources (cropped at
+// `start` when given: the secondary zoom-in window, cuArraysCopy.cu:209-224),
+// deramped on load when `tab` is given, written to dst [nimg][h][d.n] (dst may
+// be the source itself when there is no crop); `g` rows per block
+template <class T, int DIR>
+__global__ void __launch_bounds__(256) k_fft_rows_chip(const cx<T> *src, int ih, int iw, const int32_t *start,
+                                                       const cx<double> *tab, cx<T> *dst, int nimg, int h, int g,
+                                                       FftDesc d) {
+    extern __shared__ unsigned char smem_raw[];
+    cx<T> *a = reinterpret_cast<cx<T> *>(smem_raw);
+    cx<T> *b = a + g * d.n;
+    const int w = d.n, r0 = blockIdx.x * g;
+    const int nr = min(g, nimg * h - r0);
+    for (int i = threadIdx.x; i < nr * w; i += blockDim.x) {
+        const int k = i / w, c = i - k * w;
+        const int t = r0 + k, win = t / h, r = t - win * h;
+        cx<T> v = crop_origin(src, ih, iw, start, win)[r * iw + c];
+        if (tab) {
+            const cx<double> *rowf = tab + size_t(win) * (h + w);
+            v = deramp_apply(v, rowf[r], rowf[h + c]);
+        }
+        a[i] = v;
+    }
+    __syncthreads();
+    const cx<T> *res = fft_lds<T, DIR>(a, b, d, nr, 1, w);
+    cx<T> *base = dst + size_t(r0) * w;
+    for (int i = threadIdx.x; i < nr * w; i += blockDim.x) base[i] = res[i];
+}
+
 template <class T>
 __global__ void k_trim_complex(const T *in, int ih, int iw, cx<T> *out, int z) {
     const int win = blockIdx.y;
