@@ -164,8 +164,13 @@ __device__ void stockham_pass(const cx<T> *src, cx<T> *dst, const cx<T> *tw, int
                               int nsig, int es, int ss) {
     const int m = n / R;
     const int total = nsig * m;
+    // strided signals (columns, es > 1) sit side by side in LDS: consecutive
+    // threads take consecutive signals so a wave's accesses are contiguous;
+    // contiguous signals (rows) take consecutive butterflies of one signal
+    const bool across = es != 1;
     for (int t = threadIdx.x; t < total; t += blockDim.x) {
-        const int s = t / m, j = t - s * m;
+        const int s = across ? t % nsig : t / m;
+        const int j = across ? t / nsig : t - s * m;
         const int k = j % ns;
         const cx<T> *in = src + s * ss;
         cx<T> *out = dst + s * ss;
@@ -187,8 +192,10 @@ __device__ void stockham_pass(const cx<T> *src, cx<T> *dst, const cx<T> *tw, int
 template <class T, int DIR>
 __device__ void direct_dft(const cx<T> *src, cx<T> *dst, const cx<T> *tw, int n, int nsig, int es, int ss) {
     const int total = nsig * n;
+    const bool across = es != 1;   // thread order as in stockham_pass
     for (int t = threadIdx.x; t < total; t += blockDim.x) {
-        const int s = t / n, q = t - s * n;
+        const int s = across ? t % nsig : t / n;
+        const int q = across ? t / nsig : t - s * n;
         const cx<T> *in = src + s * ss;
         cx<T> acc = {T(0), T(0)};
         int m = 0;
