@@ -150,7 +150,7 @@ int ampcor_set_fft_radices(ampcor_ctx *ctx, int32_t n, const int32_t *radices, i
 /* Launch-shape options, for autotuning. No counterpart in the reference.
  *   "fine_threads": 512 or 1024 threads per workgroup of the window-64 fused
  *                   fine kernel (default 1024).
- *   "fft_block_elems": 1024, 2048, 4096 (default) or 8192 complex elements per
+ *   "fft_block_elems": 1024, 2048 (default), 4096 or 8192 complex elements per
  *                   workgroup of the staged pipeline's row/column FFT kernels.
  *   "fft_cols_width": 0, 8 (default) or 16 columns per workgroup of the staged
  *                   pipeline's FP32 column FFTs (0: as many as fft_block_elems allows). */

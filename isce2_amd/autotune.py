@@ -147,7 +147,7 @@ class Autotuner:
                 m = min(m, self.radix_sample)
                 sub = list(args[:2]) + [t[:m].contiguous() for t in args[2:]]
                 times = {}
-                for fe in (2048, 4096, 8192):
+                for fe in (1024, 2048, 4096, 8192):
                     eng.set_option("fft_block_elems", fe)
                     times[fe] = self._time(eng, sub, MODE_STAGED, min(best.chunk, m), m)
                 fe = min(times, key=times.get)

@@ -118,7 +118,7 @@ struct Engine {
     // launch-shape options (ampcor_set_option): threads per workgroup of the window-64 fine kernel
     int fine_threads = 1024;
     // complex elements per workgroup of the staged FFT kernels (float; double uses half)
-    int fft_block_elems = 4096;
+    int fft_block_elems = 2048;
     // columns per workgroup of the FP32 staged column FFTs (0: from fft_block_elems)
     int fft_cols_width = 8;
     int precision = 1;

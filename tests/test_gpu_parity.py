@@ -190,8 +190,8 @@ def test_launch_options_do_not_change_results(eng, scene1):
             r = eng.run_windows(rt, stt, *st, mode=mode, debug=True)
             torch.cuda.synchronize()
             res[(name, v)] = {k: t.cpu().numpy() for k, t in r.items()}
-        eng.set_option(name, {"fft_block_elems": 4096, "fine_threads": 1024, "fft_cols_width": 8}[name])
-    base_f, base_s = res[("fine_threads", 1024)], res[("fft_block_elems", 4096)]
+        eng.set_option(name, {"fft_block_elems": 2048, "fine_threads": 1024, "fft_cols_width": 8}[name])
+    base_f, base_s = res[("fine_threads", 1024)], res[("fft_block_elems", 2048)]
     for (name, v), r in res.items():
         base = base_f if name == "fine_threads" else base_s
         for k in ("maxloc", "zoom_idx", "offset"):
