@@ -386,8 +386,14 @@ void Engine::run_staged_t(const RunArgs &a) {
                            tspec);
         hipLaunchKernelGGL(k_rowspec<T>, dim3(idivup(n * ((sh + 1) / 2), g)), dim3(256), lds, st, sr, n, sh, sw, g, d,
                            sspec);
-        hipLaunchKernelGGL((k_mac_lags<T, 9>), dim3(idivup(H1, 64), idivup(NL, 9), n), dim3(64), 0, st, tspec, th,
-                           sspec, sh, NL, L, rq, T(1.0 / double(L)));
+        // lags per thread: 17 once there is more than one group of 9 (each group
+        // re-reads every spectrum; the sums themselves do not depend on the grouping)
+        if (NL > 9)
+            hipLaunchKernelGGL((k_mac_lags<T, 17>), dim3(idivup(H1, 64), idivup(NL, 17), n), dim3(64), 0, st, tspec,
+                               th, sspec, sh, NL, L, rq, T(1.0 / double(L)));
+        else
+            hipLaunchKernelGGL((k_mac_lags<T, 9>), dim3(idivup(H1, 64), idivup(NL, 9), n), dim3(64), 0, st, tspec,
+                               th, sspec, sh, NL, L, rq, T(1.0 / double(L)));
         hipLaunchKernelGGL(k_lagrows_real<T>, dim3(idivup(n * NL, g)), dim3(256), lds, st, rq, n * NL, g, d, NLw, out);
     };
     // the 16-sparse surface kernel needs no 256x256 buffers

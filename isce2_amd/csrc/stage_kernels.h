@@ -335,11 +335,12 @@ __global__ void __launch_bounds__(64) k_mac_lags(const cx<T> *tspec, int th, con
     }
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-        if (u0 + j >= nl) break;
-        cx<T> *row = q + (size_t(win) * nl + u0 + j) * L;
-        const cx<T> v = acc[j] * scale;
-        row[k] = v;
-        if (k && L - k != k) row[L - k] = conj(v);
+        if (u0 + j < nl) {   // no early exit: the unrolled loop keeps acc[] in registers
+            cx<T> *row = q + (size_t(win) * nl + u0 + j) * L;
+            const cx<T> v = acc[j] * scale;
+            row[k] = v;
+            if (k && L - k != k) row[L - k] = conj(v);
+        }
     }
 }
 
