@@ -81,6 +81,19 @@ def test_staged_real_image_and_small_window(eng, scene1):
     _assert_parity(c)
 
 
+@pytest.mark.parametrize("deramp", [0, 1])
+def test_staged_odd_rectangular_windows(eng, scene1, deramp):
+    """Odd and unequal chip sides: the quadrant padding drops the middle row/column
+    of an odd spectrum, the row-pair spectra end on an unpaired row."""
+    ref, sec = scene1
+    cfg = O.AmpcorConfig(window_h=33, window_w=40, hs_down=7, hs_across=9, deramp=deramp)
+    rd, ra, sd, sa, _, _ = O.start_pixels(5, 4, 7, 9, 80, 96, 0, 0, 7, 9)
+    g, o = run_both(eng, ref, sec, cfg, (rd, ra, sd, sa), MODE_STAGED, chunk=6)
+    c = compare(g, o)
+    _report(f"staged odd 33x40 deramp={deramp}", c)
+    _assert_parity(c)
+
+
 def test_staged_edges_and_errors(eng, scene1):
     ref, sec = scene1
     cfg = O.AmpcorConfig(window_h=64, window_w=64, hs_down=16, hs_across=16)
