@@ -13,8 +13,9 @@ One step = every class's launches on one stream in no-sync mode; with
 ``graph=True`` the step is captured once into a HIP graph
 (``torch.cuda.CUDAGraph``) and replayed. Under precision 1 the windows whose
 argmax margins fall below the thresholds are re-run through the FP64 pipeline
-after the step (a separate engine context, so the graph's buffers are never
-touched by the re-run).
+after the step: the margins of every class come back in one device-to-host
+copy, and each class re-runs on its own stream and engine context (so the
+graph's buffers are never touched and the classes' re-runs overlap on the GPU).
 """
 from __future__ import annotations
 
@@ -53,6 +54,8 @@ class _Class:
             self.out[k] = torch.full((m,), float("nan"), dtype=torch.float32, device=dev)
         self.sched = Schedule(mode=MODE_FUSED if self.sizes["fused"] else MODE_STAGED,
                               chunk=2048 if self.sizes["fused"] else 256)
+        self.rerun_eng = None      # FP64 re-run context and stream, made on first use
+        self.rerun_stream = None
 
     def args(self):
         return (self.ref, self.sec, *self.starts)
@@ -87,7 +90,6 @@ class ChipPairBatch:
                 c.sched.apply(c.eng)
                 c.eng.setup(c.params)
             c.eng.set_precision(2 if precision == 2 else 0)
-        self._rerun_eng = Engine(device) if precision == 1 else None
         self.graph = None
         self.stream = torch.cuda.Stream(device=self.dev)
         self.rerun_count = 0
@@ -130,21 +132,34 @@ class ChipPairBatch:
 
     def _rerun_near_ties(self):
         self.rerun_count = 0
+        mc = torch.cat([c.out["margin_coarse"] for c in self.classes]).cpu().numpy()
+        mf = torch.cat([c.out["margin_fine"] for c in self.classes]).cpu().numpy()
+        near = ~(mc >= self.tau) | ~(mf >= self.tau)
+        main = torch.cuda.current_stream(self.dev)
+        used = []
+        o0 = 0
         for c in self.classes:
-            mc = c.out["margin_coarse"].cpu().numpy()
-            mf = c.out["margin_fine"].cpu().numpy()
-            sel = np.nonzero(~(mc >= self.tau) | ~(mf >= self.tau))[0]
+            sel = np.nonzero(near[o0:o0 + c.m])[0]
+            o0 += c.m
             if len(sel) == 0:
                 continue
             self.rerun_count += len(sel)
-            e = self._rerun_eng
-            e.setup(c.params)
-            e.set_precision(2)
-            t = torch.as_tensor(sel, dtype=torch.int64, device=self.dev)
-            st = [s.index_select(0, t).contiguous() for s in c.starts]
-            o = e.run_windows(c.ref, c.sec, *st, mode=MODE_STAGED, debug=True)
-            for k in ("offset", "snr", "cov", "maxloc", "start", "shift", "zoom_idx", "maxval"):
-                c.out[k].index_copy_(0, t, o[k])
+            if c.rerun_eng is None:
+                c.rerun_eng = Engine(self.device)
+                c.rerun_eng.setup(c.params)
+                c.rerun_eng.set_precision(2)
+                c.rerun_stream = torch.cuda.Stream(device=self.dev)
+            s = c.rerun_stream
+            s.wait_stream(main)
+            with torch.cuda.stream(s):
+                t = torch.as_tensor(sel, dtype=torch.int64).to(self.dev)
+                st = [x.index_select(0, t).contiguous() for x in c.starts]
+                o = c.rerun_eng.run_windows(c.ref, c.sec, *st, mode=MODE_STAGED, debug=True)
+                for k in ("offset", "snr", "cov", "maxloc", "start", "shift", "zoom_idx", "maxval"):
+                    c.out[k].index_copy_(0, t, o[k])
+            used.append(s)
+        for s in used:
+            main.wait_stream(s)
 
     def results(self):
         """Host arrays in the caller's pair order: offset [n,2], snr [n], cov [n,3] (+ zoom_idx, maxloc)."""
@@ -160,5 +175,5 @@ class ChipPairBatch:
         self.graph = None
         for c in self.classes:
             c.eng.close()
-        if self._rerun_eng is not None:
-            self._rerun_eng.close()
+            if c.rerun_eng is not None:
+                c.rerun_eng.close()

@@ -54,6 +54,23 @@ def test_uneven_batch_equals_oracle(groups):
     assert not bad, bad
 
 
+def test_forced_fp64_rerun_equals_oracle(groups):
+    """tau = inf sends every pair through the per-class FP64 re-run (own stream and
+    context per class, overlapping): results equal the oracle's, twice in a row."""
+    b = ChipPairBatch(groups, device=0, graph=True, tau=float("inf"))
+    o = _oracle(groups)
+    for rep in range(2):
+        b.run()
+        r = b.results()
+        assert b.rerun_count == len(o)
+        for i, oi in o.items():
+            assert np.array_equal(r["maxloc"][i], oi["maxloc"][0]), (rep, i)
+            assert np.array_equal(r["zoom_idx"][i], oi["zoom_idx"][0]), (rep, i)
+            assert np.array_equal(r["offset"][i], oi["offset"][0].astype(np.float32)), (rep, i)
+            assert abs(r["snr"][i] - oi["snr"][0]) <= 1e-5 * abs(oi["snr"][0]), (rep, i)
+    b.close()
+
+
 def test_graph_replays_equal_eager(groups):
     eager = ChipPairBatch(groups, device=0, graph=False)
     eager.run()
