@@ -25,7 +25,9 @@ Extension attributes (not in the reference): ``engineMode`` ("auto",
 from __future__ import annotations
 
 import operator
+import os
 import time
+from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import torch
@@ -57,11 +59,26 @@ _STR_PROPS = {
 _ALIASES = {"grossOffsetDownStatic": 0, "grossOffsetAcrossStatic": 0}
 
 
-def _to_device(img, cplx, dev):
+def _to_device(img, cplx, dev, threads=None):
     """Raster (memory-mapped, possibly read-only or strided) -> device tensor,
-    staged through one pinned host buffer."""
+    staged through one pinned host buffer. Large rasters are read into it by
+    row bands on several host threads (NumPy copies release the GIL; reading a
+    memory-mapped file is bound by page faults and memcpy, not by one core's
+    share of the bus)."""
     host = torch.empty(img.shape, dtype=torch.complex64 if cplx else torch.float32, pin_memory=True)
-    host.numpy()[...] = img
+    dst = host.numpy()
+    rows = img.shape[0]
+    nthr = threads or min(16, os.cpu_count() or 1)
+    if nthr <= 1 or dst.nbytes < (64 << 20) or rows < 2 * nthr:
+        dst[...] = img
+    else:
+        step = -(-rows // nthr)
+
+        def band(r0):
+            dst[r0:r0 + step] = img[r0:r0 + step]
+
+        with ThreadPoolExecutor(nthr) as ex:
+            list(ex.map(band, range(0, rows, step)))
     out = host.to(dev, non_blocking=True)
     torch.cuda.current_stream(dev).synchronize()
     return out
