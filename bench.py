@@ -142,21 +142,22 @@ def main():
     ms_step = 1000 * elapsed / args.steps
     gpu_out = {k: v.cpu().numpy() for k, v in out.items()}
 
-    # host-resident variant: pinned host images -> HBM -> field -> host outputs, per pass
+    # host-resident variant: pinned host images -> HBM -> field -> host outputs, per pass;
+    # row bands upload on a copy stream while earlier bands of windows run
     h_ref = torch.from_numpy(ref_np).pin_memory()
     h_sec = torch.from_numpy(sec_np).pin_memory()
-    d_ref, d_sec = torch.empty_like(ref), torch.empty_like(sec)
+    host_starts = (ref_d, ref_a, sec_d, sec_a)
+    host_out = eng.run_windows_from_host(h_ref, h_sec, *host_starts, mode=mode)
     torch.cuda.synchronize(dev)
     t1 = time.perf_counter()
-    for _ in range(2):
-        d_ref.copy_(h_ref, non_blocking=True)
-        d_sec.copy_(h_sec, non_blocking=True)
-        o_h = eng.run_windows(d_ref, d_sec, *starts, mode=mode)
-        host_out = {k: v.cpu() for k, v in o_h.items()}
-    t_host = (time.perf_counter() - t1) / 2
+    for _ in range(3):
+        host_out = eng.run_windows_from_host(h_ref, h_sec, *host_starts, mode=mode)
+    t_host = (time.perf_counter() - t1) / 3
     pcie = {"windows_per_s": round(n / t_host, 1), "ms_per_field": round(1e3 * t_host, 2),
-            "note": "pinned host images copied to HBM, whole field, outputs copied back; not the headline value"}
-    del d_ref, d_sec, h_ref, h_sec, host_out
+            "max_abs_offset_diff_vs_device_resident": float(np.abs(host_out["offset"] - gpu_out["offset"]).max()),
+            "note": "pinned host images uploaded to HBM in 8 row bands on a copy stream overlapping the "
+                    "earlier bands' windows, outputs copied back; not the headline value"}
+    del h_ref, h_sec, host_out
 
     # roofline of the dominant kernel, from the events recorded inside the timed region
     roof = None

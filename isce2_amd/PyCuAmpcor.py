@@ -25,9 +25,7 @@ Extension attributes (not in the reference): ``engineMode`` ("auto",
 from __future__ import annotations
 
 import operator
-import os
 import time
-from concurrent.futures import ThreadPoolExecutor
 
 import numpy as np
 import torch
@@ -57,31 +55,6 @@ _STR_PROPS = {
     "snrImageName": "snr.snr", "covImageName": "cov.cov",
 }
 _ALIASES = {"grossOffsetDownStatic": 0, "grossOffsetAcrossStatic": 0}
-
-
-def _to_device(img, cplx, dev, threads=None):
-    """Raster (memory-mapped, possibly read-only or strided) -> device tensor,
-    staged through one pinned host buffer. Large rasters are read into it by
-    row bands on several host threads (NumPy copies release the GIL; reading a
-    memory-mapped file is bound by page faults and memcpy, not by one core's
-    share of the bus)."""
-    host = torch.empty(img.shape, dtype=torch.complex64 if cplx else torch.float32, pin_memory=True)
-    dst = host.numpy()
-    rows = img.shape[0]
-    nthr = threads or min(16, os.cpu_count() or 1)
-    if nthr <= 1 or dst.nbytes < (64 << 20) or rows < 2 * nthr:
-        dst[...] = img
-    else:
-        step = -(-rows // nthr)
-
-        def band(r0):
-            dst[r0:r0 + step] = img[r0:r0 + step]
-
-        with ThreadPoolExecutor(nthr) as ex:
-            list(ex.map(band, range(0, rows, step)))
-    out = host.to(dev, non_blocking=True)
-    torch.cuda.current_stream(dev).synchronize()
-    return out
 _MODES = {"auto": MODE_AUTO, "staged": MODE_STAGED, "fused": MODE_FUSED}
 
 
@@ -241,25 +214,37 @@ class PyCuAmpcor:
         cplx = np.iscomplexobj(ref_np)
         if cplx != np.iscomplexobj(sec_np):
             raise raster.RasterError("reference and secondary rasters must both be complex or both real")
-        dev = torch.device("cuda", v["deviceID"])
         t0 = time.perf_counter()
-        ref = _to_device(ref_np, cplx, dev)
-        sec = _to_device(sec_np, cplx, dev)
-        out = self.run_arrays(ref, sec)
-        torch.cuda.synchronize(dev)
+        # row bands of the rasters are staged and uploaded while earlier bands of
+        # windows run (the reference loads and runs chunk by chunk in turn)
+        eng, mode, chunk = self._prepared_engine(v["deviceID"], cplx)
+        out = eng.run_windows_from_host(ref_np, sec_np, st["ref_d"], st["ref_a"], st["sec_d"], st["sec_a"],
+                                        mode=mode, chunk=chunk)
         st["seconds"] = time.perf_counter() - t0
         nd, na = v["numberWindowDown"], v["numberWindowAcross"]
-        offset = out["offset"].cpu().numpy().reshape(nd, na, 2)
+        offset = out["offset"].reshape(nd, na, 2)
         gross = np.stack([st["gross_d"], st["gross_a"]], axis=1).astype(np.float32).reshape(nd, na, 2)
         if v["mergeGrossOffset"]:
             offset = offset + gross
         raster.write_raw(v["offsetImageName"], offset)
         raster.write_raw(v["grossOffsetImageName"], gross)
-        raster.write_raw(v["snrImageName"], out["snr"].cpu().numpy().reshape(nd, na))
-        raster.write_raw(v["covImageName"], out["cov"].cpu().numpy().reshape(nd, na, 3))
-        st["result"] = dict(offset=offset, gross=gross, snr=out["snr"].cpu().numpy().reshape(nd, na),
-                            cov=out["cov"].cpu().numpy().reshape(nd, na, 3))
+        raster.write_raw(v["snrImageName"], out["snr"].reshape(nd, na))
+        raster.write_raw(v["covImageName"], out["cov"].reshape(nd, na, 3))
+        st["result"] = dict(offset=offset, gross=gross, snr=out["snr"].reshape(nd, na),
+                            cov=out["cov"].reshape(nd, na, 3))
         return None
+
+    def _prepared_engine(self, device, cplx):
+        """The engine for ``device`` set up for this object's parameters, with the
+        mode and chunk that ``engineMode`` and the chunk attributes select."""
+        v = self._v
+        eng = self._engine(device)
+        p = self._window_params()
+        p.image_complex = 1 if cplx else 0
+        eng.setup(p)
+        mode = _MODES[v["engineMode"]]
+        chunk = v["numberWindowDownInChunk"] * v["numberWindowAcrossInChunk"] if mode == MODE_STAGED else 0
+        return eng, mode, chunk if chunk > 1 else 0
 
     def run_arrays(self, ref, sec, debug=False):
         """Run on device-resident images (torch tensors); returns device tensors.
@@ -267,14 +252,9 @@ class PyCuAmpcor:
         The windows are those set by ``set*GrossOffset``; the engine checks
         that every chip lies inside its image."""
         self._need_setup()
-        st, v = self._state, self._v
+        st = self._state
         dev = ref.device
-        eng = self._engine(dev.index or 0)
-        p = self._window_params()
-        p.image_complex = 1 if ref.is_complex() else 0
-        eng.setup(p)
+        eng, mode, chunk = self._prepared_engine(dev.index or 0, ref.is_complex())
         starts = [torch.as_tensor(np.ascontiguousarray(st[k], dtype=np.int32)).to(dev)
                   for k in ("ref_d", "ref_a", "sec_d", "sec_a")]
-        mode = _MODES[v["engineMode"]]
-        chunk = v["numberWindowDownInChunk"] * v["numberWindowAcrossInChunk"] if mode == MODE_STAGED else 0
-        return eng.run_windows(ref, sec, *starts, mode=mode, chunk=chunk if chunk > 1 else 0, debug=debug)
+        return eng.run_windows(ref, sec, *starts, mode=mode, chunk=chunk, debug=debug)

@@ -8,6 +8,8 @@ stream provider here; all arithmetic runs in the HIP kernels of
 from __future__ import annotations
 
 import ctypes as C
+import os
+from concurrent.futures import ThreadPoolExecutor
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -15,6 +17,8 @@ import torch
 
 from . import _lib
 from ._lib import FLAG_NOSYNC, MODE_AUTO, MODE_FUSED, MODE_STAGED, Params, Sizes, check
+
+TAU_DEFAULT = 2e-6   # the engine's default precision-1 margin thresholds
 
 __all__ = ["Engine", "WindowParams", "MODE_AUTO", "MODE_STAGED", "MODE_FUSED"]
 
@@ -56,6 +60,7 @@ class Engine:
         self._h = h
         self.params: WindowParams | None = None
         self.precision = 1
+        self.tau = [TAU_DEFAULT, TAU_DEFAULT]   # precision-1 margin thresholds (coarse, fine)
         self.sizes: dict | None = None
 
     def close(self):
@@ -127,10 +132,122 @@ class Engine:
         check(code, self._h)
         return res
 
+    def run_windows_from_host(self, ref, sec, ref_d, ref_a, sec_d, sec_a, *, mode=MODE_AUTO, chunk=0,
+                              bands=8, threads=None):
+        """Run on host-resident images, overlapping their upload with the computation.
+
+        ``ref``/``sec`` are 2-D NumPy arrays (memory maps included) or CPU
+        tensors; pinned contiguous tensors of the image dtype are used in
+        place, anything else is staged into pinned buffers by row band on
+        ``threads`` host threads. The windows (host integer start arrays) are
+        split into ``bands`` consecutive groups; the rows each group reads are
+        copied to the device on a copy stream, and each group runs as soon as
+        its rows have arrived, so later uploads overlap earlier groups'
+        kernels. Windows in row order give the most overlap. The engine's
+        bounds rule is checked on the host first (IndexError), so the groups
+        run without host synchronization; under precision 1 the near-tie
+        windows of all groups are then re-run once through the FP64 pipeline
+        (offsets as ``run_windows`` gives them; SNR and covariance of those
+        windows from the FP64 run). Returns host arrays ``offset [n,2]``,
+        ``snr [n]``, ``cov [n,3]``.
+        """
+        if self.params is None:
+            raise RuntimeError("setup() must be called first")
+        dev = torch.device("cuda", self.device)
+        ref_d, ref_a, sec_d, sec_a = (np.ascontiguousarray(a, dtype=np.int64).ravel()
+                                      for a in (ref_d, ref_a, sec_d, sec_a))
+        n = len(ref_d)
+        if not (len(ref_a) == len(sec_d) == len(sec_a) == n):
+            raise ValueError("start arrays must have equal length")
+        dt = torch.complex64 if self.params.image_complex else torch.float32
+
+        def pinned(img):
+            if isinstance(img, torch.Tensor):
+                if img.device.type != "cpu":
+                    raise ValueError("images must be host-resident")
+                if img.is_pinned() and img.dtype == dt and img.dim() == 2 and img.is_contiguous():
+                    return img, None
+                img = img.numpy()
+            if img.ndim != 2:
+                raise ValueError("images must be 2-D")
+            return torch.empty(img.shape, dtype=dt, pin_memory=True), img
+
+        p_ref, s_ref = pinned(ref)
+        p_sec, s_sec = pinned(sec)
+        comp = torch.cuda.current_stream(dev)
+        d_ref = torch.empty(p_ref.shape, dtype=dt, device=dev)
+        d_sec = torch.empty(p_sec.shape, dtype=dt, device=dev)
+        out = dict(offset=torch.empty((n, 2), dtype=torch.float32, device=dev),
+                   snr=torch.empty(n, dtype=torch.float32, device=dev),
+                   cov=torch.empty((n, 3), dtype=torch.float32, device=dev))
+        if n == 0:
+            return {k: v.cpu().numpy() for k, v in out.items()}
+        # the engine's bounds rule, checked here so that the bands run without host syncs
+        wh, ww, sh, sw = self.params.window_h, self.params.window_w, self.sizes["search_h"], self.sizes["search_w"]
+        outside = ((ref_d < 0) | (ref_a < 0) | (ref_d + wh > p_ref.shape[0]) | (ref_a + ww > p_ref.shape[1])
+                   | (sec_d < 0) | (sec_a < 0) | (sec_d + sh > p_sec.shape[0]) | (sec_a + sw > p_sec.shape[1]))
+        if outside.any():
+            raise IndexError(f"window {int(np.argmax(outside))} reads outside its image")
+        st = to_device_starts((ref_d, ref_a, sec_d, sec_a), dev)
+        nb = max(1, min(int(bands), n))
+        edges = np.linspace(0, n, nb + 1).astype(np.int64)
+        h_ref, h_sec = p_ref.shape[0], p_sec.shape[0]
+        need_r = np.maximum.accumulate(ref_d + wh)
+        need_s = np.maximum.accumulate(sec_d + sh)
+        copy = torch.cuda.Stream(device=dev)
+        copy.wait_stream(comp)              # the device images' memory may be in use on comp until now
+        events, up_r, up_s = [], 0, 0
+        for b in range(nb):
+            last = edges[b + 1] - 1
+            tr = h_ref if b == nb - 1 else max(up_r, int(need_r[last]))
+            ts = h_sec if b == nb - 1 else max(up_s, int(need_s[last]))
+            for src, pin, dst, r0, r1 in ((s_ref, p_ref, d_ref, up_r, tr), (s_sec, p_sec, d_sec, up_s, ts)):
+                if r1 <= r0:
+                    continue
+                if src is not None:
+                    fill_rows(pin.numpy(), src, r0, r1, threads)
+                with torch.cuda.stream(copy):
+                    dst[r0:r1].copy_(pin[r0:r1], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(copy)
+            events.append(ev)
+            up_r, up_s = tr, ts
+        d_ref.record_stream(copy)
+        d_sec.record_stream(copy)
+        # the bands run without host synchronization (the starts are checked above);
+        # under precision 1 the near-tie windows of all bands are re-run in FP64 once
+        rerun = self.precision == 1
+        if rerun:
+            for k in ("margin_coarse", "margin_fine"):
+                out[k] = torch.empty(n, dtype=torch.float32, device=dev)
+        for b in range(nb):
+            i0, i1 = int(edges[b]), int(edges[b + 1])
+            comp.wait_event(events[b])
+            if i1 > i0:
+                self.run_windows(d_ref, d_sec, *(s[i0:i1] for s in st), mode=mode, chunk=chunk, debug=rerun,
+                                 nosync=True, out={k: v[i0:i1] for k, v in out.items()})
+        if rerun:
+            mc, mf = out.pop("margin_coarse").cpu().numpy(), out.pop("margin_fine").cpu().numpy()
+            sel = np.nonzero(~(mc >= self.tau[0]) | ~(mf >= self.tau[1]))[0]
+            if len(sel):
+                t = torch.as_tensor(sel, dtype=torch.int64).to(dev)
+                self.set_precision(2)
+                try:
+                    o = self.run_windows(d_ref, d_sec, *(s.index_select(0, t).contiguous() for s in st),
+                                         mode=MODE_STAGED)
+                finally:
+                    self.set_precision(1)
+                for k in ("offset", "snr", "cov"):
+                    out[k].index_copy_(0, t, o[k])
+        return {k: v.cpu().numpy() for k, v in out.items()}
+
     def set_precision(self, mode: int, tau_coarse: float = -1.0, tau_fine: float = -1.0):
         """0: FP32 only; 1: FP32 + FP64 re-run of near-tie windows (default); 2: FP64 for all."""
         check(self.lib.ampcor_set_precision(self._h, int(mode), float(tau_coarse), float(tau_fine)), self._h)
         self.precision = int(mode)
+        for i, t in enumerate((tau_coarse, tau_fine)):
+            if t >= 0:
+                self.tau[i] = float(t)
 
     def set_option(self, name: str, value: int):
         """Launch-shape option (``fine_threads``: 512 or 1024)."""
@@ -169,6 +286,25 @@ class Engine:
     @staticmethod
     def version() -> str:
         return _lib.load().ampcor_version().decode()
+
+
+def fill_rows(dst, src, r0, r1, threads=None):
+    """dst[r0:r1] = src[r0:r1] for 2-D arrays; large copies run as row bands on
+    several host threads (NumPy copies release the GIL, and reading a memory-
+    mapped file is bound by page faults and memcpy rather than by one core)."""
+    rows = r1 - r0
+    nthr = threads or min(16, os.cpu_count() or 1)
+    if nthr <= 1 or rows < 2 * nthr or rows * dst[0].nbytes < (64 << 20):
+        dst[r0:r1] = src[r0:r1]
+        return
+    step = -(-rows // nthr)
+
+    def band(a):
+        b = min(a + step, r1)
+        dst[a:b] = src[a:b]
+
+    with ThreadPoolExecutor(nthr) as ex:
+        list(ex.map(band, range(r0, r1, step)))
 
 
 def to_device_starts(arrays, device):

@@ -214,3 +214,28 @@ def test_launch_options_do_not_change_results(eng, scene1):
         eng.set_option("fine_threads", 256)
     with pytest.raises(ValueError):
         eng.set_option("no_such_option", 1)
+
+
+@pytest.mark.parametrize("pin", [True, False])
+def test_host_inputs_banded_equals_device_resident(eng, scene1, pin):
+    """run_windows_from_host (row bands uploaded on a copy stream while earlier
+    bands run) gives the device-resident call's offsets bit for bit (SNR and
+    covariance of re-run near ties come from a full FP64 run there)."""
+    ref, sec = scene1
+    cfg = O.AmpcorConfig(hs_down=20, hs_across=20)
+    rd, ra, sd, sa = _grid(5, 5, 20, 80, 20)
+    from .gpu_helpers import params_from_cfg
+    eng.setup(params_from_cfg(cfg))
+    dev = torch.device("cuda", 0)
+    st = [torch.as_tensor(np.asarray(s, np.int32)).to(dev) for s in (rd, ra, sd, sa)]
+    a = eng.run_windows(torch.from_numpy(ref).to(dev), torch.from_numpy(sec).to(dev), *st)
+    torch.cuda.synchronize()
+    h_ref, h_sec = (torch.from_numpy(ref).pin_memory(), torch.from_numpy(sec).pin_memory()) if pin else (ref, sec)
+    b = eng.run_windows_from_host(h_ref, h_sec, rd, ra, sd, sa, bands=3)
+    assert np.array_equal(a["offset"].cpu().numpy(), b["offset"])
+    for k in ("snr", "cov"):
+        np.testing.assert_allclose(b[k], a[k].cpu().numpy(), rtol=1e-5, atol=0)
+    bad = np.array(sd)
+    bad[-1] = ref.shape[0]             # a search chip past the last row
+    with pytest.raises(IndexError):
+        eng.run_windows_from_host(h_ref, h_sec, rd, ra, bad, sa, bands=2)
