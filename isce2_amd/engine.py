@@ -196,6 +196,9 @@ class Engine:
         need_s = np.maximum.accumulate(sec_d + sh)
         copy = torch.cuda.Stream(device=dev)
         copy.wait_stream(comp)              # the device images' memory may be in use on comp until now
+        pool, nthr = None, threads or min(16, os.cpu_count() or 1)
+        if (s_ref is not None or s_sec is not None) and nthr > 1:
+            pool = ThreadPoolExecutor(nthr)
         events, up_r, up_s = [], 0, 0
         for b in range(nb):
             last = edges[b + 1] - 1
@@ -205,13 +208,15 @@ class Engine:
                 if r1 <= r0:
                     continue
                 if src is not None:
-                    fill_rows(pin.numpy(), src, r0, r1, threads)
+                    fill_rows(pin.numpy(), src, r0, r1, pool, nthr)
                 with torch.cuda.stream(copy):
                     dst[r0:r1].copy_(pin[r0:r1], non_blocking=True)
             ev = torch.cuda.Event()
             ev.record(copy)
             events.append(ev)
             up_r, up_s = tr, ts
+        if pool is not None:
+            pool.shutdown()
         d_ref.record_stream(copy)
         d_sec.record_stream(copy)
         # the bands run without host synchronization (the starts are checked above);
@@ -288,13 +293,13 @@ class Engine:
         return _lib.load().ampcor_version().decode()
 
 
-def fill_rows(dst, src, r0, r1, threads=None):
-    """dst[r0:r1] = src[r0:r1] for 2-D arrays; large copies run as row bands on
-    several host threads (NumPy copies release the GIL, and reading a memory-
-    mapped file is bound by page faults and memcpy rather than by one core)."""
+def fill_rows(dst, src, r0, r1, pool=None, nthr=1):
+    """dst[r0:r1] = src[r0:r1] for 2-D arrays; with a thread pool, large copies
+    run as row bands on its threads (NumPy copies release the GIL, and reading
+    a memory-mapped file is bound by page faults and memcpy rather than by one
+    core). ``nthr`` is the pool's thread count."""
     rows = r1 - r0
-    nthr = threads or min(16, os.cpu_count() or 1)
-    if nthr <= 1 or rows < 2 * nthr or rows * dst[0].nbytes < (64 << 20):
+    if pool is None or nthr <= 1 or rows < 2 * nthr or rows * dst[0].nbytes < (16 << 20):
         dst[r0:r1] = src[r0:r1]
         return
     step = -(-rows // nthr)
@@ -303,8 +308,7 @@ def fill_rows(dst, src, r0, r1, threads=None):
         b = min(a + step, r1)
         dst[a:b] = src[a:b]
 
-    with ThreadPoolExecutor(nthr) as ex:
-        list(ex.map(band, range(r0, r1, step)))
+    list(pool.map(band, range(r0, r1, step)))
 
 
 def to_device_starts(arrays, device):
