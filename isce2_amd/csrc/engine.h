@@ -108,8 +108,10 @@ struct Engine {
         hipStream_t stream;
         float *margin_coarse = nullptr, *margin_fine = nullptr;   // debug: top-1 minus top-2
         bool nosync = false;                                       // AMPCOR_FLAG_NOSYNC
-        // staged pipeline from the zoom stage on: per-window coarse zoom start and shift
+        // windows [fine_from, n) take their coarse zoom start and shift from these
+        // (the staged pipeline skips its raw stage for chunks of such windows only)
         const int32_t *fine_start = nullptr, *fine_shift = nullptr;
+        int32_t fine_from = 0;
     };
     // precision: 0 FP32 only; 1 FP32 + FP64 re-run of windows whose argmax
     // margins fall below the thresholds; 2 FP64 staged pipeline for every window

@@ -296,6 +296,16 @@ void fft2_chip(Engine &e, const cx<T> *src, int ih, int iw, const int32_t *start
     fft_cols<T, DIR>(e.plan(h), dst, nimg, w, fft_elems<T>(e), st, cols_width<T>(e));
 }
 
+// start[i], shift[i] = the given values for windows f0 <= i < n
+__global__ void k_given_starts(const int32_t *gs, const int32_t *gh, int f0, int n, int32_t *start, int32_t *shift) {
+    const int i = f0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    start[2 * i] = gs[2 * i];
+    start[2 * i + 1] = gs[2 * i + 1];
+    shift[2 * i] = gh[2 * i];
+    shift[2 * i + 1] = gh[2 * i + 1];
+}
+
 inline dim3 grid2(int per_img, int nimg) { return dim3(std::min(idivup(per_img, 256), 64), nimg); }
 }  // namespace
 
@@ -404,11 +414,14 @@ void Engine::run_staged_t(const RunArgs &a) {
     T *r_adj = buf<T>(tn + "r_adj", size_t(nb) * Z * Z);
     if (p.surf_method) ensure_sinc_table();
 
-    // fine-only mode: the coarse stage's zoom start and shift are given (a
-    // re-run of windows whose coarse peak is not in doubt)
-    const bool fine_only = a.fine_start != nullptr;
+    // windows from a.fine_from on have their coarse zoom start and shift given
+    // (a re-run of windows whose coarse peak is not in doubt): chunks of only
+    // such windows skip the raw stage; in a mixed chunk they are overwritten
+    // after it
+    const bool given = a.fine_start != nullptr;
     for (int w0 = 0; w0 < a.n; w0 += nb) {
         const int n = std::min(nb, a.n - w0);
+        const bool fine_only = given && w0 >= a.fine_from;
         // chip load + amplitude + mean (cuAmpcorChunk.cpp:18-39)
         hipLaunchKernelGGL(k_gather<T>, grid2(W_h * W_w, n), dim3(256), 0, st, a.ref, a.ref_pitch,
                            a.ref_d + w0, a.ref_a + w0, W_h, W_w, mode, c_ref, fine_only ? nullptr : r_ref);
@@ -443,6 +456,11 @@ void Engine::run_staged_t(const RunArgs &a) {
             hipLaunchKernelGGL(k_stats<T>, dim3(n), dim3(64), 0, st, corr, sz.raw_corr_h, sz.raw_corr_w, n,
                                loc, val, W_h * W_w, sz.corr_raw_zoom_h, sz.corr_raw_zoom_w, p.hs_down, p.hs_across,
                                sz.half_zoom, a.cov + 3 * size_t(w0), a.snr + w0, start, shift);
+            if (given && a.fine_from < w0 + n) {
+                const int f0 = a.fine_from - w0;
+                hipLaunchKernelGGL(k_given_starts, dim3(idivup(n - f0, 256)), dim3(256), 0, st,
+                                   a.fine_start + 2 * size_t(w0), a.fine_shift + 2 * size_t(w0), f0, n, start, shift);
+            }
         }
         // reference oversampling (cuAmpcorChunk.cpp:115-126)
         if (p.deramp == 1)
@@ -565,9 +583,10 @@ __global__ void k_put(const U *src, const int32_t *idx, U *dst, int n, int width
 // its threshold, and overwrite that window's outputs. FP32 rounding can only
 // reorder near-equal candidates, so this makes the integer peaks those of an
 // FP64 evaluation. When the coarse zoom start and shift of every window are
-// given (start32, shift32) and only a window's fine margin is small, only its
-// fine stage is re-run, from that start: its coarse peak, SNR and covariance
-// stand as computed.
+// given (start32, shift32) and only a window's fine margin is small, its fine
+// stage is re-run from that start and its coarse peak, SNR and covariance stand
+// as computed. Both kinds go through one staged FP64 pass (one launch per
+// stage for all of them: the pass is latency-bound at these window counts).
 void Engine::rerun_fp64(const RunArgs &a, const float *mc, const float *mf, const int32_t *start32,
                         const int32_t *shift32) {
     std::vector<float> hc(a.n), hf(a.n);
@@ -582,53 +601,54 @@ void Engine::rerun_fp64(const RunArgs &a, const float *mc, const float *mf, cons
     }
     prec_checked += a.n;
     prec_rerun += long(full.size() + fine.size());
-    auto subset = [&](const std::vector<int32_t> &idx, bool fine_only) {
-        if (idx.empty()) return;
-        const int m = int(idx.size());
-        const std::string pre = fine_only ? "q_" : "p_";
-        int32_t *d_idx = buf<int32_t>(pre + "idx", m);
-        HIP_OK(hipMemcpyAsync(d_idx, idx.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, a.stream));
-        int32_t *st4 = buf<int32_t>(pre + "starts", size_t(8) * m);
-        const int32_t *srcs[4] = {a.ref_d, a.ref_a, a.sec_d, a.sec_a};
-        for (int k = 0; k < 4; ++k)
-            hipLaunchKernelGGL(k_take_i32, dim3(idivup(m, 256)), dim3(256), 0, a.stream, srcs[k], d_idx, st4 + k * m, m, 1);
-        float *o_off = buf<float>(pre + "off", size_t(2) * m), *o_snr = buf<float>(pre + "snr", m);
-        float *o_cov = buf<float>(pre + "cov", size_t(3) * m), *o_val = buf<float>(pre + "val", m);
-        int32_t *o_i = buf<int32_t>(pre + "ints", size_t(8) * m);
-        RunArgs sub = a;
-        sub.ref_d = st4; sub.ref_a = st4 + m; sub.sec_d = st4 + 2 * m; sub.sec_a = st4 + 3 * m;
-        sub.n = m;
-        sub.margin_coarse = sub.margin_fine = nullptr;
-        sub.offset = o_off; sub.snr = o_snr; sub.cov = o_cov; sub.maxval = o_val;
-        sub.maxloc = o_i; sub.start = o_i + 2 * m; sub.shift = o_i + 4 * m; sub.zoom_idx = o_i + 6 * m;
-        sub.chunk = 0;
-        if (fine_only) {
-            hipLaunchKernelGGL(k_take_i32, dim3(idivup(2 * m, 256)), dim3(256), 0, a.stream, start32, d_idx, st4 + 4 * m, m, 2);
-            hipLaunchKernelGGL(k_take_i32, dim3(idivup(2 * m, 256)), dim3(256), 0, a.stream, shift32, d_idx, st4 + 6 * m, m, 2);
-            sub.fine_start = st4 + 4 * m;
-            sub.fine_shift = st4 + 6 * m;
-        }
-        run_staged_t<double>(sub);
-        auto put_f = [&](const float *src, float *dst, int width) {
-            if (dst) hipLaunchKernelGGL(k_put<float>, dim3(idivup(m * width, 256)), dim3(256), 0, a.stream, src, d_idx, dst, m, width);
-        };
-        auto put_i = [&](const int32_t *src, int32_t *dst) {
-            if (dst) hipLaunchKernelGGL(k_put<int32_t>, dim3(idivup(m * 2, 256)), dim3(256), 0, a.stream, src, d_idx, dst, m, 2);
-        };
-        put_f(o_off, a.offset, 2);
-        put_i(sub.zoom_idx, a.zoom_idx);
-        if (!fine_only) {
-            put_f(o_snr, a.snr, 1);
-            put_f(o_cov, a.cov, 3);
-            put_f(o_val, a.maxval, 1);
-            put_i(sub.maxloc, a.maxloc);
-            put_i(sub.start, a.start);
-            put_i(sub.shift, a.shift);
-        }
-        HIP_OK(hipGetLastError());
+    // one FP64 pass over both kinds: the windows needing the whole pipeline
+    // first, then those re-run from the zoom stage on (their coarse results,
+    // SNR and covariance stay the FP32 ones)
+    std::vector<int32_t> idx = full;
+    idx.insert(idx.end(), fine.begin(), fine.end());
+    if (idx.empty()) return;
+    const int m = int(idx.size()), m_full = int(full.size());
+    int32_t *d_idx = buf<int32_t>("p_idx", m);
+    HIP_OK(hipMemcpyAsync(d_idx, idx.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, a.stream));
+    int32_t *st4 = buf<int32_t>("p_starts", size_t(8) * m);
+    const int32_t *srcs[4] = {a.ref_d, a.ref_a, a.sec_d, a.sec_a};
+    for (int k = 0; k < 4; ++k)
+        hipLaunchKernelGGL(k_take_i32, dim3(idivup(m, 256)), dim3(256), 0, a.stream, srcs[k], d_idx, st4 + k * m, m, 1);
+    float *o_off = buf<float>("p_off", size_t(2) * m), *o_snr = buf<float>("p_snr", m);
+    float *o_cov = buf<float>("p_cov", size_t(3) * m), *o_val = buf<float>("p_val", m);
+    int32_t *o_i = buf<int32_t>("p_ints", size_t(8) * m);
+    RunArgs sub = a;
+    sub.ref_d = st4; sub.ref_a = st4 + m; sub.sec_d = st4 + 2 * m; sub.sec_a = st4 + 3 * m;
+    sub.n = m;
+    sub.margin_coarse = sub.margin_fine = nullptr;
+    sub.offset = o_off; sub.snr = o_snr; sub.cov = o_cov; sub.maxval = o_val;
+    sub.maxloc = o_i; sub.start = o_i + 2 * m; sub.shift = o_i + 4 * m; sub.zoom_idx = o_i + 6 * m;
+    sub.chunk = 0;
+    sub.fine_start = sub.fine_shift = nullptr;
+    if (m_full < m) {
+        hipLaunchKernelGGL(k_take_i32, dim3(idivup(2 * m, 256)), dim3(256), 0, a.stream, start32, d_idx, st4 + 4 * m, m, 2);
+        hipLaunchKernelGGL(k_take_i32, dim3(idivup(2 * m, 256)), dim3(256), 0, a.stream, shift32, d_idx, st4 + 6 * m, m, 2);
+        sub.fine_start = st4 + 4 * m;
+        sub.fine_shift = st4 + 6 * m;
+        sub.fine_from = m_full;
+    }
+    run_staged_t<double>(sub);
+    // offsets and zoom peaks of every re-run window; the rest of the full ones only
+    auto put_f = [&](const float *src, float *dst, int cnt, int width) {
+        if (dst && cnt) hipLaunchKernelGGL(k_put<float>, dim3(idivup(cnt * width, 256)), dim3(256), 0, a.stream, src, d_idx, dst, cnt, width);
     };
-    subset(full, false);
-    subset(fine, true);
+    auto put_i = [&](const int32_t *src, int32_t *dst, int cnt) {
+        if (dst && cnt) hipLaunchKernelGGL(k_put<int32_t>, dim3(idivup(cnt * 2, 256)), dim3(256), 0, a.stream, src, d_idx, dst, cnt, 2);
+    };
+    put_f(o_off, a.offset, m, 2);
+    put_i(sub.zoom_idx, a.zoom_idx, m);
+    put_f(o_snr, a.snr, m_full, 1);
+    put_f(o_cov, a.cov, m_full, 3);
+    put_f(o_val, a.maxval, m_full, 1);
+    put_i(sub.maxloc, a.maxloc, m_full);
+    put_i(sub.start, a.start, m_full);
+    put_i(sub.shift, a.shift, m_full);
+    HIP_OK(hipGetLastError());
 }
 
 void Engine::run(const RunArgs &in) {

@@ -108,3 +108,36 @@ def test_default_precision_equals_fp64(eng, field, mode):
     for k in INT_KEYS:
         assert np.array_equal(a[k], b[k]), k
     assert np.array_equal(a["offset"], b["offset"])
+
+
+def test_mixed_fp64_rerun_kinds(eng, field):
+    """Thresholds that send about half of the windows through the whole FP64
+    pipeline and all the others through its fine stage only, in one staged pass
+    whose chunks hold either kind or both: integer results and offsets equal the
+    FP64 pipeline's, and the fine-only windows keep their FP32 SNR."""
+    a0, _ = _run(eng, field, 0, MODE_FUSED)
+    b, _ = _run(eng, field, 2, MODE_STAGED)
+    tc = float(np.median(a0["margin_coarse"]))
+    ref, sec, starts = field
+    dev = torch.device("cuda", 0)
+    eng.setup(WindowParams(window_h=64, window_w=64, hs_down=20, hs_across=20, deramp=1))
+    eng.set_precision(1, tc, float("inf"))
+    try:
+        st = [torch.as_tensor(s.astype(np.int32)).to(dev) for s in starts]
+        out = eng.run_windows(torch.from_numpy(ref).to(dev), torch.from_numpy(sec).to(dev), *st, mode=MODE_FUSED,
+                              debug=True)
+        torch.cuda.synchronize(dev)
+        checked, rerun = eng.precision_stats()
+    finally:
+        eng.set_precision(1, 2e-6, 2e-6)
+    a = {k: v.cpu().numpy() for k, v in out.items()}
+    n = len(a["offset"])
+    fine_only = a0["margin_coarse"] >= tc
+    print(f"\n[mixed re-run] n={n} rerun={rerun} fine_only={int(fine_only.sum())}", flush=True)
+    assert checked == n and rerun == n and 0 < fine_only.sum() < n
+    for k in INT_KEYS:
+        assert np.array_equal(a[k], b[k]), k
+    assert np.array_equal(a["offset"], b["offset"])
+    assert np.array_equal(a["snr"][fine_only], a0["snr"][fine_only])
+    full = ~fine_only
+    np.testing.assert_allclose(a["snr"][full], b["snr"][full], rtol=1e-6)
