@@ -28,10 +28,10 @@ MFLOP_PER_WINDOW = 12.364
 KERNEL_MFLOP = {"k_coarse": 1.235, "k_oversample_ref": 1.434, "k_oversample_search": 1.817, "k_fine": 7.878}
 FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per window of each fused kernel, from rocprofv3 FETCH_SIZE + WRITE_SIZE (KiB per dispatch of a
-# 7812-window launch) scaled by a 1 GiB device copy's counts (FETCH x2, WRITE x1); re-measure after kernel changes
-HBM_BYTES_PER_WINDOW = {k: v * 1024 / 7812 for k, v in {
-    "k_fine": 594726.9 * 2 + 161.6, "k_coarse": 149361.2 * 2 + 384.0,
-    "k_oversample_ref": 122693.8 * 2 + 524416.1, "k_oversample_search": 143811.6 * 2 + 663552.0}.items()}
+# 31248-window launch) scaled by a 1 GiB device copy's counts (FETCH x2, WRITE x1); re-measure after kernel changes
+HBM_BYTES_PER_WINDOW = {k: v * 1024 / 31248 for k, v in {
+    "k_fine": 2267825.9 * 2 + 721.0, "k_coarse": 565839.8 * 2 + 1471.7,
+    "k_oversample_ref": 487912.3 * 2 + 2000363.8, "k_oversample_search": 557867.2 * 2 + 2531088.0}.items()}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -67,6 +67,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the staged-pipeline comparison")
     ap.add_argument("--cpu-windows", type=int, default=4096)
     ap.add_argument("--oracle-windows", type=int, default=384)
+    ap.add_argument("--no-host", action="store_true", help="skip the host-resident-input (PCIe-inclusive) leg")
     ap.add_argument("--no-uneven", action="store_true", help="skip the uneven chip-pair batch leg")
     ap.add_argument("--uneven-pairs", type=int, default=8192)
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
@@ -144,20 +145,9 @@ def main():
 
     # host-resident variant: pinned host images -> HBM -> field -> host outputs, per pass;
     # row bands upload on a copy stream while earlier bands of windows run
-    h_ref = torch.from_numpy(ref_np).pin_memory()
-    h_sec = torch.from_numpy(sec_np).pin_memory()
-    host_starts = (ref_d, ref_a, sec_d, sec_a)
-    host_out = eng.run_windows_from_host(h_ref, h_sec, *host_starts, mode=mode)
-    torch.cuda.synchronize(dev)
-    t1 = time.perf_counter()
-    for _ in range(3):
-        host_out = eng.run_windows_from_host(h_ref, h_sec, *host_starts, mode=mode)
-    t_host = (time.perf_counter() - t1) / 3
-    pcie = {"windows_per_s": round(n / t_host, 1), "ms_per_field": round(1e3 * t_host, 2),
-            "max_abs_offset_diff_vs_device_resident": float(np.abs(host_out["offset"] - gpu_out["offset"]).max()),
-            "note": "pinned host images uploaded to HBM in 8 row bands on a copy stream overlapping the "
-                    "earlier bands' windows, outputs copied back; not the headline value"}
-    del h_ref, h_sec, host_out
+    pcie = None
+    if not args.no_host:
+        pcie = bench_host_inputs(eng, ref_np, sec_np, (ref_d, ref_a, sec_d, sec_a), mode, gpu_out, n)
 
     # roofline of the dominant kernel, from the events recorded inside the timed region
     roof = None
@@ -265,6 +255,24 @@ def main():
         "scene_gen_s": round(gen_s, 1),
     }
     print(json.dumps(line), flush=True)
+
+
+def bench_host_inputs(eng, ref_np, sec_np, host_starts, mode, gpu_out, n):
+    """Pinned host images -> HBM -> field -> host outputs (PCIe-inclusive), 3 timed passes."""
+    import torch
+    dev = torch.device("cuda", eng.device)
+    h_ref = torch.from_numpy(ref_np).pin_memory()
+    h_sec = torch.from_numpy(sec_np).pin_memory()
+    host_out = eng.run_windows_from_host(h_ref, h_sec, *host_starts, mode=mode)
+    torch.cuda.synchronize(dev)
+    t1 = time.perf_counter()
+    for _ in range(3):
+        host_out = eng.run_windows_from_host(h_ref, h_sec, *host_starts, mode=mode)
+    t_host = (time.perf_counter() - t1) / 3
+    return {"windows_per_s": round(n / t_host, 1), "ms_per_field": round(1e3 * t_host, 2),
+            "max_abs_offset_diff_vs_device_resident": float(np.abs(host_out["offset"] - gpu_out["offset"]).max()),
+            "note": "pinned host images uploaded to HBM in 8 row bands on a copy stream overlapping the "
+                    "earlier bands' windows, outputs copied back; not the headline value"}
 
 
 # model MFLOP per chip pair by (window, half search) at the reference's transform sizes

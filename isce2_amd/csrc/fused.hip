@@ -604,7 +604,7 @@ void Engine::run_fused(const RunArgs &a) {
     const int L1 = coarse_L(sz.search_w);
     const int K1 = pick_k(sz.raw_corr_h, NT / (L1 / 2));
     const int mode = p.image_complex ? (p.deramp == 0 ? 1 : 0) : 2;
-    const int chunk = std::min(a.chunk > 0 ? a.chunk : 8192, 65535);
+    const int chunk = std::min(a.chunk > 0 ? a.chunk : 32768, 65535);
     const int nb = std::min(chunk, a.n);
 
     int32_t *loc = a.maxloc ? a.maxloc : buf<int32_t>("f_loc", size_t(a.n) * 2);
