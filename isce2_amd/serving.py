@@ -30,7 +30,7 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from .engine import MODE_AUTO, Engine, WindowParams
+from .engine import MODE_AUTO, MODE_FUSED, Engine, WindowParams
 
 
 @dataclass
@@ -210,6 +210,22 @@ class OffsetServer:
             self.engines[key] = e
         return self.engines[key]
 
+    def _presize(self, eng, mode, chunk):
+        """Size a new class engine's device buffers for its largest launch now
+        (one throw-away pass over windows at the arena origin), so that a later
+        large job does not pay for growing them (a free synchronizes the device)."""
+        if self.arena.ref.shape[0] < eng.sizes["search_h"] or self.arena.pitch < eng.sizes["search_w"]:
+            return
+        fused = mode == MODE_FUSED or (mode == MODE_AUTO and eng.sizes["fused"])
+        m = min(self.max_batch, chunk if chunk > 0 else (self.max_batch if fused else 1024))
+        z = torch.zeros(m, dtype=torch.int32, device=self.dev)
+        prec = eng.precision
+        eng.set_precision(0)
+        try:
+            eng.run_windows(self.arena.ref, self.arena.sec, z, z, z, z, mode=mode, chunk=chunk)
+        finally:
+            eng.set_precision(prec)
+
     def _admit(self):
         while True:
             try:
@@ -280,6 +296,7 @@ class OffsetServer:
                     self.schedules[key] = (sch.mode, sch.chunk)
                 else:
                     self.schedules[key] = (MODE_AUTO, 0)
+                self._presize(eng, *self.schedules[key])
             mode, chunk = self.schedules[key]
             out = eng.run_windows(self.arena.ref, self.arena.sec, rd, ra, sd, sa, mode=mode, chunk=chunk)
             pos = 0
