@@ -79,7 +79,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this engine runs on one GPU per process; bench.py measures N=1 only "
-                                   "(windows are independent, see the sharding notes in DESIGN.md)",
+                                   "(windows are independent: each GPU would take a band of window rows plus its image halo)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
