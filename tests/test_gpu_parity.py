@@ -134,6 +134,19 @@ def test_fused_matches_oracle(eng, scene1, W, hs, deramp):
     _assert_parity(c)
 
 
+@pytest.mark.parametrize("W,hd,ha,deramp", [(64, 12, 24, 1), (64, 24, 12, 0), (32, 20, 8, 1)])
+def test_fused_unequal_half_search(eng, scene1, W, hd, ha, deramp):
+    """Different half search ranges down and across (search chip and raw
+    surface not square) through the fused kernels."""
+    ref, sec = scene1
+    cfg = O.AmpcorConfig(window_h=W, window_w=W, hs_down=hd, hs_across=ha, deramp=deramp)
+    rd, ra, sd, sa, _, _ = O.start_pixels(4, 4, hd, ha, 100, 100, 0, 0, hd, ha)
+    g, o = run_both(eng, ref, sec, cfg, (rd, ra, sd, sa), MODE_FUSED, chunk=6)
+    c = compare(g, o)
+    _report(f"fused W={W} hs={hd}x{ha} deramp={deramp}", c)
+    _assert_parity(c)
+
+
 def test_fused_real_image_and_edges(eng, scene1):
     ref, sec = scene1
     ref_r = np.abs(ref).astype(np.float32)
