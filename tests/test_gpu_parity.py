@@ -252,3 +252,29 @@ def test_host_inputs_banded_equals_device_resident(eng, scene1, pin):
     bad[-1] = ref.shape[0]             # a search chip past the last row
     with pytest.raises(IndexError):
         eng.run_windows_from_host(h_ref, h_sec, rd, ra, bad, sa, bands=2)
+
+
+@pytest.mark.parametrize("mode", [MODE_FUSED, MODE_STAGED])
+def test_pitched_images_of_different_sizes(eng, scene1, mode):
+    """Reference and secondary of different shapes, the secondary a column view
+    of a wider image (row pitch 512 > width 470): chips are read through the pitch."""
+    ref, sec = scene1
+    sec_small = np.ascontiguousarray(sec[10:490, 20:490])          # 480 x 470
+    cfg = O.AmpcorConfig(hs_down=16, hs_across=16)
+    rd, ra, _, _, _, _ = O.start_pixels(4, 4, 40, 40, 96, 96, 0, 0, 16, 16)
+    sd, sa = rd - 10 - 16, ra - 20 - 16                              # same scene points in the crop
+    from .gpu_helpers import params_from_cfg
+    eng.setup(params_from_cfg(cfg))
+    dev = torch.device("cuda", 0)
+    wide = torch.zeros((480, 512), dtype=torch.complex64, device=dev)
+    wide[:, :470] = torch.from_numpy(sec_small).to(dev)
+    sec_view = wide[:, :470]
+    assert sec_view.stride(0) == 512
+    st = [torch.as_tensor(np.asarray(s, np.int32)).to(dev) for s in (rd, ra, sd, sa)]
+    g = eng.run_windows(torch.from_numpy(ref).to(dev), sec_view, *st, mode=mode, debug=True)
+    torch.cuda.synchronize()
+    g = {k: v.cpu().numpy() for k, v in g.items()}
+    o = O.run_windows(ref, sec_small, rd, ra, sd, sa, cfg)
+    c = compare(g, o)
+    _report(f"pitched mode={mode}", c)
+    _assert_parity(c)
