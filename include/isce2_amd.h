@@ -99,7 +99,8 @@ int ampcor_setup(ampcor_ctx *ctx, const ampcor_params *params, ampcor_sizes *siz
  *                     chip and secondary search chip (cuAmpcorParameter.cpp:155-205).
  *   out_offset : float[n][2] (down, across); out_snr : float[n];
  *   out_cov : float[n][3] (cov_down, cov_across, cov_cross).
- *   chunk : windows per staged batch (0 = engine default); mode : AMPCOR_MODE_*.
+ *   chunk : windows per launch (0 = engine default: 1024 for the staged
+ *           pipeline, 32768 for the fused kernels); mode : AMPCOR_MODE_*.
  *   stream : hipStream_t (NULL = default stream). The call is asynchronous
  *   with respect to the host; synchronize the stream before reading outputs. */
 int ampcor_run_windows(ampcor_ctx *ctx,
