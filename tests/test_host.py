@@ -154,3 +154,21 @@ def test_fft_factorizations_multiply_to_length():
         for r in orders:
             assert int(np.prod(r)) == n and set(r) <= {2, 3, 4, 5, 7, 8, 11, 13, 16, 17}
     assert factorizations(2 * 19) == []
+
+
+def test_fill_rows_banded_copy_from_memmap(tmp_path):
+    """The host staging copy of the banded uploader: any row range of a
+    memory-mapped raster lands intact, on one thread or as row bands on a pool."""
+    from concurrent.futures import ThreadPoolExecutor
+    from isce2_amd.engine import fill_rows
+    rng = np.random.default_rng(3)
+    a = (rng.standard_normal((3001, 1400)) + 1j * rng.standard_normal((3001, 1400))).astype(np.complex64)
+    path = tmp_path / "img.c64"
+    a.tofile(path)
+    mm = np.memmap(path, dtype=np.complex64, mode="r", shape=a.shape)
+    with ThreadPoolExecutor(5) as pool:
+        for r0, r1, p, nthr in ((0, 3001, pool, 5), (17, 2990, pool, 5), (5, 9, pool, 5), (0, 3001, None, 1)):
+            dst = np.zeros_like(a)
+            fill_rows(dst, mm, r0, r1, p, nthr)
+            assert np.array_equal(dst[r0:r1], a[r0:r1])
+            assert not dst[:r0].any() and not dst[r1:].any()
