@@ -70,3 +70,35 @@ def test_random_config_matches_oracle(eng, scene, seed):
         assert c["zoom_mismatch"] == 0 and c["offset_over_tol"] == 0, c
         assert c["snr_max_rel"] < 1e-3, c
         assert c["cov_sentinel_mismatch"] == 0 and c["cov_max_rel"] < 1e-3, c
+
+
+EXOTIC = [dict(window_h=48, window_w=40, hs_down=10, hs_across=12, zoom=8),
+          dict(window_h=48, window_w=48, hs_down=12, hs_across=10, zoom=32),
+          dict(window_h=40, window_w=56, hs_down=9, hs_across=7, corr_stat=9),
+          dict(window_h=64, window_w=64, hs_down=16, hs_across=16, surf_ovs=8),
+          dict(window_h=32, window_w=32, hs_down=8, hs_across=8, raw_ovs=4),
+          dict(window_h=36, window_w=44, hs_down=11, hs_across=9, surf_method=1),
+          dict(window_h=128, window_w=128, hs_down=8, hs_across=10)]
+
+
+@pytest.mark.parametrize("kw", EXOTIC, ids=[",".join(f"{k}={v}" for k, v in d.items()) for d in EXOTIC])
+def test_non_default_settings_match_oracle(eng, scene, kw):
+    """Zoom windows of 8 and 32 (a non-16 zoom takes the general FFT surface
+    path), a smaller SNR patch, surface oversampling 8, raw oversampling 4, the
+    sinc surface on unequal sides and a 128 window, through the staged pipeline."""
+    ref, sec = scene
+    cfg = O.AmpcorConfig(**kw)
+    wh, ww, hd, ha = cfg.window_h, cfg.window_w, cfg.hs_down, cfg.hs_across
+    H, W = ref.shape
+    kd, ka = wh // 2 + 8, ww // 2 + 8
+    nd = min(4, (H - wh - 2 * hd - 8) // kd + 1)
+    na = min(4, (W - ww - 2 * ha - 8) // ka + 1)
+    rd, ra, sd, sa, _, _ = O.start_pixels(nd, na, hd + 3, ha + 5, kd, ka, 0, 0, hd, ha)
+    g, o = run_both(eng, ref, sec, cfg, (rd, ra, sd, sa), MODE_STAGED, chunk=7)
+    c = compare(g, o)
+    print(f"\n[settings {kw}] " + " ".join(f"{k}={v}" for k, v in c.items()), flush=True)
+    assert c["maxloc_mismatch"] == 0, c
+    assert c["start_mismatch"] == 0 and c["shift_mismatch"] == 0, c
+    assert c["zoom_mismatch"] == 0 and c["offset_over_tol"] == 0, c
+    assert c["snr_max_rel"] < 1e-3, c
+    assert c["cov_sentinel_mismatch"] == 0 and c["cov_max_rel"] < 1e-3, c
