@@ -190,7 +190,7 @@ struct OsArgs {
 __host__ __device__ inline int os_lds_bytes(int N) { return al16(N * (N + 1) * 8 + N * 8) + 2 * N * 16 + 32 * 8; }
 
 template <int R1, int R2, bool MEAN>
-__global__ void __launch_bounds__(NT) k_oversample(OsArgs a, const cf *tw, int w0) {
+__global__ void __launch_bounds__(NT, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversample(OsArgs a, const cf *tw, int w0) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int N = R1 * R2, PC = N + 1;
     static_assert(2 * N <= NT, "one thread per deramp row and column factor");
