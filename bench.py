@@ -324,6 +324,10 @@ def bench_serving(dev, ref_np, sec_np, n_jobs, rate, seed=5):
     from isce2_amd.autotune import Autotuner
     from isce2_amd.serving import OffsetJob, OffsetServer
     H, Wd = ref_np.shape
+    # the server's 200 GB arena plus its class engines take most of the 288 GB:
+    # hand the blocks earlier legs left in torch's cache back to the device first
+    torch.cuda.synchronize(dev)
+    torch.cuda.empty_cache()
     pool_ref = torch.from_numpy(ref_np).pin_memory()
     pool_sec = torch.from_numpy(sec_np).pin_memory()
     rng = np.random.default_rng(seed)
