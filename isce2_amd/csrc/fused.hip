@@ -168,8 +168,8 @@ __global__ void __launch_bounds__(NT, (K >= 9 ? 2 : 4)) k_coarse(CoarseArgs a, c
 }
 
 // ---------------------------------------------------------------------------
-// x2 chip oversampling (cuOverSampler.cpp:72-78, cuArraysPadding.cu:223-266,
-// cuDeramp.cu:343-390): per axis, odd samples = (1/N) FFT_-(FFT_+(x) e^{-i pi k_s/N}),
+// x2 chip oversampling (cuOverSampler.cpp:72-78, cuArraysPadding.cu:59-103,
+// cuDeramp.cu:67-148): per axis, odd samples = (1/N) FFT_-(FFT_+(x) e^{-i pi k_s/N}),
 // even samples = x. Output pixel (2r+p, 2c+q) comes from:
 //   p=0,q=0: X;  p=1,q=0: columns of X interpolated;
 //   p=0,q=1: Y = rows of X interpolated;  p=1,q=1: columns of Y interpolated.

@@ -68,7 +68,7 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
 
 // ---------------------------------------------------------------------------
 // FP64 window sums -> nrm[u][v] = 1/sqrt((s2 - s1^2/n) t2 + FLT_EPSILON)
-// (cuCorrNormalization.cu:402-423, cuCorrNormalizationSAT.cu:237-240).
+// (cuCorrNormalization.cu:230-314, cuCorrNormalizationSAT.cu:237-240).
 // Vertical sums of s and s^2 over the first TH rows of every column come from
 // NB row blocks summed by all threads at once (`part`: 2*NB*SW doubles of
 // scratch); each wave then slides them down to its first lag row, and walks

@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(512) k_fft_cols(cx<T> *data, int w, int c, Fft
 }
 
 // Inverse 2-D FFT of the quadrant padding of h x w spectra into H x W
-// (cuArraysPadding.cu:223-266 followed by the oversampler's second transform,
+// (cuArraysPadding.cu:59-103 followed by the oversampler's second transform,
 // cuOverSampler.cpp:72-78 and :141-148), in two passes that never store the
 // padding. Row pass: each of the 2*(h/2) non-zero padded rows is built in LDS
 // from the small spectrum (scaled by 1/(h*w), zeros between the quadrants) and
@@ -185,7 +185,7 @@ __device__ __forceinline__ double block_sum256(double v, double *sh) {
     return s;
 }
 
-// mean subtraction per image, FP64 sum (cuCorrNormalization.cu:210-235)
+// mean subtraction per image, FP64 sum (cuCorrNormalization.cu:98-139)
 template <class T>
 __global__ void __launch_bounds__(1024) k_sub_mean(T *data, int size, double *ssd) {
     __shared__ double sh[16];
@@ -356,7 +356,7 @@ __global__ void k_crop_real(const cx<T> *z, int Lh, int Lw, T *out, int oh, int 
 }
 
 // ---------------------------------------------------------------------------
-// normalization (formula of cuCorrNormalization.cu:402-423 and
+// normalization (formula of cuCorrNormalization.cu:230-314 and
 // cuCorrNormalizationSAT.cu:237-240) with FP64 window sums computed directly:
 // per-column vertical sums over the template height, slid down one lag row at
 // a time, and horizontal window sums from a block prefix scan of them. One
@@ -547,7 +547,7 @@ __global__ void __launch_bounds__(256) k_maxloc(const T *surf, int h, int w, int
 
 // ---------------------------------------------------------------------------
 // covariance, SNR, zoom start/shift; one thread per window
-// (cuEstimateStats.cu:20-110, cuArraysCopy.cu:249-284, cuCorrNormalization.cu:255-285,
+// (cuEstimateStats.cu:20-110, cuArraysCopy.cu:249-284, cuCorrNormalization.cu:143-192,
 //  cuOffset.cu:190-241)
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void adjust_axis(int maxloc, int old_range, int new_range, int &start, int &shift) {
@@ -625,7 +625,7 @@ __global__ void __launch_bounds__(64) k_stats(const T *corr, int nx, int ny, int
 }
 
 // ---------------------------------------------------------------------------
-// deramp method 1 (cuDeramp.cu:343-390) with float2.h's operator*= (imaginary
+// deramp method 1 (cuDeramp.cu:67-148) with float2.h's operator*= (imaginary
 // part reuses the updated real part); sums, phases and rotation in FP64. The
 // phase is separable, c*phx + r*phy: k_deramp_phase writes one FP64 table per
 // chip, e^{i r phy} for its h rows then e^{i c phx} for its w columns, and the

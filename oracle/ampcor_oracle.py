@@ -24,7 +24,7 @@ SNR                        cuArraysCopy.cu:249-284, cuCorrNormalization.cu:143-1
                            cuEstimateStats.cu:20-29
 zoom start / shift         cuOffset.cu:190-241
 deramp (method 1)          cuDeramp.cu:67-148 with float2.h:71-75
-chip oversampling (C2C)    cuOverSampler.cpp:72-78, cuArraysPadding.cu:222-266
+chip oversampling (C2C)    cuOverSampler.cpp:72-78, cuArraysPadding.cu:59-103
 surface oversampling FFT   cuOverSampler.cpp:141-148, cuArraysCopy.cu:326-339
 surface oversampling sinc  cuSincOverSampler.cu:21-195, cuSincOverSampler.h:27-57
 sub-pixel offset           cuOffset.cu:137-187
@@ -197,7 +197,7 @@ def amplitude(chips):
 
 
 def subtract_mean(images):
-    """Per-image mean removal (cuCorrNormalization.cu:210-235)."""
+    """Per-image mean removal (cuCorrNormalization.cu:98-139)."""
     return images - images.mean(axis=(-2, -1), keepdims=True)
 
 
@@ -240,7 +240,7 @@ def window_sums(searches, th, tw):
 
 
 def normalize(corr, templates, searches):
-    """NCC normalization (cuCorrNormalization.cu:342-426; same formula as the SAT
+    """NCC normalization (cuCorrNormalization.cu:230-314; same formula as the SAT
     path cuCorrNormalizationSAT.cu:237-240)."""
     th, tw = templates.shape[1:]
     t_sum2 = np.einsum("nij,nij->n", templates, templates)
@@ -322,8 +322,8 @@ def estimate_variance(surf, loc, peak, template_size):
 def estimate_snr(surf, loc, peak, patch_h, patch_w):
     """SNR from the ``patch_h x patch_w`` patch centred on the peak
     (cuArraysCopy.cu:249-284 with ``-size/2`` centring and zero-fill,
-    cuCorrNormalization.cu:255-285 sum of squares and valid count,
-    cuEstimateStats.cu:284-294 ``max^2 / ((sum - max^2)/(count-1))``)."""
+    cuCorrNormalization.cu:143-192 sum of squares and valid count,
+    cuEstimateStats.cu:20-46 ``max^2 / ((sum - max^2)/(count-1))``)."""
     n, nx, ny = surf.shape
     ii = loc[:, 0:1] + np.arange(patch_h)[None, :] - patch_h // 2
     jj = loc[:, 1:2] + np.arange(patch_w)[None, :] - patch_w // 2
@@ -363,7 +363,7 @@ def subpixel_offset(start, zoom_idx, surf_ovs, raw_ovs, hs_down, hs_across):
 # deramp and FFT oversampling
 # ---------------------------------------------------------------------------
 def deramp_method1(chips):
-    """Deramp method 1 (cuDeramp.cu:343-390) in FP64.
+    """Deramp method 1 (cuDeramp.cu:67-148) in FP64.
 
     ``phaseAcross = arg sum z[r,c] conj(z[r,c+1])``, ``phaseDown = arg sum
     z[r,c] conj(z[r+1,c])``; each pixel is multiplied by ``exp(i*phase)`` with
@@ -387,7 +387,7 @@ def deramp_method1(chips):
 
 
 def _pad_quadrants(spec, out_h, out_w):
-    """Quadrant padding (cuArraysPadding.cu:223-266): the ``h//2 x w//2`` corners
+    """Quadrant padding (cuArraysPadding.cu:59-103): the ``h//2 x w//2`` corners
     of the spectrum move to the output corners, scaled by ``1/(h*w)``."""
     n, h, w = spec.shape
     h2, w2 = h // 2, w // 2

@@ -269,6 +269,83 @@ def test_sinc_oversample_window_and_identity():
     assert np.abs(out[0, 128, 128] - surf[0, 8, 8]) < 0.2
 
 
+def _tone16(f, g, c1, c2):
+    i = np.arange(16)
+    return np.cos(2 * np.pi * f * (i[:, None] - c1)) * np.cos(2 * np.pi * g * (i[None, :] - c2))
+
+
+@pytest.mark.parametrize("covs,factor,shift,f,g", [
+    (16, 32, (0, 0), 2 / 16, 3 / 16),
+    (16, 32, (3, 5), 2 / 16, 3 / 16),     # shifted window that wraps past the 256 edge
+    (16, 16, (7, 0), 1 / 16, 2 / 16),
+    (8, 16, (2, 1), 3 / 16, 1 / 16),
+    (8, 8, (0, 6), 2 / 16, 2 / 16),
+])
+def test_sinc_oversample_band_limited_tone(covs, factor, shift, f, g):
+    """Interpolation meaning: output sample o is the input signal at position
+    o/covs. A separable tone periodic over the 16 samples (so the circular tap
+    wrap is consistent) must come back at every fractional phase of the window,
+    within the 8-tap windowed-sinc passband error (0.011 at 1/16 cycle/sample).
+    A reversed tap direction or a mirrored fractional phase misses by ~0.4-1.1."""
+    c1, c2 = 0.3, -1.7
+    out_n = 16 * covs
+    s = _tone16(f, g, c1, c2)[None]
+    out = O.oversample_sinc(s, out_n, covs, np.array([shift]), factor)
+    rng_ = 2 * covs
+    start = out_n // 2 - rng_
+    ox = (np.arange(2 * rng_ + 1) + start + shift[0] * factor) % out_n
+    oy = (np.arange(2 * rng_ + 1) + start + shift[1] * factor) % out_n
+    truth = (np.cos(2 * np.pi * f * (ox[:, None] / covs - c1))
+             * np.cos(2 * np.pi * g * (oy[None, :] / covs - c2)))
+    assert np.abs(out[0][np.ix_(ox, oy)] - truth).max() < 0.03
+    # everything outside the (4 covs + 1)^2 patch stays zero
+    mask = np.ones((out_n, out_n), bool)
+    mask[np.ix_(ox, oy)] = False
+    assert not out[0][mask].any()
+
+
+def _sinc_point_fp32(surf, outx, outy, covs, table):
+    """One output of the reference's interpolation kernel, scalar and in FP32:
+    r_xout = outx/covs, i_xfrac = int(frac*4096), taps inx = i_xout - i + 4 with
+    the wrap (x wrapped by the input width), weights normalized by their sum
+    (cuSincOverSampler.cu:108-157)."""
+    f32 = np.float32
+    n = surf.shape[0]
+    rx = f32(outx) / f32(covs)
+    ix = int(rx)
+    fx = int((rx - f32(ix)) * f32(4096))
+    ry = f32(outy) / f32(covs)
+    iy = int(ry)
+    fy = int((ry - f32(iy)) * f32(4096))
+    acc, wsum = f32(0), f32(0)
+    for i in range(8):
+        inx = ix - i + 4
+        inx = inx + n if inx < 0 else inx
+        inx = inx - n if inx >= n else inx
+        cx = f32(table[i * 4096 + fx])
+        for j in range(8):
+            iny = iy - j + 4
+            iny = iny + n if iny < 0 else iny
+            iny = iny - n if iny >= n else iny
+            c = cx * f32(table[j * 4096 + fy])
+            wsum += c
+            acc += f32(surf[inx, iny]) * c
+    return float(acc / wsum)
+
+
+def test_sinc_oversample_matches_index_formulas():
+    surf = rng.random((1, 16, 16))
+    table = O.sinc_filter_table()[0]
+    shift = np.array([[2, 4]])
+    out = O.oversample_sinc(surf, 256, 16, shift, 32)
+    for ox, oy in [(96 + 64, 96 + 128), (100 + 64, 131 + 128 - 256), (160 + 64, 97 + 128 - 256),
+                   (123 + 64, 111 + 128)]:
+        ox %= 256
+        oy %= 256
+        ref = _sinc_point_fp32(surf[0], ox, oy, 16, table)
+        assert out[0, ox, oy] == pytest.approx(ref, rel=2e-6, abs=2e-6)
+
+
 # ---------------------------------------------------------------------------
 # start pixels, range check, whole pipeline
 # ---------------------------------------------------------------------------
