@@ -169,8 +169,40 @@ int ampcor_profile_report(ampcor_ctx *ctx, char *buf, int64_t len);
  * so that a timed region carries the fewest event records. */
 int ampcor_profile_only(ampcor_ctx *ctx, const char *name);
 
-/* Text of the last error on this context (or of the last ampcor_create failure
- * when ctx is NULL). */
+/* Window start pixels of a regular grid, in HOST memory (no device is used).
+ * Replaces cuAmpcorParameter::setStartPixels, the constant- and varying-gross
+ * overloads (contrib/PyCuAmpcor/src/cuAmpcorParameter.cpp:167-205).
+ * Window i = row * n_across + col:
+ *   ref_d[i] = start_down0 + row * skip_down,  ref_a[i] = start_across0 + col * skip_across,
+ *   sec_d[i] = ref_d[i] + gross_d - hs_down,   sec_a[i] = ref_a[i] + gross_a - hs_across,
+ * with gross_d/gross_a = gross_down[i]/gross_across[i] when those int32[n] arrays
+ * are given (setVaryingGrossOffset), else gross_down0/gross_across0
+ * (setConstantGrossOffset). Outputs are int32[n_down * n_across]. The half
+ * search ranges come from params (each axis its own, see ampcor_setup). */
+int ampcor_start_pixels(const ampcor_params *params, int32_t n_down, int32_t n_across,
+                        int32_t start_down0, int32_t start_across0, int32_t skip_down, int32_t skip_across,
+                        int32_t gross_down0, int32_t gross_across0,
+                        const int32_t *gross_down, const int32_t *gross_across,
+                        int32_t *ref_d, int32_t *ref_a, int32_t *sec_d, int32_t *sec_a);
+
+/* The reference's range check on HOST start arrays (no device is used).
+ * Replaces cuAmpcorParameter::checkPixelInImageRange
+ * (contrib/PyCuAmpcor/src/cuAmpcorParameter.cpp:275-331), with its strict end
+ * test: a window fails when start < 0 or start + chip size >= image size (so a
+ * chip ending on the image's last row or column fails too). Returns AMPCOR_OK,
+ * or AMPCOR_E_RANGE with *bad_window = the first failing window and *which =
+ * the first failing test: 0/1 reference start down/across, 2/3 reference end
+ * down/across, 4/5 secondary start down/across, 6/7 secondary end down/across
+ * (the reference's order); ampcor_last_error(NULL) names it. Where the reference
+ * exits the process, this returns. */
+int ampcor_check_range(const ampcor_params *params, const int32_t *ref_d, const int32_t *ref_a,
+                       const int32_t *sec_d, const int32_t *sec_a, int32_t n,
+                       int64_t ref_h, int64_t ref_w, int64_t sec_h, int64_t sec_w,
+                       int32_t *bad_window, int32_t *which);
+
+/* Text of the last error on this context (or, when ctx is NULL, of the last
+ * failure of a call that takes no context: ampcor_create, ampcor_start_pixels,
+ * ampcor_check_range). */
 const char *ampcor_last_error(const ampcor_ctx *ctx);
 
 /* Library version string. */

@@ -24,13 +24,14 @@ Extension attributes (not in the reference): ``engineMode`` ("auto",
 """
 from __future__ import annotations
 
+import ctypes as C
 import operator
 import time
 
 import numpy as np
 import torch
 
-from . import raster
+from . import _lib, raster
 from .engine import MODE_AUTO, MODE_FUSED, MODE_STAGED, Engine, WindowParams
 
 __all__ = ["PyCuAmpcor"]
@@ -56,6 +57,10 @@ _STR_PROPS = {
 }
 _ALIASES = {"grossOffsetDownStatic": 0, "grossOffsetAcrossStatic": 0}
 _MODES = {"auto": MODE_AUTO, "staged": MODE_STAGED, "fused": MODE_FUSED}
+
+
+def _cptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
 
 
 class PyCuAmpcor:
@@ -135,19 +140,30 @@ class PyCuAmpcor:
             raise RuntimeError("setupParams() must be called first")
 
     def _set_starts(self, g_down, g_across):
-        """cuAmpcorParameter::setStartPixels (cuAmpcorParameter.cpp:170-205)."""
+        """cuAmpcorParameter::setStartPixels (cuAmpcorParameter.cpp:167-205),
+        through the C ABI's ``ampcor_start_pixels``."""
         v = self._v
         nd, na = v["numberWindowDown"], v["numberWindowAcross"]
-        rows, cols = np.meshgrid(np.arange(nd), np.arange(na), indexing="ij")
-        ref_d = (v["referenceStartPixelDownStatic"] + rows * v["skipSampleDown"]).ravel()
-        ref_a = (v["referenceStartPixelAcrossStatic"] + cols * v["skipSampleAcross"]).ravel()
-        gd = np.broadcast_to(np.asarray(g_down, np.int64), ref_d.shape).astype(np.int64)
-        ga = np.broadcast_to(np.asarray(g_across, np.int64), ref_a.shape).astype(np.int64)
+        n = nd * na
+        lib = _lib.load()
+        p = self._window_params().to_c()
+        out = [np.empty(n, np.int32) for _ in range(4)]
+        varying = np.ndim(g_down) > 0
+        if varying:
+            gd = np.ascontiguousarray(np.broadcast_to(np.asarray(g_down), (n,)), np.int32)
+            ga = np.ascontiguousarray(np.broadcast_to(np.asarray(g_across), (n,)), np.int32)
+            g0 = (0, 0)
+        else:
+            gd = ga = None
+            g0 = (int(g_down), int(g_across))
+        code = lib.ampcor_start_pixels(C.byref(p), nd, na, v["referenceStartPixelDownStatic"],
+                                       v["referenceStartPixelAcrossStatic"], v["skipSampleDown"],
+                                       v["skipSampleAcross"], g0[0], g0[1], _cptr(gd), _cptr(ga), *map(_cptr, out))
+        _lib.check(code, None)
         st = self._state
-        st["ref_d"], st["ref_a"] = ref_d.astype(np.int64), ref_a.astype(np.int64)
-        st["gross_d"], st["gross_a"] = gd, ga
-        st["sec_d"] = ref_d + gd - v["halfSearchRangeDown"]
-        st["sec_a"] = ref_a + ga - v["halfSearchRangeAcross"]
+        st["ref_d"], st["ref_a"], st["sec_d"], st["sec_a"] = (a.astype(np.int64) for a in out)
+        st["gross_d"] = gd.astype(np.int64) if varying else np.full(n, g0[0], np.int64)
+        st["gross_a"] = ga.astype(np.int64) if varying else np.full(n, g0[1], np.int64)
 
     def setConstantGrossOffset(self, goDown, goAcross):
         self._need_setup()
@@ -163,33 +179,34 @@ class PyCuAmpcor:
         self._set_starts(vD[:n], vA[:n])
 
     def checkPixelInImageRange(self):
-        """cuAmpcorParameter::checkPixelInImageRange (cuAmpcorParameter.cpp:275-331):
-        strict end test (``start + size >= dim`` fails). Raises ``IndexError``."""
+        """cuAmpcorParameter::checkPixelInImageRange (cuAmpcorParameter.cpp:275-331),
+        through the C ABI's ``ampcor_check_range``: strict end test
+        (``start + size >= dim`` fails). Raises ``IndexError`` naming the first
+        failing window (row, col), the test and the offending pixel."""
         st, v = self._state, self._v
         if "ref_d" not in st:
             raise RuntimeError("set a gross offset first")
         sz = st["sizes"]
-        tests = [
-            (st["ref_d"] < 0, "Reference Window start pixel out ot range in Down", st["ref_d"]),
-            (st["ref_a"] < 0, "Reference Window start pixel out ot range in Across", st["ref_a"]),
-            (st["ref_d"] + v["windowSizeHeight"] >= v["referenceImageHeight"],
-             "Reference Window end pixel out ot range in Down", st["ref_d"] + v["windowSizeHeight"]),
-            (st["ref_a"] + v["windowSizeWidth"] >= v["referenceImageWidth"],
-             "Reference Window end pixel out ot range in Across", st["ref_a"] + v["windowSizeWidth"]),
-            (st["sec_d"] < 0, "Secondary Window start pixel out ot range in Down", st["sec_d"]),
-            (st["sec_a"] < 0, "Secondary Window start pixel out ot range in Across", st["sec_a"]),
-            (st["sec_d"] + sz["search_h"] >= v["secondaryImageHeight"],
-             "Secondary Window end pixel out ot range in Down", st["sec_d"] + sz["search_h"]),
-            (st["sec_a"] + sz["search_w"] >= v["secondaryImageWidth"],
-             "Secondary Window end pixel out ot range in Across", st["sec_a"] + sz["search_w"]),
-        ]
-        bad = np.stack([t[0] for t in tests])
-        idx = np.flatnonzero(bad.any(axis=0))
-        if idx.size:
-            i = int(idx[0])
-            k = int(np.argmax(bad[:, i]))
-            row, col = divmod(i, v["numberWindowAcross"])
-            raise IndexError(f"{tests[k][1]}, window ({row},{col}), pixel {int(tests[k][2][i])}")
+        arrs = [np.ascontiguousarray(st[k], np.int32) for k in ("ref_d", "ref_a", "sec_d", "sec_a")]
+        p = self._window_params().to_c()
+        bad, which = C.c_int32(), C.c_int32()
+        lib = _lib.load()
+        code = lib.ampcor_check_range(C.byref(p), *map(_cptr, arrs), len(arrs[0]), v["referenceImageHeight"],
+                                      v["referenceImageWidth"], v["secondaryImageHeight"], v["secondaryImageWidth"],
+                                      C.byref(bad), C.byref(which))
+        if code == _lib.AMPCOR_OK:
+            return
+        if code != -2:
+            _lib.check(code, None)
+        i, k = bad.value, which.value
+        start = int(arrs[k // 4 * 2 + (k % 2)][i])
+        size = ((v["windowSizeHeight"], v["windowSizeWidth"]), (sz["search_h"], sz["search_w"]))[k // 4][k % 2]
+        pixel = start if (k % 4) < 2 else start + size
+        kind = ("Reference", "Secondary")[k // 4]
+        edge = ("start", "end")[(k % 4) // 2]
+        axis = ("Down", "Across")[k % 2]
+        row, col = divmod(i, v["numberWindowAcross"])
+        raise IndexError(f"{kind} Window {edge} pixel out of range in {axis}, window ({row},{col}), pixel {pixel}")
 
     # -- run -----------------------------------------------------------------------
     def _engine(self, dev_index):

@@ -25,7 +25,8 @@ FLAG_NOSYNC = 0x100
 EXPORTS = ("ampcor_create", "ampcor_destroy", "ampcor_setup", "ampcor_run_windows",
            "ampcor_run_windows_debug", "ampcor_set_precision", "ampcor_precision_stats",
            "ampcor_profile", "ampcor_profile_report", "ampcor_last_error", "ampcor_version",
-           "ampcor_set_fft_radices", "ampcor_set_option", "ampcor_profile_only")
+           "ampcor_set_fft_radices", "ampcor_set_option", "ampcor_profile_only",
+           "ampcor_start_pixels", "ampcor_check_range")
 
 
 class Params(C.Structure):
@@ -71,13 +72,17 @@ def load() -> C.CDLL:
     lib.ampcor_profile_only.argtypes = [vp, C.c_char_p]
     lib.ampcor_profile.argtypes = [vp, i32]
     lib.ampcor_profile_report.argtypes = [vp, C.c_char_p, i64]
+    lib.ampcor_start_pixels.argtypes = [C.POINTER(Params), i32, i32, i32, i32, i32, i32, i32, i32,
+                                        vp, vp, vp, vp, vp, vp]
+    lib.ampcor_check_range.argtypes = [C.POINTER(Params), vp, vp, vp, vp, i32, i64, i64, i64, i64,
+                                       C.POINTER(i32), C.POINTER(i32)]
     lib.ampcor_last_error.argtypes = [vp]
     lib.ampcor_last_error.restype = C.c_char_p
     lib.ampcor_version.restype = C.c_char_p
     for name in ("ampcor_create", "ampcor_destroy", "ampcor_setup", "ampcor_run_windows",
                  "ampcor_run_windows_debug", "ampcor_profile", "ampcor_profile_report",
                  "ampcor_set_precision", "ampcor_precision_stats", "ampcor_set_fft_radices",
-                 "ampcor_set_option", "ampcor_profile_only"):
+                 "ampcor_set_option", "ampcor_profile_only", "ampcor_start_pixels", "ampcor_check_range"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib

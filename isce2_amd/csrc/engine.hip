@@ -883,4 +883,65 @@ const char *ampcor_last_error(const ampcor_ctx *ctx) {
 
 const char *ampcor_version(void) { return "isce2_amd 0.1.0 (gfx950)"; }
 
+// cuAmpcorParameter::setStartPixels, grid overloads (cuAmpcorParameter.cpp:167-205):
+// window i = row * n_across + col; each search axis uses its own half range.
+int ampcor_start_pixels(const ampcor_params *params, int32_t n_down, int32_t n_across, int32_t start_down0,
+                        int32_t start_across0, int32_t skip_down, int32_t skip_across, int32_t gross_down0,
+                        int32_t gross_across0, const int32_t *gross_down, const int32_t *gross_across,
+                        int32_t *ref_d, int32_t *ref_a, int32_t *sec_d, int32_t *sec_a) {
+    return guarded(nullptr, [&] {
+        if (!params || !ref_d || !ref_a || !sec_d || !sec_a)
+            throw ampcor::Error(AMPCOR_E_ARG, "null pointer argument");
+        if (n_down <= 0 || n_across <= 0 || int64_t(n_down) * n_across > INT32_MAX)
+            throw ampcor::Error(AMPCOR_E_ARG, "window grid must be positive and below 2^31 windows");
+        if ((gross_down == nullptr) != (gross_across == nullptr))
+            throw ampcor::Error(AMPCOR_E_ARG, "give both varying gross offset arrays or neither");
+        for (int64_t row = 0; row < n_down; ++row)
+            for (int64_t col = 0; col < n_across; ++col) {
+                const int64_t i = row * n_across + col;
+                const int64_t rd = start_down0 + row * skip_down, ra = start_across0 + col * skip_across;
+                const int64_t gd = gross_down ? gross_down[i] : gross_down0;
+                const int64_t ga = gross_across ? gross_across[i] : gross_across0;
+                const int64_t sd = rd + gd - params->hs_down, sa = ra + ga - params->hs_across;
+                for (int64_t v : {rd, ra, sd, sa})
+                    if (v < INT32_MIN || v > INT32_MAX)
+                        throw ampcor::Error(AMPCOR_E_ARG, "start pixel of window " + std::to_string(i) +
+                                                              " overflows int32");
+                ref_d[i] = int32_t(rd); ref_a[i] = int32_t(ra);
+                sec_d[i] = int32_t(sd); sec_a[i] = int32_t(sa);
+            }
+    });
+}
+
+// cuAmpcorParameter::checkPixelInImageRange (cuAmpcorParameter.cpp:275-331), the
+// reference's strict rule: start >= 0 and start + chip size < image size.
+int ampcor_check_range(const ampcor_params *params, const int32_t *ref_d, const int32_t *ref_a,
+                       const int32_t *sec_d, const int32_t *sec_a, int32_t n, int64_t ref_h, int64_t ref_w,
+                       int64_t sec_h, int64_t sec_w, int32_t *bad_window, int32_t *which) {
+    if (bad_window) *bad_window = -1;
+    if (which) *which = -1;
+    return guarded(nullptr, [&] {
+        if (!params || (n > 0 && (!ref_d || !ref_a || !sec_d || !sec_a)))
+            throw ampcor::Error(AMPCOR_E_ARG, "null pointer argument");
+        if (n < 0) throw ampcor::Error(AMPCOR_E_ARG, "negative window count");
+        const int64_t wh = params->window_h, ww = params->window_w;
+        const int64_t sh = wh + 2 * int64_t(params->hs_down), sw = ww + 2 * int64_t(params->hs_across);
+        static const char *what[8] = {
+            "reference window start pixel out of range in down", "reference window start pixel out of range in across",
+            "reference window end pixel out of range in down", "reference window end pixel out of range in across",
+            "secondary window start pixel out of range in down", "secondary window start pixel out of range in across",
+            "secondary window end pixel out of range in down", "secondary window end pixel out of range in across"};
+        for (int32_t i = 0; i < n; ++i) {
+            const bool fail[8] = {ref_d[i] < 0, ref_a[i] < 0, ref_d[i] + wh >= ref_h, ref_a[i] + ww >= ref_w,
+                                  sec_d[i] < 0, sec_a[i] < 0, sec_d[i] + sh >= sec_h, sec_a[i] + sw >= sec_w};
+            for (int k = 0; k < 8; ++k)
+                if (fail[k]) {
+                    if (bad_window) *bad_window = i;
+                    if (which) *which = k;
+                    throw ampcor::Error(AMPCOR_E_RANGE, std::string(what[k]) + ", window " + std::to_string(i));
+                }
+        }
+    });
+}
+
 }  // extern "C"

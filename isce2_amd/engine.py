@@ -146,9 +146,11 @@ class Engine:
         kernels. Windows in row order give the most overlap. The engine's
         bounds rule is checked on the host first (IndexError), so the groups
         run without host synchronization; under precision 1 the near-tie
-        windows of all groups are then re-run once through the FP64 pipeline
-        (offsets as ``run_windows`` gives them; SNR and covariance of those
-        windows from the FP64 run). Returns host arrays ``offset [n,2]``,
+        windows of all groups are then re-run once through the FP64 pipeline,
+        with the rule ``run_windows`` applies: every re-run window takes the
+        FP64 offset, and SNR and covariance come from the FP64 run only where
+        the raw-surface margin was small (so both methods return the same
+        outputs). Returns host arrays ``offset [n,2]``,
         ``snr [n]``, ``cov [n,3]``.
         """
         if self.params is None:
@@ -233,7 +235,8 @@ class Engine:
                                  nosync=True, out={k: v[i0:i1] for k, v in out.items()})
         if rerun:
             mc, mf = out.pop("margin_coarse").cpu().numpy(), out.pop("margin_fine").cpu().numpy()
-            sel = np.nonzero(~(mc >= self.tau[0]) | ~(mf >= self.tau[1]))[0]
+            coarse = ~(mc >= self.tau[0])
+            sel = np.nonzero(coarse | ~(mf >= self.tau[1]))[0]
             if len(sel):
                 t = torch.as_tensor(sel, dtype=torch.int64).to(dev)
                 self.set_precision(2)
@@ -242,8 +245,15 @@ class Engine:
                                          mode=MODE_STAGED)
                 finally:
                     self.set_precision(1)
-                for k in ("offset", "snr", "cov"):
-                    out[k].index_copy_(0, t, o[k])
+                # the same rule as the engine's own re-run: a window whose raw-surface
+                # argmax is safe keeps its FP32 SNR and covariance (its raw peak cannot
+                # change) and takes only the FP64 offset; the others take every output
+                out["offset"].index_copy_(0, t, o["offset"])
+                keep = torch.as_tensor(coarse[sel], device=dev)
+                for k in ("snr", "cov"):
+                    cur = out[k].index_select(0, t)
+                    mask = keep.view(-1, *([1] * (cur.dim() - 1)))
+                    out[k].index_copy_(0, t, torch.where(mask, o[k], cur))
         return {k: v.cpu().numpy() for k, v in out.items()}
 
     def set_precision(self, mode: int, tau_coarse: float = -1.0, tau_fine: float = -1.0):

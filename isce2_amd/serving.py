@@ -162,6 +162,8 @@ class OffsetServer:
         self.active = []
         self.stats = dict(batches=0, windows=0, jobs=0)
         self._stop = threading.Event()
+        self._dead = None           # the exception that ended the scheduler thread
+        self._lock = threading.Lock()
         self._thread = None
         self._next_id = 0
         self.tuner = tuner
@@ -170,11 +172,14 @@ class OffsetServer:
     # -- client side ------------------------------------------------------
     def submit(self, job: OffsetJob) -> Future:
         fut = Future()
-        job.arrival = time.perf_counter()
-        if job.job_id < 0:
-            job.job_id = self._next_id
-            self._next_id += 1
-        self.inbox.put((job, fut))
+        with self._lock:
+            if self._dead is not None:
+                raise RuntimeError(f"offset server stopped after an error: {self._dead!r}")
+            job.arrival = time.perf_counter()
+            if job.job_id < 0:
+                job.job_id = self._next_id
+                self._next_id += 1
+            self.inbox.put((job, fut))
         return fut
 
     def start(self):
@@ -205,8 +210,12 @@ class OffsetServer:
         if key not in self.engines:
             W, hs, deramp = key
             e = Engine(self.device)
-            e.setup(WindowParams(window_h=W, window_w=W, hs_down=hs, hs_across=hs, deramp=deramp))
-            e.set_precision(self.precision)
+            try:
+                e.setup(WindowParams(window_h=W, window_w=W, hs_down=hs, hs_across=hs, deramp=deramp))
+                e.set_precision(self.precision)
+            except Exception:
+                e.close()
+                raise
             self.engines[key] = e
         return self.engines[key]
 
@@ -235,36 +244,82 @@ class OffsetServer:
         while self.waiting:
             job, fut = self.waiting[0]
             try:
-                nd, na, *starts = job_windows(job)
-                rows = int(job.height or job.ref.shape[0])
-                width = int(job.width or job.ref.shape[1])
-                if width > self.pitch or job.window + 2 * job.half_search > self.pitch:
-                    raise ValueError(f"job {job.job_id}: width {width} exceeds the arena pitch {self.pitch}")
+                nd, na, *starts = self._validate(job)
             except Exception as exc:   # bad request: fail it, keep serving
                 self.waiting.pop(0)
                 fut.set_exception(exc)
                 continue
+            rows = int(job.height or job.ref.shape[0])
             need = self._footprint(job, nd * na)
-            if self.used + need > self.budget:
+            if self.used + need > self.budget or need > self.budget:
+                if need > self.budget:      # can never fit: fail it rather than block the queue
+                    self.waiting.pop(0)
+                    fut.set_exception(MemoryError(f"job {job.job_id}: footprint {need / 1e9:.2f} GB exceeds "
+                                                  f"the HBM budget {self.budget / 1e9:.2f} GB"))
+                    continue
                 return
+            if rows > self.arena.rows:
+                self.waiting.pop(0)
+                fut.set_exception(MemoryError(f"job {job.job_id}: {rows} rows exceed the image arena's "
+                                              f"{self.arena.rows}"))
+                continue
             row0 = self.arena.alloc(rows)
             if row0 is None:
                 return
             self.waiting.pop(0)
             self.used += need
-            a = _Active(job, fut, row0, starts, nd, na, self.device)
-            a.footprint = need
-            with torch.cuda.stream(self.copy_stream):
-                for src, dst in ((job.ref, self.arena.ref), (job.sec, self.arena.sec)):
-                    t = src if isinstance(src, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(src))
-                    if t.shape[1] == self.pitch:
-                        dst[row0:row0 + rows].copy_(t[:rows], non_blocking=True)
-                    else:
-                        dst[row0:row0 + rows, :t.shape[1]].copy_(t[:rows], non_blocking=True)
-                ev = torch.cuda.Event()
-                ev.record(self.copy_stream)
+            try:
+                a = _Active(job, fut, row0, starts, nd, na, self.device)
+                a.footprint = need
+                with torch.cuda.stream(self.copy_stream):
+                    for src, dst in ((job.ref, self.arena.ref), (job.sec, self.arena.sec)):
+                        t = self._as_tensor(src)
+                        if t.shape[1] == self.pitch:
+                            dst[row0:row0 + rows].copy_(t[:rows], non_blocking=True)
+                        else:
+                            dst[row0:row0 + rows, :t.shape[1]].copy_(t[:rows], non_blocking=True)
+                    ev = torch.cuda.Event()
+                    ev.record(self.copy_stream)
+            except Exception as exc:   # the upload failed: give its rows and budget back
+                self.copy_stream.synchronize()
+                self.arena.release(row0, rows)
+                self.used -= need
+                fut.set_exception(exc)
+                continue
             a.loaded = ev
             self.active.append(a)
+
+    @staticmethod
+    def _as_tensor(src):
+        return src if isinstance(src, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(src))
+
+    def _validate(self, job):
+        """Everything that can be wrong with a request, checked before it takes
+        arena rows or budget: window placement, array shapes and dtypes, and the
+        window class (its engine is built here, so a class the engine cannot run
+        fails this request only)."""
+        if int(job.window) <= 0 or int(job.half_search) <= 0:
+            raise ValueError(f"job {job.job_id}: window {job.window} and half search {job.half_search} "
+                             "must be positive")
+        if job.skip_down <= 0 or job.skip_across <= 0:
+            raise ValueError(f"job {job.job_id}: skips must be positive")
+        nd, na, *starts = job_windows(job)
+        rows = int(job.height or job.ref.shape[0])
+        width = int(job.width or job.ref.shape[1])
+        for name, arr in (("ref", job.ref), ("sec", job.sec)):
+            shape = tuple(arr.shape)
+            dtype = arr.dtype
+            ok_dtype = dtype == torch.complex64 if isinstance(arr, torch.Tensor) else dtype == np.complex64
+            if len(shape) != 2 or not ok_dtype:
+                raise ValueError(f"job {job.job_id}: {name} must be a 2-D complex64 array, got {dtype} {shape}")
+            if shape[0] < rows or shape[1] < width:
+                raise ValueError(f"job {job.job_id}: {name} {shape} is smaller than the declared {rows}x{width}")
+            if shape[1] > self.pitch:
+                raise ValueError(f"job {job.job_id}: {name} rows of {shape[1]} exceed the arena pitch {self.pitch}")
+        if job.window + 2 * job.half_search > self.pitch:
+            raise ValueError(f"job {job.job_id}: search chip wider than the arena pitch {self.pitch}")
+        self._engine((job.window, job.half_search, job.deramp))
+        return (nd, na, *starts)
 
     def _pick_batch(self):
         ready = [a for a in self.active if a.cursor < a.n and a.loaded.query()]
@@ -327,14 +382,49 @@ class OffsetServer:
             self.stats["jobs"] += 1
             a.fut.set_result(res)
 
-    def _loop(self):
-        torch.cuda.set_device(self.dev)
-        while not self._stop.is_set() or self.active or self.waiting or not self.inbox.empty():
+    def _drop(self, a, exc):
+        """Fail one admitted job and give back its arena rows and budget."""
+        if a in self.active:
+            self.active.remove(a)
+            self.arena.release(a.row0, int(a.job.height or a.job.ref.shape[0]))
+            self.used -= a.footprint
+        if not a.fut.done():
+            a.fut.set_exception(exc)
+
+    def _fail_all(self, exc):
+        """The scheduler cannot go on: resolve every future it holds or will
+        receive, so that no client waits forever, and refuse new requests."""
+        with self._lock:
+            self._dead = exc
+        for a in list(self.active):
+            if not a.fut.done():
+                a.fut.set_exception(exc)
+        self.active.clear()
+        pending = list(self.waiting)
+        self.waiting.clear()
+        while True:
             try:
+                pending.append(self.inbox.get_nowait())
+            except queue.Empty:
+                break
+        for _, fut in pending:
+            if not fut.done():
+                fut.set_exception(exc)
+
+    def _loop(self):
+        try:
+            torch.cuda.set_device(self.dev)
+            while not self._stop.is_set() or self.active or self.waiting or not self.inbox.empty():
                 self._admit()
                 key, parts = self._pick_batch()
                 if parts:
-                    self._run_batch(key, parts)
+                    try:
+                        self._run_batch(key, parts)
+                    except Exception as exc:    # this launch failed: fail its jobs, keep serving the rest
+                        self.compute_stream.synchronize()
+                        for a, _, _ in parts:
+                            self._drop(a, exc)
+                        continue
                     self._finish()
                 elif self.active or self.waiting:
                     time.sleep(0.0002)
@@ -344,9 +434,6 @@ class OffsetServer:
                         self.waiting.append(item)
                     except queue.Empty:
                         pass
-            except Exception as exc:    # fail every in-flight job rather than hang its client
-                for a in self.active:
-                    if not a.fut.done():
-                        a.fut.set_exception(exc)
-                self.active.clear()
-                raise
+        except BaseException as exc:    # fail every job rather than hang its client
+            self._fail_all(exc)
+            raise

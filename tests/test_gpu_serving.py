@@ -66,6 +66,54 @@ def test_alone_equals_mixed_equals_direct(scene):
         assert np.array_equal(r["offset"], d[0]), m
 
 
+def test_bad_requests_fail_alone(scene):
+    """Malformed requests fail their own future at admission; good requests
+    submitted around them complete and match a direct engine call, every
+    arena row and budget byte comes back, and the server keeps accepting work."""
+    good = dict(r0=100, c0=60, h=600, w=700, W=64, hs=16, skip=64)
+    bad = []
+    j = _job(scene, **good)
+    j.window = 0                                   # no window
+    bad.append(j)
+    j = _job(scene, **good)
+    j.half_search = 0                              # no search range
+    bad.append(j)
+    j = _job(scene, **good)
+    j.window, j.half_search = 600, 40              # the window grid does not fit the image
+    bad.append(j)
+    j = _job(scene, **good)
+    j.ref = np.zeros((600, 1100), np.complex64)    # rows wider than the arena pitch
+    j.sec = np.zeros((600, 1100), np.complex64)
+    j.width = 700
+    bad.append(j)
+    j = _job(scene, **good)
+    j.ref = j.ref.real.copy()                      # not complex64
+    bad.append(j)
+    j = _job(scene, **good)
+    j.skip_down = 0                                # no window grid
+    bad.append(j)
+    j = _job(scene, **good)
+    j.n_down = 50                                  # explicit grid running off the image
+    bad.append(j)
+    with OffsetServer(device=0, hbm_budget=2e9, pitch=1024) as srv:
+        futs = []
+        for b in bad:
+            futs.append(srv.submit(_job(scene, **good)))
+            futs.append(srv.submit(b))
+        res = []
+        for i, f in enumerate(futs):
+            if i % 2 == 0:
+                res.append(f.result(timeout=120))
+            else:
+                with pytest.raises(Exception):
+                    f.result(timeout=120)
+        later = srv.submit(_job(scene, **good)).result(timeout=120)
+        assert srv.used < 1.0 and srv.arena.free_rows() == srv.arena.rows
+    d_off, d_snr, _ = _direct(_job(scene, **good))
+    for r in res + [later]:
+        assert np.array_equal(r["offset"], d_off) and np.array_equal(r["snr"], d_snr)
+
+
 def test_admission_waits_for_budget(scene):
     jobs = [dict(r0=0, c0=0, h=600, w=600, W=64, hs=16, skip=64) for _ in range(5)]
     # budget for about two jobs at a time: 2 images x 600 rows x 1024 pitch x 8 B + margin

@@ -94,8 +94,76 @@ def test_check_range_strict_rule():
     p.setConstantGrossOffset(0, 0)
     p.checkPixelInImageRange()                      # 20+64+104-20 = 168 < 200
     p.referenceImageWidth = 148                     # 84 + 64 = 148 >= 148 -> strict failure
-    with pytest.raises(IndexError, match=r"Reference Window end pixel out ot range in Across, window \(0,1\)"):
+    with pytest.raises(IndexError, match=r"Reference Window end pixel out of range in Across, window \(0,1\), "
+                                         r"pixel 148"):
         p.checkPixelInImageRange()
+
+
+def _c_params(wh, ww, hd, ha):
+    return _lib.Params(window_h=wh, window_w=ww, hs_down=hd, hs_across=ha, raw_ovs=2, zoom=16, surf_ovs=16,
+                       surf_method=0, corr_stat=21, deramp=1, algorithm=0, image_complex=1)
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def test_c_abi_start_pixels_match_oracle():
+    """ampcor_start_pixels (host memory, no device) against the oracle's
+    setStartPixels restatement, for constant and varying gross offsets."""
+    lib = _lib.load()
+    rng = np.random.default_rng(7)
+    for nd, na, s0d, s0a, kd, ka, hd, ha in [(3, 5, 11, 7, 40, 30, 3, 3), (7, 2, 0, 5, 17, 64, 20, 9),
+                                              (1, 1, 100, 3, 1, 1, 16, 16)]:
+        n = nd * na
+        p = _c_params(64, 48, hd, ha)
+        out = [np.full(n, -7, np.int32) for _ in range(4)]
+        assert lib.ampcor_start_pixels(ctypes.byref(p), nd, na, s0d, s0a, kd, ka, 2, -1, None, None,
+                                       *map(_p, out)) == 0
+        ref = O.start_pixels(nd, na, s0d, s0a, kd, ka, 2, -1, hd, ha)
+        for a, b in zip(out, ref[:4]):
+            np.testing.assert_array_equal(a, b)
+        gd = rng.integers(-30, 30, n).astype(np.int32)
+        ga = rng.integers(-30, 30, n).astype(np.int32)
+        assert lib.ampcor_start_pixels(ctypes.byref(p), nd, na, s0d, s0a, kd, ka, 0, 0, _p(gd), _p(ga),
+                                       *map(_p, out)) == 0
+        ref = O.start_pixels(nd, na, s0d, s0a, kd, ka, gd, ga, hd, ha)
+        for a, b in zip(out, ref[:4]):
+            np.testing.assert_array_equal(a, b)
+    # argument errors come back as codes
+    assert lib.ampcor_start_pixels(ctypes.byref(p), 0, 3, 0, 0, 1, 1, 0, 0, None, None, *map(_p, out)) == -1
+    assert lib.ampcor_start_pixels(ctypes.byref(p), 1, 1, 0, 0, 1, 1, 0, 0, _p(gd), None, *map(_p, out)) == -1
+    assert b"gross" in lib.ampcor_last_error(None)
+
+
+def test_c_abi_check_range_every_test_at_its_edge():
+    """ampcor_check_range against the oracle's strict rule: each of the eight
+    tests, placed exactly at its boundary (passing) and one pixel beyond
+    (failing), on a later window so that the reported index is checked too."""
+    lib = _lib.load()
+    wh, ww, hd, ha = 32, 40, 6, 9
+    p = _c_params(wh, ww, hd, ha)
+    cfg = O.AmpcorConfig(window_h=wh, window_w=ww, hs_down=hd, hs_across=ha)
+    H, Wd = 300, 310
+    base = [np.full(5, v, np.int32) for v in (50, 60, 44, 51)]
+    # (array, value that just passes, value that just fails) for tests 0..7
+    cases = [(0, 0, -1), (1, 0, -1), (0, H - wh - 1, H - wh), (1, Wd - ww - 1, Wd - ww),
+             (2, 0, -1), (3, 0, -1), (2, H - (wh + 2 * hd) - 1, H - (wh + 2 * hd)),
+             (3, Wd - (ww + 2 * ha) - 1, Wd - (ww + 2 * ha))]
+    bad, which = ctypes.c_int32(), ctypes.c_int32()
+    for k, (arr, ok_v, bad_v) in enumerate(cases):
+        for v, expect in ((ok_v, 0), (bad_v, -2)):
+            a = [b.copy() for b in base]
+            a[arr][3] = v
+            code = lib.ampcor_check_range(ctypes.byref(p), *map(_p, a), 5, H, Wd, H, Wd,
+                                          ctypes.byref(bad), ctypes.byref(which))
+            o = O.check_range(*[x.astype(np.int64) for x in a], cfg, (H, Wd), (H, Wd))
+            assert code == expect, (k, v)
+            if expect == 0:
+                assert o is None and bad.value == -1
+            else:
+                assert (bad.value, which.value) == (3, k) and o[0] == 3
+                assert lib.ampcor_last_error(None).decode().endswith("window 3")
 
 
 def test_raster_readers_roundtrip(tmp_path):
