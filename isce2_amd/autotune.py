@@ -99,7 +99,11 @@ class Autotuner:
 
     @staticmethod
     def key(params: WindowParams) -> str:
-        return f"W{params.window_h}x{params.window_w}_hs{params.hs_down}x{params.hs_across}_d{params.deramp}"
+        """Every field of the configuration: a schedule tuned for one surface
+        method, zoom or oversampling factor is not reused for another."""
+        p = params
+        return (f"W{p.window_h}x{p.window_w}_hs{p.hs_down}x{p.hs_across}_d{p.deramp}_ro{p.raw_ovs}_z{p.zoom}"
+                f"_so{p.surf_ovs}_sm{p.surf_method}_cs{p.corr_stat}_a{p.algorithm}_c{p.image_complex}")
 
     def _time(self, eng, args, mode, chunk, n):
         dev = torch.device("cuda", self.device)
@@ -117,9 +121,11 @@ class Autotuner:
         """Choose a schedule for ``params`` from ``args`` = (ref, sec, ref_d, ref_a, sec_d, sec_a)
         (device tensors holding at least a sample of real windows of this class)."""
         k = self.key(params)
-        if k in self.cache:
-            return self.cache[k]
         sizes = eng.setup(params)
+        if k in self.cache:
+            if self.cache[k].mode != MODE_FUSED or sizes["fused"]:
+                return self.cache[k]
+            del self.cache[k]           # a fused schedule for a configuration the fused kernels do not cover
         m = min(n, self.sample)
         sub = list(args[:2]) + [t[:m].contiguous() for t in args[2:]]
         prev = eng.precision

@@ -45,72 +45,75 @@ GEOMETRY_BASES = ("lat", "lon", "los", "hgt", "z", "shadowMask", "waterMask")
 
 def create_parser() -> argparse.ArgumentParser:
     """Argument surface of cuDenseOffsets.createParser (cuDenseOffsets.py:34-146)."""
-    p = argparse.ArgumentParser(description="Generate offset field between two SLCs",
+    p = argparse.ArgumentParser(description="Dense pixel offsets between a reference and a secondary SLC "
+                                            "on an MI355X GPU",
                                 formatter_class=argparse.RawTextHelpFormatter, epilog=EXAMPLE)
-    p.add_argument("-r", "--reference", dest="reference", required=True, help="Reference image")
-    p.add_argument("-s", "--secondary", dest="secondary", required=True, help="Secondary image")
+    p.add_argument("-r", "--reference", dest="reference", required=True, help="reference SLC file")
+    p.add_argument("-s", "--secondary", dest="secondary", required=True, help="secondary SLC file")
     p.add_argument("--fix-xml", "--fix-image-xml", dest="fixImageXml", action="store_true",
-                   help="Rewrite the image .xml with the image's absolute path.")
+                   help="rewrite each input's .xml sidecar with absolute file paths")
     p.add_argument("--fix-vrt", "--fix-image-vrt", dest="fixImageVrt", action="store_true",
-                   help="Rewrite the image .vrt from its .xml.")
+                   help="regenerate each input's .vrt sidecar from its .xml")
     p.add_argument("--op", "--outprefix", "--output-prefix", dest="outprefix", default="offset", required=True,
-                   help="Output prefix (default: %(default)s).")
-    p.add_argument("--os", "--outsuffix", dest="outsuffix", default="", help="Output suffix (default: %(default)s).")
-    p.add_argument("--ww", type=int, dest="winwidth", default=64, help="Window width (default: %(default)s).")
-    p.add_argument("--wh", type=int, dest="winhgt", default=64, help="Window height (default: %(default)s).")
+                   help="path prefix of every output file [%(default)s]")
+    p.add_argument("--os", "--outsuffix", dest="outsuffix", default="", help="text appended to the prefix [%(default)s]")
+    p.add_argument("--ww", type=int, dest="winwidth", default=64, help="columns per reference window [%(default)s]")
+    p.add_argument("--wh", type=int, dest="winhgt", default=64, help="rows per reference window [%(default)s]")
     p.add_argument("--sw", type=int, dest="srcwidth", default=20,
-                   help="Half search range along width (default: %(default)s).")
+                   help="search half-width across, in pixels [%(default)s]")
     p.add_argument("--sh", type=int, dest="srchgt", default=20,
-                   help="Half search range along height (default: %(default)s).")
-    p.add_argument("--kw", type=int, dest="skipwidth", default=64, help="Skip across (default: %(default)s).")
-    p.add_argument("--kh", type=int, dest="skiphgt", default=64, help="Skip down (default: %(default)s).")
-    p.add_argument("--mm", type=int, dest="margin", default=0, help="Margin (default: %(default)s).")
+                   help="search half-height down, in pixels [%(default)s]")
+    p.add_argument("--kw", type=int, dest="skipwidth", default=64, help="column step between windows [%(default)s]")
+    p.add_argument("--kh", type=int, dest="skiphgt", default=64, help="row step between windows [%(default)s]")
+    p.add_argument("--mm", type=int, dest="margin", default=0, help="pixels left unused at each image edge [%(default)s]")
     p.add_argument("--nwa", type=int, dest="numWinAcross", default=-1,
-                   help="Number of windows across (default: %(default)s = from the image size).")
+                   help="window columns; -1 fits as many as the image allows [%(default)s]")
     p.add_argument("--nwd", type=int, dest="numWinDown", default=-1,
-                   help="Number of windows down (default: %(default)s = from the image size).")
+                   help="window rows; -1 fits as many as the image allows [%(default)s]")
     p.add_argument("--startpixelac", dest="startpixelac", type=int, default=-1,
-                   help="First reference pixel across (default: %(default)s = margin + search range).")
+                   help="column of the first reference window; -1 = margin plus search half-width [%(default)s]")
     p.add_argument("--startpixeldw", dest="startpixeldw", type=int, default=-1,
-                   help="First reference pixel down (default: %(default)s = margin + search range).")
+                   help="row of the first reference window; -1 = margin plus search half-height [%(default)s]")
     p.add_argument("--alg", "--algorithm", dest="algorithm", type=int, default=0,
-                   help="cross-correlation algorithm (0 = frequency domain, 1 = time domain).")
+                   help="correlation method: 0 spectral, 1 direct sums (same result)")
     p.add_argument("--raw-osf", "--raw-over-samp-factor", type=int, dest="raw_oversample", default=2,
-                   choices=range(2, 5), help="anti-aliasing oversampling factor (default: %(default)s).")
+                   choices=range(2, 5), help="chip interpolation factor before the fine search [%(default)s]")
     p.add_argument("--drmp", "--deramp", dest="deramp", type=int, default=0,
-                   help="deramp method (0: magnitude, 1: complex with linear ramp) (default: %(default)s).")
-    g = p.add_argument_group("Initial gross offset")
-    g.add_argument("-g", "--gross", type=int, dest="gross", default=0, help="Use varying gross offset or not")
-    g.add_argument("--aa", type=int, dest="azshift", default=0, help="Gross azimuth offset (default: %(default)s).")
-    g.add_argument("--rr", type=int, dest="rgshift", default=0, help="Gross range offset (default: %(default)s).")
-    g.add_argument("--gf", "--gross-file", dest="gross_offset_file", help="Varying gross offset input file")
+                   help="chip preparation: 0 amplitude, 1 remove the linear phase ramp [%(default)s]")
+    g = p.add_argument_group("Gross offsets", "a constant or per-window prior shift of the search area")
+    g.add_argument("-g", "--gross", type=int, dest="gross", default=0,
+                   help="1: read per-window gross offsets from --gf [%(default)s]")
+    g.add_argument("--aa", type=int, dest="azshift", default=0, help="constant gross shift down, pixels [%(default)s]")
+    g.add_argument("--rr", type=int, dest="rgshift", default=0, help="constant gross shift across, pixels [%(default)s]")
+    g.add_argument("--gf", "--gross-file", dest="gross_offset_file",
+                   help="int32 (down, across) pairs, one per window, row-major")
     g.add_argument("--mg", "--merge-gross-offset", type=int, dest="merge_gross_offset", default=0,
-                   help="Add the gross offset to the output offsets (default: %(default)s).")
-    c = p.add_argument_group("Correlation surface")
+                   help="1: write offsets with the gross shift added back [%(default)s]")
+    c = p.add_argument_group("Correlation surface", "refinement of the coarse peak")
     c.add_argument("--corr-stat-size", type=int, dest="corr_stat_win_size", default=21,
-                   help="Zoom-in window size for snr/variance statistics (default: %(default)s).")
+                   help="side of the surface patch used for SNR and covariance [%(default)s]")
     c.add_argument("--corr-srch-size", type=int, dest="corr_srch_size", default=4,
-                   help="(half) zoom-in window size of the surface to oversample (default: %(default)s).")
+                   help="half side, in raw pixels, of the surface patch that is refined [%(default)s]")
     c.add_argument("--corr-osf", "--oo", "--corr-over-samp-factor", type=int, dest="corr_oversample", default=32,
-                   help="Oversampling factor of the zoom-in correlation surface (default: %(default)s).")
+                   help="interpolation factor of the refined surface patch [%(default)s]")
     c.add_argument("--corr-osm", "--corr-over-samp-method", type=int, dest="corr_oversamplemethod", default=0,
-                   help="Surface oversampling method 0=fft, 1=sinc (default: %(default)s).")
-    geo = p.add_argument_group("Geometry", "generate corresponding geometry datasets ")
-    geo.add_argument("--full-geom", dest="full_geometry_dir", help="(Input) full-resolution geometry directory.")
-    geo.add_argument("--out-geom", dest="out_geometry_dir", help="(Output) geometry directory on the offset grid.")
-    proc = p.add_argument_group("Processing parameters")
-    proc.add_argument("--gpuid", "--gid", "--gpu-id", dest="gpuid", type=int, default=0, help="GPU ID.")
-    proc.add_argument("--nstreams", dest="nstreams", type=int, default=2, help="Number of streams (kept for "
-                      "compatibility; the engine schedules its own launches).")
-    proc.add_argument("--usemmap", dest="usemmap", type=int, default=1, help="Memory-map the images.")
-    proc.add_argument("--mmapsize", dest="mmapsize", type=int, default=8, help="Memory map buffer size in GB.")
+                   help="surface interpolation: 0 spectral zero padding, 1 windowed sinc [%(default)s]")
+    geo = p.add_argument_group("Geometry", "resample full-resolution geometry rasters onto the offset grid")
+    geo.add_argument("--full-geom", dest="full_geometry_dir", help="directory holding the full-resolution *.rdr files")
+    geo.add_argument("--out-geom", dest="out_geometry_dir", help="directory for the resampled geometry files")
+    proc = p.add_argument_group("Execution")
+    proc.add_argument("--gpuid", "--gid", "--gpu-id", dest="gpuid", type=int, default=0, help="HIP device index")
+    proc.add_argument("--nstreams", dest="nstreams", type=int, default=2, help="accepted and ignored: the engine "
+                      "schedules its own launches")
+    proc.add_argument("--usemmap", dest="usemmap", type=int, default=1, help="read inputs through memory maps")
+    proc.add_argument("--mmapsize", dest="mmapsize", type=int, default=8, help="accepted and ignored (GB)")
     proc.add_argument("--nwac", type=int, dest="numWinAcrossInChunk", default=10,
-                      help="Number of windows across in a chunk (default: %(default)s).")
+                      help="window columns per staged launch group [%(default)s]")
     proc.add_argument("--nwdc", type=int, dest="numWinDownInChunk", default=1,
-                      help="Number of windows down in a chunk (default: %(default)s).")
-    proc.add_argument("--redo", dest="redo", action="store_true", help="Redo even if the offset file exists.")
+                      help="window rows per staged launch group [%(default)s]")
+    proc.add_argument("--redo", dest="redo", action="store_true", help="recompute when the offset file exists")
     proc.add_argument("--engine", dest="engine", default="auto", choices=["auto", "fused", "staged"],
-                      help="Engine path (extension; default: %(default)s).")
+                      help="kernel path (extension) [%(default)s]")
     return p
 
 
@@ -141,8 +144,8 @@ def read_gross_file(path: str, nwd: int, nwa: int):
     g = np.fromfile(path, dtype=np.int32)
     n = nwd * nwa
     if g.size != 2 * n:
-        raise ValueError(f"the input gross offsets do not match the number of windows: {nwd} by {nwa} in int32 "
-                         f"type ({g.size} values, expected {2 * n})")
+        raise ValueError(f"gross-offset file {path} holds {g.size} int32 values; a {nwd} x {nwa} window grid "
+                         f"needs {2 * n}, so its size does not match the grid")
     g = g.reshape(n, 2)
     return np.ascontiguousarray(g[:, 0]), np.ascontiguousarray(g[:, 1])
 
@@ -229,7 +232,7 @@ def estimate_offset_field(reference: str, secondary: str, inps):
                 x_step=obj.skipSampleAcross, y_step=obj.skipSampleDown,
                 x_win_num=obj.numberWindowAcross, y_win_num=obj.numberWindowDown)
     if not inps.redo and os.path.exists(names["offsetImageName"]):
-        print(f"offset field file: {names['offsetImageName']} exists and w/o redo, skip re-estimation.")
+        print(f"{names['offsetImageName']} is already there; pass --redo to compute it again")
         return obj, geom
     obj.runAmpcor()
     nwa, nwd = obj.numberWindowAcross, obj.numberWindowDown
@@ -300,7 +303,7 @@ def prepare_geometry(full_dir, out_dir, x_start, y_start, x_step, y_step, x_win_
     exts = [".rdr.full", ".rdr"] if full_dir != out_dir else [".rdr.full"]
     first = [e for e in exts if os.path.isfile(os.path.join(full_dir, fbases[0] + e))]
     if not first:
-        raise ValueError(f"No full resolution {fbases[0]}.rdr* file found in: {full_dir}")
+        raise ValueError(f"{full_dir} holds no {fbases[0]}.rdr.full or {fbases[0]}.rdr geometry raster")
     ext = first[0]
     os.makedirs(out_dir, exist_ok=True)
     written = []
@@ -326,7 +329,7 @@ def main(argv=None):
     if inps.full_geometry_dir and inps.out_geometry_dir:
         prepare_geometry(inps.full_geometry_dir, inps.out_geometry_dir, **geom)
     m, s = divmod(time.time() - t0, 60)
-    print(f"time used: {m:02.0f} mins {s:02.1f} secs.")
+    print(f"done in {int(m)} min {s:.1f} s")
     return 0
 
 

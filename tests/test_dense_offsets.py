@@ -53,7 +53,7 @@ def test_gross_file_roundtrip_and_size_check(tmp_path):
     assert np.array_equal(raw[:, 0], d) and np.array_equal(raw[:, 1], a)
     gd, ga = D.read_gross_file(f, 3, 4)
     assert np.array_equal(gd, d) and np.array_equal(ga, a)
-    with pytest.raises(ValueError, match="do not match"):
+    with pytest.raises(ValueError, match="does not match the grid"):
         D.read_gross_file(f, 4, 4)
 
 
@@ -94,7 +94,7 @@ def test_prepare_geometry_multiband(tmp_path):
         assert np.array_equal(got_los[b], los[b][np.ix_(rows, cols)])
     hdr = (out / "los.rdr.hdr").read_text()
     assert "bands = 2" in hdr and "interleave = bsq" in hdr and "data type = 4" in hdr
-    with pytest.raises(ValueError, match="No full resolution"):
+    with pytest.raises(ValueError, match="geometry raster"):
         D.prepare_geometry(str(tmp_path / "empty"), str(out), 0, 0, 1, 1, 1, 1)
 
 

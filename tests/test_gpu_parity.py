@@ -254,6 +254,40 @@ def test_host_inputs_banded_equals_device_resident(eng, scene1, pin):
         eng.run_windows_from_host(h_ref, h_sec, rd, ra, bad, sa, bands=2)
 
 
+@pytest.mark.parametrize("taus", [(2e-6, float("inf")), (float("inf"), float("inf"))])
+def test_host_inputs_fp64_rerun_of_every_window(eng, scene1, taus):
+    """The host-input runner's FP64 re-run, forced for every window: from the
+    zoom stage (every fine margin below tau, raw-surface margins at the default
+    threshold) and whole-pipeline. Its outputs equal
+    the device-resident call's (which re-runs inside the engine) and the FP64
+    oracle's integer peaks and offsets."""
+    from .gpu_helpers import params_from_cfg
+    ref, sec = scene1
+    cfg = O.AmpcorConfig(hs_down=20, hs_across=20)
+    rd, ra, sd, sa = _grid(5, 5, 20, 80, 20)
+    eng.setup(params_from_cfg(cfg))
+    dev = torch.device("cuda", 0)
+    st = [torch.as_tensor(np.asarray(s, np.int32)).to(dev) for s in (rd, ra, sd, sa)]
+    eng.set_precision(1, *taus)
+    try:
+        eng.precision_stats()
+        a = eng.run_windows(torch.from_numpy(ref).to(dev), torch.from_numpy(sec).to(dev), *st)
+        torch.cuda.synchronize()
+        checked, rerun = eng.precision_stats()
+        b = eng.run_windows_from_host(ref, sec, rd, ra, sd, sa, bands=2)
+    finally:
+        eng.set_precision(1, 2e-6, 2e-6)
+    print(f"\n[host rerun taus={taus}] checked={checked} rerun={rerun}", flush=True)
+    assert rerun == len(rd)
+    assert np.array_equal(a["offset"].cpu().numpy(), b["offset"])
+    for k in ("snr", "cov"):
+        np.testing.assert_allclose(b[k], a[k].cpu().numpy(), rtol=1e-6, atol=0)
+    o = O.run_windows(ref, sec, rd, ra, sd, sa, cfg)
+    assert np.abs(b["offset"] - o["offset"]).max() < 0.01
+    rel = np.abs(b["snr"] - o["snr"]) / np.abs(o["snr"])
+    assert rel.max() < 1e-3
+
+
 @pytest.mark.parametrize("mode", [MODE_FUSED, MODE_STAGED])
 def test_pitched_images_of_different_sizes(eng, scene1, mode):
     """Reference and secondary of different shapes, the secondary a column view
