@@ -117,6 +117,13 @@ int ampcor_run_windows(ampcor_ctx *ctx,
  * (i_maxLocShift) and oversampled-surface argmax (i_offsetZoomIn), plus the raw
  * peak value (r_maxval). Names follow CUAMPCOR_DEBUG (cuAmpcorChunk.cpp:82-198).
  * The fused path also reports each window's argmax margins (float[n]).
+ * out_offset_fp64 (double[n][2], down/across) is an FP64 sub-pixel refinement
+ * with no counterpart in the reference (whose offset is the bare argmax on the
+ * 1/(surf_ovs*raw_ovs) grid, cuOffset.cu:137-187): the FFT-interpolated surface
+ * is evaluated in FP64 at the peak and its axis neighbours, and a parabola per
+ * axis gives the vertex; needs corrSurfaceOverSamplingMethod 0 and a zoom
+ * window of at most 64 (else AMPCOR_E_UNSUPPORTED). out_offset stays the
+ * reference's grid offset.
  * Any of the extra pointers may be NULL. */
 int ampcor_run_windows_debug(ampcor_ctx *ctx,
                        const void *ref_img, int64_t ref_h, int64_t ref_w, int64_t ref_pitch,
@@ -127,6 +134,7 @@ int ampcor_run_windows_debug(ampcor_ctx *ctx,
                        int32_t *out_maxloc, int32_t *out_start, int32_t *out_shift,
                        int32_t *out_zoom_idx, float *out_maxval,
                        float *out_margin_coarse, float *out_margin_fine,
+                       double *out_offset_fp64,
                        int32_t chunk, int32_t mode, void *stream);
 
 /* Precision control. No counterpart in the reference (it runs FP32 only).

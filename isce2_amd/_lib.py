@@ -64,7 +64,7 @@ def load() -> C.CDLL:
     lib.ampcor_setup.argtypes = [vp, C.POINTER(Params), C.POINTER(Sizes)]
     run_args = [vp, vp, i64, i64, i64, vp, i64, i64, i64, vp, vp, vp, vp, i32, vp, vp, vp]
     lib.ampcor_run_windows.argtypes = run_args + [i32, i32, vp]
-    lib.ampcor_run_windows_debug.argtypes = run_args + [vp, vp, vp, vp, vp, vp, vp, i32, i32, vp]
+    lib.ampcor_run_windows_debug.argtypes = run_args + [vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp]
     lib.ampcor_set_precision.argtypes = [vp, i32, C.c_float, C.c_float]
     lib.ampcor_precision_stats.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
     lib.ampcor_set_fft_radices.argtypes = [vp, i32, vp, i32]

@@ -142,13 +142,15 @@ class Autotuner:
                     best = Schedule(mode=mode, chunk=chunk, ms_per_window=t)
             if best.mode == MODE_FUSED and params.window_h == 64:
                 times = {}
-                for ft in (512, 1024):
+                for fk, ft in ((2, 1024), (1, 512), (1, 1024)):
+                    eng.set_option("fine_kernel", fk)
                     eng.set_option("fine_threads", ft)
-                    times[ft] = self._time(eng, sub, MODE_FUSED, min(best.chunk, m), m)
-                ft = min(times, key=times.get)
+                    times[(fk, ft)] = self._time(eng, sub, MODE_FUSED, min(best.chunk, m), m)
+                fk, ft = min(times, key=times.get)
+                eng.set_option("fine_kernel", fk)
                 eng.set_option("fine_threads", ft)
-                best.options["fine_threads"] = ft
-                best.ms_per_window = min(best.ms_per_window, times[ft])
+                best.options.update(fine_kernel=fk, fine_threads=ft)
+                best.ms_per_window = min(best.ms_per_window, times[(fk, ft)])
             if best.mode == MODE_STAGED:
                 m = min(m, self.radix_sample)
                 sub = list(args[:2]) + [t[:m].contiguous() for t in args[2:]]

@@ -45,6 +45,27 @@ struct Sizes {
     ampcor_sizes s{};
 };
 
+// Device tables the fused kernels read (FP32 twiddles, half-sample shifts).
+struct FusedTables {
+    const void *tw1, *twW, *twZ, *tw2, *tw256, *h_ref, *h_zoom;
+    const void *tw256_64;          // FP64 e^{-2 pi i k/256} (peak refinement)
+};
+
+// Pointers of one fused launch: images and starts, per-window stage outputs,
+// final outputs (all indexed by absolute window) and the oversampled-chip
+// intermediates (this launch's windows only).
+struct FusedLaunch {
+    const void *ref; int64_t ref_pitch;
+    const void *sec; int64_t sec_pitch;
+    const int32_t *ref_d, *ref_a, *sec_d, *sec_a;
+    int32_t *loc, *start, *shift, *zidx;
+    float *val, *mc, *mf;
+    float *offset, *snr, *cov;
+    float *tamp; double *tmean; float *samp;
+    float *surf = nullptr;         // [n][16][16] trimmed surfaces for the refinement (this launch), or null
+    double *refined = nullptr;     // [n][2] refined offsets (absolute), or null
+};
+
 struct Engine {
     int device = 0;
     ampcor_params p{};
@@ -112,6 +133,7 @@ struct Engine {
         // (the staged pipeline skips its raw stage for chunks of such windows only)
         const int32_t *fine_start = nullptr, *fine_shift = nullptr;
         int32_t fine_from = 0;
+        double *refined = nullptr;   // [n][2] FP64 parabolic refinement of the offset, or null
     };
     // precision: 0 FP32 only; 1 FP32 + FP64 re-run of windows whose argmax
     // margins fall below the thresholds; 2 FP64 staged pipeline for every window
@@ -119,6 +141,9 @@ struct Engine {
     // on a 31k-window speckle field every FP32/FP64 flip had a margin <= 6e-8
     // launch-shape options (ampcor_set_option): threads per workgroup of the window-64 fine kernel
     int fine_threads = 1024;
+    // window-64 fine kernel: 1 = whole template in LDS (one workgroup per CU),
+    // 2 = template streamed in 32-row chunks (two workgroups per CU)
+    int fine_kernel = 2;
     // complex elements per workgroup of the staged FFT kernels (float; double uses half)
     int fft_block_elems = 2048;
     // columns per workgroup of the FP32 staged column FFTs (0: from fft_block_elems)
@@ -132,6 +157,8 @@ struct Engine {
     void rerun_fp64(const RunArgs &a, const float *mc, const float *mf, const int32_t *start32 = nullptr,
                     const int32_t *shift32 = nullptr);
     void run_fused(const RunArgs &a);
+    FusedTables fused_tables();
+    void launch_fused(const FusedTables &tb, const FusedLaunch &c, int n, int w0, hipStream_t st);
 };
 
 // FFT schedule for length n (radices in pass order; direct DFT if a prime > 17 remains)

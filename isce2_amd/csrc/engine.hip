@@ -152,6 +152,12 @@ void Engine::setup(const ampcor_params &q) {
     p = q;
     sz = s;
     configured = true;
+    // the fused kernels' tables are built here, so that runs (the allocation-free
+    // entry in particular) find them in place
+    if (s.fused) {
+        HIP_OK(hipSetDevice(device));
+        (void)fused_tables();
+    }
 }
 
 // Replace the radix schedule of the length-n staged FFTs (autotuning hook).
@@ -518,6 +524,11 @@ void Engine::run_staged_t(const RunArgs &a) {
             hipLaunchKernelGGL(k_subpixel, dim3(idivup(n, 64)), dim3(64), 0, st, start, zidx, n, ratio, p.hs_down,
                                p.hs_across, a.offset + 2 * size_t(w0));
         }
+        if (a.refined)
+            hipLaunchKernelGGL(k_refine<T>, dim3(n), dim3(64), refine_lds(Z), st, corr_os, sz.os_corr_h, sz.os_corr_w,
+                               Z, p.surf_ovs, 1.0 / double(p.surf_ovs * p.raw_ovs), zidx, start, p.hs_down,
+                               p.hs_across, static_cast<const cx<double> *>(plan(So).tw64.ptr),
+                               a.refined + 2 * size_t(w0));
         if (a.maxloc && !fine_only) HIP_OK(hipMemcpyAsync(a.maxloc + 2 * size_t(w0), loc, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToDevice, st));
         if (a.start) HIP_OK(hipMemcpyAsync(a.start + 2 * size_t(w0), start, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToDevice, st));
         if (a.shift) HIP_OK(hipMemcpyAsync(a.shift + 2 * size_t(w0), shift, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToDevice, st));
@@ -541,32 +552,6 @@ void Engine::run_staged(const RunArgs &a) {
 template void Engine::run_staged_t<double>(const RunArgs &a);
 
 namespace {
-// no-sync bounds check: copy the start arrays, replacing every window whose
-// chips leave their image by window (0, 0) and flagging it
-__global__ void k_sanitize_starts(const int32_t *rd, const int32_t *ra, const int32_t *sd, const int32_t *sa, int n,
-                                  int64_t ref_h, int64_t ref_w, int64_t sec_h, int64_t sec_w, int wh, int ww,
-                                  int sh, int sw, int32_t *out, int32_t *bad) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int64_t r0 = rd[i], a0 = ra[i], s0 = sd[i], t0 = sa[i];
-    const bool ok = r0 >= 0 && a0 >= 0 && r0 + wh <= ref_h && a0 + ww <= ref_w && s0 >= 0 && t0 >= 0 &&
-                    s0 + sh <= sec_h && t0 + sw <= sec_w && ref_h >= wh && ref_w >= ww && sec_h >= sh && sec_w >= sw;
-    out[i] = ok ? int32_t(r0) : 0;
-    out[n + i] = ok ? int32_t(a0) : 0;
-    out[2 * n + i] = ok ? int32_t(s0) : 0;
-    out[3 * n + i] = ok ? int32_t(t0) : 0;
-    bad[i] = ok ? 0 : 1;
-}
-
-__global__ void k_mark_bad(const int32_t *bad, int n, float *offset, float *snr, float *cov) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !bad[i]) return;
-    const float q = __builtin_nanf("");
-    offset[2 * i] = q; offset[2 * i + 1] = q;
-    snr[i] = q;
-    cov[3 * i] = q; cov[3 * i + 1] = q; cov[3 * i + 2] = q;
-}
-
 __global__ void k_take_i32(const int32_t *src, const int32_t *idx, int32_t *dst, int n, int width) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n * width) dst[i] = src[size_t(idx[i / width]) * width + i % width];
@@ -625,6 +610,8 @@ void Engine::rerun_fp64(const RunArgs &a, const float *mc, const float *mf, cons
     sub.maxloc = o_i; sub.start = o_i + 2 * m; sub.shift = o_i + 4 * m; sub.zoom_idx = o_i + 6 * m;
     sub.chunk = 0;
     sub.fine_start = sub.fine_shift = nullptr;
+    double *o_ref = a.refined ? buf<double>("p_refined", size_t(2) * m) : nullptr;
+    sub.refined = o_ref;
     if (m_full < m) {
         hipLaunchKernelGGL(k_take_i32, dim3(idivup(2 * m, 256)), dim3(256), 0, a.stream, start32, d_idx, st4 + 4 * m, m, 2);
         hipLaunchKernelGGL(k_take_i32, dim3(idivup(2 * m, 256)), dim3(256), 0, a.stream, shift32, d_idx, st4 + 6 * m, m, 2);
@@ -641,6 +628,8 @@ void Engine::rerun_fp64(const RunArgs &a, const float *mc, const float *mf, cons
         if (dst && cnt) hipLaunchKernelGGL(k_put<int32_t>, dim3(idivup(cnt * 2, 256)), dim3(256), 0, a.stream, src, d_idx, dst, cnt, 2);
     };
     put_f(o_off, a.offset, m, 2);
+    if (o_ref)
+        hipLaunchKernelGGL(k_put<double>, dim3(idivup(m * 2, 256)), dim3(256), 0, a.stream, o_ref, d_idx, a.refined, m, 2);
     put_i(sub.zoom_idx, a.zoom_idx, m);
     put_f(o_snr, a.snr, m_full, 1);
     put_f(o_cov, a.cov, m_full, 3);
@@ -672,6 +661,9 @@ void Engine::run(const RunArgs &in) {
     struct Restore { bool &f; bool v; ~Restore() { f = v; } } restore{no_alloc, no_alloc};
     no_alloc = capturing;
 
+    if (a.refined && (p.surf_method != 0 || sz.zoom > 64))
+        throw Error(AMPCOR_E_UNSUPPORTED, "the FP64 peak refinement needs FFT surface oversampling and a zoom "
+                                          "window of at most 64");
     const bool fused = (a.mode == AMPCOR_MODE_FUSED) || (a.mode == AMPCOR_MODE_AUTO && sz.fused);
     if (a.mode == AMPCOR_MODE_FUSED && !sz.fused)
         throw Error(AMPCOR_E_UNSUPPORTED, "the fused kernels do not cover this configuration");
@@ -683,7 +675,7 @@ void Engine::run(const RunArgs &in) {
         // instead and report NaN outputs
         int32_t *st4 = buf<int32_t>("ns_starts", 4 * size_t(a.n));
         bad = buf<int32_t>("ns_bad", a.n);
-        hipLaunchKernelGGL(k_sanitize_starts, dim3(idivup(a.n, 256)), dim3(256), 0, a.stream, a.ref_d, a.ref_a,
+        hipLaunchKernelGGL(k_sanitize_starts<int32_t>, dim3(idivup(a.n, 256)), dim3(256), 0, a.stream, a.ref_d, a.ref_a,
                            a.sec_d, a.sec_a, a.n, a.ref_h, a.ref_w, a.sec_h, a.sec_w, p.window_h, p.window_w,
                            sz.search_h, sz.search_w, st4, bad);
         a.ref_d = st4; a.ref_a = st4 + a.n; a.sec_d = st4 + 2 * size_t(a.n); a.sec_a = st4 + 3 * size_t(a.n);
@@ -707,7 +699,7 @@ void Engine::run(const RunArgs &in) {
     else if (fused) run_fused(a);
     else run_staged(a);
     if (bad)
-        hipLaunchKernelGGL(k_mark_bad, dim3(idivup(a.n, 256)), dim3(256), 0, a.stream, bad, a.n, a.offset, a.snr, a.cov);
+        hipLaunchKernelGGL(k_mark_bad<float>, dim3(idivup(a.n, 256)), dim3(256), 0, a.stream, bad, a.n, a.offset, a.snr, a.cov);
     HIP_OK(hipGetLastError());
 }
 
@@ -779,8 +771,8 @@ int ampcor_run_windows_debug(ampcor_ctx *ctx, const void *ref_img, int64_t ref_h
                              const int32_t *ref_d, const int32_t *ref_a, const int32_t *sec_d,
                              const int32_t *sec_a, int32_t n, float *out_offset, float *out_snr, float *out_cov,
                              int32_t *out_maxloc, int32_t *out_start, int32_t *out_shift, int32_t *out_zoom_idx,
-                             float *out_maxval, float *out_margin_coarse, float *out_margin_fine, int32_t chunk,
-                             int32_t mode, void *stream) {
+                             float *out_maxval, float *out_margin_coarse, float *out_margin_fine,
+                             double *out_offset_fp64, int32_t chunk, int32_t mode, void *stream) {
     if (!ctx) return AMPCOR_E_ARG;
     return guarded(ctx, [&] {
         ampcor::Engine::RunArgs a{ref_img, ref_h, ref_w, ref_pitch, sec_img, sec_h, sec_w, sec_pitch,
@@ -788,6 +780,7 @@ int ampcor_run_windows_debug(ampcor_ctx *ctx, const void *ref_img, int64_t ref_h
                                   out_maxloc, out_start, out_shift, out_zoom_idx, out_maxval,
                                   chunk, mode, static_cast<hipStream_t>(stream),
                                   out_margin_coarse, out_margin_fine};
+        a.refined = out_offset_fp64;
         ctx->eng.run(a);
     });
 }
@@ -812,6 +805,9 @@ int ampcor_set_option(ampcor_ctx *ctx, const char *name, int64_t value) {
             if (value != 0 && value != 8 && value != 16)
                 throw ampcor::Error(AMPCOR_E_ARG, "fft_cols_width must be 0, 8 or 16");
             ctx->eng.fft_cols_width = int(value);
+        } else if (k == "fine_kernel") {
+            if (value != 1 && value != 2) throw ampcor::Error(AMPCOR_E_ARG, "fine_kernel must be 1 or 2");
+            ctx->eng.fine_kernel = int(value);
         } else if (k == "fine_threads") {
             if (value != 512 && value != 1024) throw ampcor::Error(AMPCOR_E_ARG, "fine_threads must be 512 or 1024");
             ctx->eng.fine_threads = int(value);
@@ -844,7 +840,7 @@ int ampcor_run_windows(ampcor_ctx *ctx, const void *ref_img, int64_t ref_h, int6
                        void *stream) {
     return ampcor_run_windows_debug(ctx, ref_img, ref_h, ref_w, ref_pitch, sec_img, sec_h, sec_w, sec_pitch, ref_d,
                                     ref_a, sec_d, sec_a, n, out_offset, out_snr, out_cov, nullptr, nullptr,
-                                    nullptr, nullptr, nullptr, nullptr, nullptr, chunk, mode, stream);
+                                    nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, chunk, mode, stream);
 }
 
 int ampcor_profile(ampcor_ctx *ctx, int32_t enable) {

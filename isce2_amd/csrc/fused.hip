@@ -4,6 +4,8 @@
 // k_coarse -> k_oversample (reference chip; search zoom chip) -> k_fine.
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
 #include <vector>
 #include "engine.h"
 #include "fused_kernels.h"
@@ -350,7 +352,138 @@ struct FineArgs {
     const cf *tw256;                           // e^{-2 pi i k/256}
     int32_t *zoom_idx; float *offset;
     float *margin;                             // [n] top-1 minus top-2 of the surface, or null
+    float *surf;                               // [n][16][16] trimmed surface for the FP64 refinement, or null
 };
+
+// ---------------------------------------------------------------------------
+// Surface stage of the fine kernels: trimmed 16x16 normalized surface (corr,
+// row pitch NL) -> 16x16 +i DFT -> 16-sparse spectrum G[16][256] -> column
+// bounds -> evaluation of the kept columns with 16-point DFTs -> argmax with the
+// reference tie rule -> zoom index, offset, margin. LDS: G (16*256 cf), abuf
+// (512 cf: the 16x16 spectrum and 256 twiddles), cols (256 ints), red (64 doubles).
+// ---------------------------------------------------------------------------
+template <int NTH>
+__device__ __forceinline__ void fine_surface(const FineArgs &a, const float *corr, int NL, cf *G, cf *abuf, int *cols,
+                                             double *red, int win) {
+    constexpr int Z = 16, SO = 256;
+    // trimmed 16x16 surface -> 2-D +i DFT / 256 (cuOverSampler.cpp:141-145)
+    cf *a16 = abuf;          // [16][16]
+    cf *tws = abuf + 256;    // e^{-2 pi i k/256}
+    for (int i = threadIdx.x; i < Z * Z; i += NTH) {
+        const int r = i / Z, c = i - r * Z;
+        a16[i] = {corr[r * NL + c] * (1.0f / 256.0f), 0.f};
+        if (a.surf) a.surf[size_t(win) * Z * Z + i] = corr[r * NL + c];
+    }
+    for (int i = threadIdx.x; i < SO; i += NTH) tws[i] = a.tw256[i];
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        cf x[16];
+#pragma unroll
+        for (int c = 0; c < 16; ++c) x[c] = a16[threadIdx.x * 16 + c];
+        dft16<float, +1>(x);
+#pragma unroll
+        for (int c = 0; c < 16; ++c) a16[threadIdx.x * 16 + c] = x[c];
+    }
+    __syncthreads();
+    if (threadIdx.x < 16) {
+        cf x[16];
+#pragma unroll
+        for (int r = 0; r < 16; ++r) x[r] = a16[r * 16 + threadIdx.x];
+        dft16<float, +1>(x);
+#pragma unroll
+        for (int r = 0; r < 16; ++r) a16[r * 16 + threadIdx.x] = x[r];
+    }
+    __syncthreads();
+    // G[k1][16c+dd] = DFT16_c( A[k1][k2] e^{-2 pi i dd ks2/256} ), ks2 = signed k2
+    if (threadIdx.x < 256) {
+        const int k1 = threadIdx.x >> 4, dd = threadIdx.x & 15;
+        cf x[16];
+#pragma unroll
+        for (int k2 = 0; k2 < 16; ++k2) {
+            const int ks = k2 < 8 ? k2 : k2 - 16;
+            x[k2] = cmul(a16[k1 * 16 + k2], tws[(dd * ks + 4096) & 255]);
+        }
+        dft16<float, -1>(x);
+#pragma unroll
+        for (int c = 0; c < 16; ++c) G[k1 * SO + 16 * c + dd] = x[c];
+    }
+    __syncthreads();
+    // Column pruning. Every value of surface column q is at most
+    // B_q = sum_k1 |G[k1][q]|, and the surface passes through the trimmed
+    // samples, so its maximum is at least their largest value amax. A column with
+    // B_q (1 + 2^-10) + 2^-11 |a|max < amax can hold neither the argmax nor any
+    // value within 2^-12 |a|max of it (the slack covers FP32 rounding many times
+    // over): it is skipped, which leaves the argmax, the tie rule and every
+    // margin below 2^-12 |a|max unchanged. NaN bounds keep their column.
+    static_assert(NTH >= SO, "one thread per surface column for the bounds");
+    float amax = -FLT_MAX, amag = 0.f;
+    for (int i = threadIdx.x & 63; i < Z * Z; i += 64) {
+        const float v = corr[(i / Z) * NL + (i % Z)];
+        amax = fmaxf(amax, v);
+        amag = fmaxf(amag, fabsf(v));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        amax = fmaxf(amax, __shfl_xor(amax, o));
+        amag = fmaxf(amag, __shfl_xor(amag, o));
+    }
+    uint64_t *masks = reinterpret_cast<uint64_t *>(red + 48);   // ballots of the SO/64 bound waves
+    bool keep = false;
+    if (threadIdx.x < SO) {
+        float bq = 0.f;
+#pragma unroll
+        for (int k1 = 0; k1 < 16; ++k1) {
+            const cf g = G[k1 * SO + threadIdx.x];
+            bq += sqrtf(g.x * g.x + g.y * g.y);
+        }
+        keep = !(bq * (1.0f + 0x1p-10f) + 0x1p-11f * amag < amax);
+    }
+    const uint64_t wm = __ballot(keep);
+    if (threadIdx.x < SO && (threadIdx.x & 63) == 0) masks[threadIdx.x >> 6] = wm;
+    __syncthreads();
+    int ncols = 0, before = 0;
+#pragma unroll
+    for (int w = 0; w < SO / 64; ++w) {
+        const int c = __popcll(masks[w]);
+        if (w < (threadIdx.x >> 6)) before += c;
+        ncols += c;
+    }
+    if (keep) cols[before + __popcll(wm & ((uint64_t(1) << (threadIdx.x & 63)) - 1))] = threadIdx.x;
+    __syncthreads();
+    // out[16aa+b][q] = Re DFT16_aa( G[k1][q] e^{-2 pi i b ks1/256} ) over the kept
+    // columns q; task t = (b, column list entry); running argmax
+    Top2<float> top;
+    #pragma unroll 1
+    for (int t = threadIdx.x; t < 16 * ncols; t += NTH) {
+        const int b = t / ncols, qq = cols[t - b * ncols];
+        // e^{-2 pi i b k/256} for k = 1..8 (negative bins use the conjugate)
+        cf w[9];
+#pragma unroll
+        for (int k = 1; k <= 8; ++k) w[k] = tws[(b * k) & 255];
+        cf x[16];
+        x[0] = G[qq];
+#pragma unroll
+        for (int k1 = 1; k1 < 16; ++k1) {
+            const cf tk = k1 < 8 ? w[k1] : conj(w[16 - k1]);
+            x[k1] = cmul(G[k1 * SO + qq], tk);
+        }
+        dft16<float, -1>(x);
+#pragma unroll
+        for (int aa = 0; aa < 16; ++aa) top.add_idx(x[aa].x, uint32_t((16 * aa + b) * SO + qq));
+    }
+    // red: [0, 16) block sums, then NTH/64 top-1 values, runner-up values and keys
+    top.reduce<NTH>(reinterpret_cast<float *>(red + 16), reinterpret_cast<float *>(red + 16) + 16,
+                    reinterpret_cast<uint64_t *>(red + 32));
+    if (threadIdx.x == 0) {
+        const int idx = top.index();
+        const int px = idx / SO, py = idx % SO;
+        a.zoom_idx[2 * win] = px;
+        a.zoom_idx[2 * win + 1] = py;
+        a.offset[2 * win] = a.ratio * float(px) + float(a.start[2 * win]) - float(a.hs_d);
+        a.offset[2 * win + 1] = a.ratio * float(py) + float(a.start[2 * win + 1]) - float(a.hs_a);
+        if (a.margin) a.margin[win] = top.margin();
+    }
+}
 
 template <int T, int R1, int R2, int K, int NTH>
 __global__ void __launch_bounds__(NTH) k_fine(FineArgs a, const cf *tw, int w0) {
@@ -438,124 +571,136 @@ __global__ void __launch_bounds__(NTH) k_fine(FineArgs a, const cf *tw, int w0) 
     __syncthreads();
     inverse_lags<R1, R2, NTH>(Qs, NL, NL, S + lay.y_off, PP, tw, nrm, corr);
 
-    // trimmed 16x16 surface -> 2-D +i DFT / 256 (cuOverSampler.cpp:141-145)
-    cf *a16 = Tb;            // [16][16]
-    cf *tws = Tb + 256;      // e^{-2 pi i k/256}
-    for (int i = threadIdx.x; i < Z * Z; i += NTH) {
-        const int r = i / Z, c = i - r * Z;
-        a16[i] = {corr[r * NL + c] * (1.0f / 256.0f), 0.f};
+    fine_surface<NTH>(a, corr, NL, S, Tb, reinterpret_cast<int *>(nrm), red, win);
+}
+
+// ---------------------------------------------------------------------------
+// Fine stage, streamed over template rows: the computation of k_fine with the
+// template taken TC rows at a time and only the search rows those rows meet in
+// LDS (TC + NL - 1 rows; the last NL - 1 carry over to the next chunk), so a
+// workgroup needs ~68 KB of LDS and two share a CU. The window sums of the
+// normalization come from FP64 column sums accumulated as the search rows
+// arrive, with the first and last NL - 1 rows kept for the slides.
+// ---------------------------------------------------------------------------
+struct Fine2Lds {
+    int sr, tb, save, col, nrm, corr, red, total;   // bytes
+    __host__ __device__ Fine2Lds(int P, int NL, int TC) {
+        const int PP = P + 1;
+        const int srp = (TC + NL - 1 + 1) / 2, tcp = TC / 2;
+        sr = 0;
+        tb = sr + srp * PP * 8;                    // template pairs directly after the search pairs
+        save = tb + tcp * PP * 8;
+        col = al16(save + 2 * (NL - 1) * P * 4);
+        nrm = col + 2 * P * 8;
+        corr = al16(nrm + NL * NL * 4);
+        red = al16(corr + NL * NL * 4);
+        total = red + 64 * 8;
     }
-    for (int i = threadIdx.x; i < SO; i += NTH) tws[i] = a.tw256[i];
-    __syncthreads();
-    if (threadIdx.x < 16) {
-        cf x[16];
+};
+
+template <int T, int R1, int R2, int K, int NTH, int TC>
+__global__ void __launch_bounds__(NTH, 2) k_fine2(FineArgs a, const cf *tw, int w0) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    constexpr int P = R1 * R2, L = P, PP = L + 1, H = L / 2;
+    constexpr int NL = P - T + 1;            // lags per axis (17)
+    constexpr int NCH = T / TC;              // template chunks
+    constexpr int SROWS = TC + NL - 1;       // search rows a chunk meets
+    constexpr int SRP = (SROWS + 1) / 2;     // as row pairs
+    constexpr int KEEP = (NL - 1) / 2;       // pairs carried to the next chunk
+    constexpr int TCP = TC / 2;
+    constexpr int NS = NL - 1;               // rows kept at each end for the slides
+    constexpr int Q = P / 2, TQ = T / 2;     // polyphase plane sides
+    static_assert(T % TC == 0 && TC % 2 == 0 && NS % 2 == 0 && NS == 16, "even chunks, 16x16 zoom surface");
+    static_assert(2 * P <= NTH, "one thread per column and sum");
+    const Fine2Lds lay(P, NL, TC);
+    cf *SR = reinterpret_cast<cf *>(smem + lay.sr);
+    cf *TB = reinterpret_cast<cf *>(smem + lay.tb);
+    float *save = reinterpret_cast<float *>(smem + lay.save);   // rows [0, NS) then [T, T + NS), pitch P
+    double *col = reinterpret_cast<double *>(smem + lay.col);   // column sums of s, then s^2, over rows [0, T)
+    float *nrm = reinterpret_cast<float *>(smem + lay.nrm);
+    float *corr = reinterpret_cast<float *>(smem + lay.corr);
+    double *red = reinterpret_cast<double *>(smem + lay.red);
+    const int win = w0 + xcd_window(blockIdx.x, gridDim.x);
+    const float *tp = a.tamp + size_t(win) * T * T;
+    const float *sg = a.samp + size_t(win) * P * P;
+    const double mean = a.tmean[2 * win], t2 = a.tmean[2 * win + 1];
+    // planar oversampled chips: pixel (2j + p, 2c + q) sits in plane 2p + q at (j, c)
+    auto sval = [&](int r, int x) -> float {
+        return sg[size_t(2 * (r & 1) + (x & 1)) * Q * Q + (r >> 1) * Q + (x >> 1)];
+    };
+    auto tval = [&](int r, int x) -> float {
+        return tp[size_t(2 * (r & 1) + (x & 1)) * TQ * TQ + (r >> 1) * TQ + (x >> 1)];
+    };
+    const int csum_c = threadIdx.x % P, csum_k = threadIdx.x / P;    // column and sum (0: s, 1: s^2)
+    double cacc = 0.0;
+
+    const int f = threadIdx.x % H, blk = threadIdx.x / H, u0 = blk * K;
+    const bool active = (blk < NTH / H) && (u0 < NL);
+    cf q[K];
 #pragma unroll
-        for (int c = 0; c < 16; ++c) x[c] = a16[threadIdx.x * 16 + c];
-        dft16<float, +1>(x);
-#pragma unroll
-        for (int c = 0; c < 16; ++c) a16[threadIdx.x * 16 + c] = x[c];
-    }
-    __syncthreads();
-    if (threadIdx.x < 16) {
-        cf x[16];
-#pragma unroll
-        for (int r = 0; r < 16; ++r) x[r] = a16[r * 16 + threadIdx.x];
-        dft16<float, +1>(x);
-#pragma unroll
-        for (int r = 0; r < 16; ++r) a16[r * 16 + threadIdx.x] = x[r];
-    }
-    __syncthreads();
-    // G[k1][16c+dd] = DFT16_c( A[k1][k2] e^{-2 pi i dd ks2/256} ), ks2 = signed k2
-    cf *G = S;
-    if (threadIdx.x < 256) {
-        const int k1 = threadIdx.x >> 4, dd = threadIdx.x & 15;
-        cf x[16];
-#pragma unroll
-        for (int k2 = 0; k2 < 16; ++k2) {
-            const int ks = k2 < 8 ? k2 : k2 - 16;
-            x[k2] = cmul(a16[k1 * 16 + k2], tws[(dd * ks + 4096) & 255]);
-        }
-        dft16<float, -1>(x);
-#pragma unroll
-        for (int c = 0; c < 16; ++c) G[k1 * SO + 16 * c + dd] = x[c];
-    }
-    __syncthreads();
-    // Column pruning. Every value of surface column q is at most
-    // B_q = sum_k1 |G[k1][q]|, and the surface passes through the trimmed
-    // samples, so its maximum is at least their largest value amax. A column with
-    // B_q (1 + 2^-10) + 2^-11 |a|max < amax can hold neither the argmax nor any
-    // value within 2^-12 |a|max of it (the slack covers FP32 rounding many times
-    // over): it is skipped, which leaves the argmax, the tie rule and every
-    // margin below 2^-12 |a|max unchanged. NaN bounds keep their column.
-    static_assert(NTH >= SO, "one thread per surface column for the bounds");
-    float amax = -FLT_MAX, amag = 0.f;
-    for (int i = threadIdx.x & 63; i < Z * Z; i += 64) {
-        const float v = corr[(i / Z) * NL + (i % Z)];
-        amax = fmaxf(amax, v);
-        amag = fmaxf(amag, fabsf(v));
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        amax = fmaxf(amax, __shfl_xor(amax, o));
-        amag = fmaxf(amag, __shfl_xor(amag, o));
-    }
-    uint64_t *masks = reinterpret_cast<uint64_t *>(red + 48);   // ballots of the SO/64 bound waves
-    int *cols = reinterpret_cast<int *>(nrm);                    // swept columns (window sums are spent)
-    bool keep = false;
-    if (threadIdx.x < SO) {
-        float bq = 0.f;
-#pragma unroll
-        for (int k1 = 0; k1 < 16; ++k1) {
-            const cf g = G[k1 * SO + threadIdx.x];
-            bq += sqrtf(g.x * g.x + g.y * g.y);
-        }
-        keep = !(bq * (1.0f + 0x1p-10f) + 0x1p-11f * amag < amax);
-    }
-    const uint64_t wm = __ballot(keep);
-    if (threadIdx.x < SO && (threadIdx.x & 63) == 0) masks[threadIdx.x >> 6] = wm;
-    __syncthreads();
-    int ncols = 0, before = 0;
-#pragma unroll
-    for (int w = 0; w < SO / 64; ++w) {
-        const int c = __popcll(masks[w]);
-        if (w < (threadIdx.x >> 6)) before += c;
-        ncols += c;
-    }
-    if (keep) cols[before + __popcll(wm & ((uint64_t(1) << (threadIdx.x & 63)) - 1))] = threadIdx.x;
-    __syncthreads();
-    // out[16aa+b][q] = Re DFT16_aa( G[k1][q] e^{-2 pi i b ks1/256} ) over the kept
-    // columns q; task t = (b, column list entry); running argmax
-    Top2<float> top;
+    for (int k = 0; k < K; ++k) q[k] = {0.f, 0.f};
+
     #pragma unroll 1
-    for (int t = threadIdx.x; t < 16 * ncols; t += NTH) {
-        const int b = t / ncols, qq = cols[t - b * ncols];
-        // e^{-2 pi i b k/256} for k = 1..8 (negative bins use the conjugate)
-        cf w[9];
-#pragma unroll
-        for (int k = 1; k <= 8; ++k) w[k] = tws[(b * k) & 255];
-        cf x[16];
-        x[0] = G[qq];
-#pragma unroll
-        for (int k1 = 1; k1 < 16; ++k1) {
-            const cf tk = k1 < 8 ? w[k1] : conj(w[16 - k1]);
-            x[k1] = cmul(G[k1 * SO + qq], tk);
+    for (int ch = 0; ch < NCH; ++ch) {
+        const int c0 = ch * TC;
+        const int first = ch ? KEEP : 0;     // first pair slot loaded in this chunk
+        // search rows [c0 + 2 first, c0 + SROWS) into pair slots [first, SRP)
+#pragma unroll 4
+        for (int i = threadIdx.x; i < (SRP - first) * L; i += NTH) {
+            const int jj = first + i / L, x = i - (i / L) * L;
+            const int r = c0 + 2 * jj;
+            SR[jj * PP + x] = {sval(r, x), sval(r + 1, x)};
         }
-        dft16<float, -1>(x);
+        // template rows [c0, c0 + TC) minus the FP64 mean, zero beyond column T
+#pragma unroll 4
+        for (int i = threadIdx.x; i < TCP * L; i += NTH) {
+            const int j = i / L, x = i - j * L;
+            cf v = {0.f, 0.f};
+            if (x < T) {
+                const int r = c0 + 2 * j;
+                v = {float(double(tval(r, x)) - mean), float(double(tval(r + 1, x)) - mean)};
+            }
+            TB[j * PP + x] = v;
+        }
+        __syncthreads();
+        // column sums of the new rows below T; the first and last NS rows kept
+        if (csum_k < 2) {
+            for (int rr = 2 * first; rr < SROWS; ++rr) {
+                const int r = c0 + rr;
+                const cf pv = SR[(rr >> 1) * PP + csum_c];
+                const float v = (rr & 1) ? pv.y : pv.x;
+                if (r < T) cacc += csum_k ? double(v) * v : double(v);
+                if (csum_k == 0) {
+                    if (r < NS) save[r * P + csum_c] = v;
+                    else if (r >= T) save[(NS + r - T) * P + csum_c] = v;
+                }
+            }
+        }
+        __syncthreads();
+        // row spectra of the new search pairs and the template pairs (contiguous in LDS)
+        fft_nat2dr<float, -1, R1, R2, NTH>(SR + first * PP, SRP - first + TCP, PP, 1, tw);
+        split_pairs_inplace<R1, R2, NTH>(SR + first * PP, SRP - first + TCP, PP);
+        if (active) mac_rows<H, K>(q, SR, PP, TB, PP, 0, 0, TC, u0, SROWS, f);
+        __syncthreads();
+        if (ch + 1 < NCH) {
+            // the last KEEP pairs are the next chunk's first
+            for (int i = threadIdx.x; i < KEEP * PP; i += NTH) SR[i] = SR[(SRP - KEEP) * PP + i];
+            __syncthreads();
+        }
+    }
+    if (csum_k < 2) col[csum_k * P + csum_c] = cacc;
+    cf *Qs = SR;
+    if (active) {
 #pragma unroll
-        for (int aa = 0; aa < 16; ++aa) top.add_idx(x[aa].x, uint32_t((16 * aa + b) * SO + qq));
+        for (int k = 0; k < K; ++k)
+            if (u0 + k < NL) Qs[(u0 + k) * H + f] = q[k];
     }
-    // red: [0, 16) block sums, then NTH/64 top-1 values, runner-up values and keys
-    top.reduce<NTH>(reinterpret_cast<float *>(red + 16), reinterpret_cast<float *>(red + 16) + 16,
-                    reinterpret_cast<uint64_t *>(red + 32));
-    if (threadIdx.x == 0) {
-        const int idx = top.index();
-        const int px = idx / SO, py = idx % SO;
-        a.zoom_idx[2 * win] = px;
-        a.zoom_idx[2 * win + 1] = py;
-        a.offset[2 * win] = a.ratio * float(px) + float(a.start[2 * win]) - float(a.hs_d);
-        a.offset[2 * win + 1] = a.ratio * float(py) + float(a.start[2 * win + 1]) - float(a.hs_a);
-        if (a.margin) a.margin[win] = top.margin();
-    }
+    __syncthreads();
+    window_norms_rows<(P + 63) / 64, NTH>(
+        [&](int r, int c) -> double { return double(r < NS ? save[r * P + c] : save[(NS + r - T) * P + c]); },
+        col, 1, P, T, T, NL, NL, t2, nrm);
+    inverse_lags<R1, R2, NTH>(Qs, NL, NL, SR + ((NL * H + 1) & ~1), PP, tw, nrm, corr);
+    fine_surface<NTH>(a, corr, NL, SR, SR + 16 * 256, reinterpret_cast<int *>(nrm), red, win);
 }
 
 // ---------------------------------------------------------------------------
@@ -596,31 +741,14 @@ bool fused_supported(const ampcor_params &p, const ampcor_sizes &s) {
     return true;
 }
 
-void Engine::run_fused(const RunArgs &a) {
+// Tables the fused kernels read (twiddles, half-sample shifts). Built on first
+// use; Engine::setup builds them for fused configurations, so that the
+// allocation-free entry finds them in place.
+FusedTables Engine::fused_tables() {
     using namespace fused;
-    const hipStream_t st = a.stream;
-    const int W = p.window_w, Zc = sz.zoom_in_w;          // 64 -> 72, 32 -> 40
-    const int T2 = 2 * W, P = 2 * Zc;
-    const int L1 = coarse_L(sz.search_w);
-    const int K1 = pick_k(sz.raw_corr_h, NT / (L1 / 2));
-    const int mode = p.image_complex ? (p.deramp == 0 ? 1 : 0) : 2;
-    const int chunk = std::min(a.chunk > 0 ? a.chunk : 32768, 65535);
-    const int nb = std::min(chunk, a.n);
-
-    int32_t *loc = a.maxloc ? a.maxloc : buf<int32_t>("f_loc", size_t(a.n) * 2);
-    float *val = a.maxval ? a.maxval : buf<float>("f_val", a.n);
-    int32_t *start = a.start ? a.start : buf<int32_t>("f_start", size_t(a.n) * 2);
-    int32_t *shift = a.shift ? a.shift : buf<int32_t>("f_shift", size_t(a.n) * 2);
-    int32_t *zidx = a.zoom_idx ? a.zoom_idx : buf<int32_t>("f_zidx", size_t(a.n) * 2);
-    float *tamp = buf<float>("f_tamp", size_t(nb) * T2 * T2);
-    double *tmean = buf<double>("f_tmean", 2 * size_t(nb));
-    float *samp = buf<float>("f_samp", size_t(nb) * P * P);
-    const bool want_margin = precision == 1 || a.margin_coarse || a.margin_fine;
-    float *mc = a.margin_coarse ? a.margin_coarse : (want_margin ? buf<float>("f_mc", a.n) : nullptr);
-    float *mf = a.margin_fine ? a.margin_fine : (want_margin ? buf<float>("f_mf", a.n) : nullptr);
-
+    const int W = p.window_w, Zc = sz.zoom_in_w;
     // half-sample shift tables h[k] = e^{-i pi k_s/N}/N for the chip sizes
-    auto hshift = [&](int N) -> const cf * {
+    auto hshift = [&](int N) -> const void * {
         const std::string key = "f_hsh" + std::to_string(N);
         auto &b = bufs[key];
         if (!b) {
@@ -636,33 +764,56 @@ void Engine::run_fused(const RunArgs &a) {
             b->reserve(h.size() * sizeof(float));
             HIP_OK(hipMemcpy(b->ptr, h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
         }
-        return b->as<cf>();
+        return b->ptr;
     };
-    const cf *h_ref = hshift(W), *h_zoom = hshift(Zc);
-    auto tw = [&](int n) { return static_cast<const cf *>(plan(n).tw32.ptr); };
-    const cf *tw1 = tw(L1), *twW = tw(W), *twZ = tw(Zc), *tw2 = tw(P), *tw256 = tw(256);
+    auto tw = [&](int n) { return static_cast<const void *>(plan(n).tw32.ptr); };
+    FusedTables t;
+    t.h_ref = hshift(W);
+    t.h_zoom = hshift(Zc);
+    t.tw1 = tw(coarse_L(sz.search_w));
+    t.twW = tw(W);
+    t.twZ = tw(Zc);
+    t.tw2 = tw(2 * Zc);
+    t.tw256 = tw(256);
+    t.tw256_64 = plan(256).tw64.ptr;
+    return t;
+}
 
-    CoarseArgs ca{a.ref, a.ref_pitch, a.sec, a.sec_pitch, mode, a.ref_d, a.ref_a, a.sec_d, a.sec_a,
+// One launch of each fused kernel over windows [w0, w0 + n) of the arrays in c
+// (starts, stage outputs and final outputs indexed by absolute window; the
+// oversampled-chip intermediates hold this launch's n windows only).
+void Engine::launch_fused(const FusedTables &tb, const FusedLaunch &c, int n, int w0, hipStream_t st) {
+    using namespace fused;
+    const int W = p.window_w, Zc = sz.zoom_in_w;          // 64 -> 72, 32 -> 40
+    const int T2 = 2 * W, P = 2 * Zc;
+    const int L1 = coarse_L(sz.search_w);
+    const int K1 = pick_k(sz.raw_corr_h, NT / (L1 / 2));
+    const int mode = p.image_complex ? (p.deramp == 0 ? 1 : 0) : 2;
+    const cf *tw1 = static_cast<const cf *>(tb.tw1), *twW = static_cast<const cf *>(tb.twW);
+    const cf *twZ = static_cast<const cf *>(tb.twZ), *tw2 = static_cast<const cf *>(tb.tw2);
+    const cf *tw256 = static_cast<const cf *>(tb.tw256);
+    const cf *h_ref = static_cast<const cf *>(tb.h_ref), *h_zoom = static_cast<const cf *>(tb.h_zoom);
+    float *tamp = c.tamp - size_t(w0) * T2 * T2;          // indexed by absolute window inside the kernels
+    double *tmean = c.tmean - 2 * size_t(w0);
+    float *samp = c.samp - size_t(w0) * P * P;
+
+    CoarseArgs ca{c.ref, c.ref_pitch, c.sec, c.sec_pitch, mode, c.ref_d, c.ref_a, c.sec_d, c.sec_a,
                   p.hs_down, p.hs_across, sz.half_zoom, sz.corr_raw_zoom_h, sz.corr_raw_zoom_w,
-                  loc, val, start, shift, a.snr, a.cov, mc};
+                  c.loc, c.val, c.start, c.shift, c.snr, c.cov, c.mc};
     const size_t lds_c = CoarseLds(W, L1, sz.search_h, sz.raw_corr_h, sz.raw_corr_w).total;
     const size_t lds_or = os_lds_bytes(W), lds_oz = os_lds_bytes(Zc);
     const size_t lds_f = FineLds(P, P - T2 + 1, T2).total;
-
-    for (int w0 = 0; w0 < a.n; w0 += nb) {
-        const int n = std::min(nb, a.n - w0);
+    const size_t lds_f2 = Fine2Lds(P, P - T2 + 1, 32).total;
+    {
         timed("k_coarse", n, st, [&] {
             if (W == 64) launch_coarse<64, 8, 16>(K1, ca, tw1, n, w0, lds_c, st);
             else if (L1 == 128) launch_coarse<32, 8, 16>(K1, ca, tw1, n, w0, lds_c, st);
             else launch_coarse<32, 8, 8>(K1, ca, tw1, n, w0, lds_c, st);
         });
-        // chips use absolute window indices; oversampler outputs are per chunk
-        OsArgs oref{a.ref, a.ref_pitch, mode, a.ref_d, a.ref_a, nullptr, p.deramp, h_ref,
-                    tamp - size_t(w0) * T2 * T2, tmean - 2 * size_t(w0)};
-        OsArgs osec{a.sec, a.sec_pitch, mode, a.sec_d, a.sec_a, start, p.deramp, h_zoom,
-                    samp - size_t(w0) * P * P, nullptr};
-        FineArgs fa{tamp - size_t(w0) * T2 * T2, tmean - 2 * size_t(w0), samp - size_t(w0) * P * P, start, p.hs_down,
-                    p.hs_across, 1.0f / float(p.surf_ovs * p.raw_ovs), tw256, zidx, a.offset, mf};
+        OsArgs oref{c.ref, c.ref_pitch, mode, c.ref_d, c.ref_a, nullptr, p.deramp, h_ref, tamp, tmean};
+        OsArgs osec{c.sec, c.sec_pitch, mode, c.sec_d, c.sec_a, c.start, p.deramp, h_zoom, samp, nullptr};
+        FineArgs fa{tamp, tmean, samp, c.start, p.hs_down, p.hs_across, 1.0f / float(p.surf_ovs * p.raw_ovs),
+                    tw256, c.zidx, c.offset, c.mf, c.surf ? c.surf - size_t(w0) * 256 : nullptr};
         if (W == 64) {
             timed("k_oversample_ref", n, st, [&] {
                 hipLaunchKernelGGL((k_oversample<8, 8, true>), dim3(n), dim3(NT), lds_or, st, oref, twW, w0);
@@ -671,7 +822,9 @@ void Engine::run_fused(const RunArgs &a) {
                 hipLaunchKernelGGL((k_oversample<8, 9, false>), dim3(n), dim3(NT), lds_oz, st, osec, twZ, w0);
             });
             timed("k_fine", n, st, [&] {
-                if (fine_threads == 1024)
+                if (fine_kernel == 2)
+                    hipLaunchKernelGGL((k_fine2<128, 9, 16, 3, NT, 32>), dim3(n), dim3(NT), lds_f2, st, fa, tw2, w0);
+                else if (fine_threads == 1024)
                     hipLaunchKernelGGL((k_fine<128, 9, 16, 2, 1024>), dim3(n), dim3(1024), lds_f, st, fa, tw2, w0);
                 else
                     hipLaunchKernelGGL((k_fine<128, 16, 9, 3, NT>), dim3(n), dim3(NT), lds_f, st, fa, tw2, w0);
@@ -687,9 +840,42 @@ void Engine::run_fused(const RunArgs &a) {
                 hipLaunchKernelGGL((k_fine<64, 16, 5, 2, NT>), dim3(n), dim3(NT), lds_f, st, fa, tw2, w0);
             });
         }
+        if (c.refined)
+            hipLaunchKernelGGL(k_refine<float>, dim3(n), dim3(64), refine_lds(16), st, c.surf, 16, 16, 16,
+                               p.surf_ovs, 1.0 / double(p.surf_ovs * p.raw_ovs), c.zidx + 2 * size_t(w0),
+                               c.start + 2 * size_t(w0), p.hs_down, p.hs_across,
+                               static_cast<const cx<double> *>(tb.tw256_64), c.refined + 2 * size_t(w0));
         HIP_OK(hipGetLastError());
     }
-    if (precision == 1 && !a.nosync) rerun_fp64(a, mc, mf, start, shift);
+}
+
+void Engine::run_fused(const RunArgs &a) {
+    const int W = p.window_w, Zc = sz.zoom_in_w;
+    const int T2 = 2 * W, P = 2 * Zc;
+    const int chunk = std::min(a.chunk > 0 ? a.chunk : 32768, 65535);
+    const int nb = std::min(chunk, a.n);
+    const FusedTables tb = fused_tables();
+    FusedLaunch c{};
+    c.ref = a.ref; c.ref_pitch = a.ref_pitch; c.sec = a.sec; c.sec_pitch = a.sec_pitch;
+    c.ref_d = a.ref_d; c.ref_a = a.ref_a; c.sec_d = a.sec_d; c.sec_a = a.sec_a;
+    c.loc = a.maxloc ? a.maxloc : buf<int32_t>("f_loc", size_t(a.n) * 2);
+    c.val = a.maxval ? a.maxval : buf<float>("f_val", a.n);
+    c.start = a.start ? a.start : buf<int32_t>("f_start", size_t(a.n) * 2);
+    c.shift = a.shift ? a.shift : buf<int32_t>("f_shift", size_t(a.n) * 2);
+    c.zidx = a.zoom_idx ? a.zoom_idx : buf<int32_t>("f_zidx", size_t(a.n) * 2);
+    c.tamp = buf<float>("f_tamp", size_t(nb) * T2 * T2);
+    c.tmean = buf<double>("f_tmean", 2 * size_t(nb));
+    c.samp = buf<float>("f_samp", size_t(nb) * P * P);
+    const bool want_margin = precision == 1 || a.margin_coarse || a.margin_fine;
+    c.mc = a.margin_coarse ? a.margin_coarse : (want_margin ? buf<float>("f_mc", a.n) : nullptr);
+    c.mf = a.margin_fine ? a.margin_fine : (want_margin ? buf<float>("f_mf", a.n) : nullptr);
+    c.offset = a.offset; c.snr = a.snr; c.cov = a.cov;
+    if (a.refined) {
+        c.surf = buf<float>("f_surf", size_t(nb) * 256);
+        c.refined = a.refined;
+    }
+    for (int w0 = 0; w0 < a.n; w0 += nb) launch_fused(tb, c, std::min(nb, a.n - w0), w0, a.stream);
+    if (precision == 1 && !a.nosync) rerun_fp64(a, c.mc, c.mf, c.start, c.shift);
 }
 
 }  // namespace ampcor

@@ -104,30 +104,12 @@ __device__ __forceinline__ double lane63_f64(double x) {
                             __builtin_amdgcn_readlane(__double2loint(x), 63));
 }
 
-template <int CPL, int NTH = NT, class Amp>
-__device__ __forceinline__ void window_norms(Amp amp, int SW, int TH, int TW, int NLh, int NLw, double t2, float *nrm,
-                             double *part) {
-    constexpr int NB = NORM_BLOCKS;
-    const int rows_per = (TH + NB - 1) / NB;
-    #pragma unroll 1
-    for (int t = threadIdx.x; t < NB * SW; t += NTH) {
-        const int b = t / SW, c = t - b * SW;
-        const int r0 = b * rows_per, r1 = min(TH, r0 + rows_per);
-        double a1 = 0.0, a2 = 0.0, b1 = 0.0, b2 = 0.0;
-        int r = r0;
-        for (; r + 1 < r1; r += 2) {
-            const double x = amp(r, c), y = amp(r + 1, c);
-            a1 += x; a2 += x * x;
-            b1 += y; b2 += y * y;
-        }
-        if (r < r1) {
-            const double x = amp(r, c);
-            a1 += x; a2 += x * x;
-        }
-        part[b * SW + c] = a1 + b1;
-        part[(NB + b) * SW + c] = a2 + b2;
-    }
-    __syncthreads();
+// Lag-row phase of window_norms: part holds, for NB row blocks, the column sums
+// of s ([b][SW]) then of s^2 ([NB + b][SW]) over rows [0, TH); amp(r, c) is read
+// for the slides only (rows u and u + TH, u < NLh - 1).
+template <int CPL, int NTH, class Amp>
+__device__ __forceinline__ void window_norms_rows(Amp amp, const double *part, int NB, int SW, int TH, int TW,
+                                                  int NLh, int NLw, double t2, float *nrm) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     constexpr int NW = NTH / 64;
     const int per = (NLh + NW - 1) / NW;
@@ -139,7 +121,6 @@ __device__ __forceinline__ void window_norms(Amp amp, int SW, int TH, int TW, in
         g1[m] = 0.0; g2[m] = 0.0;
         const int c = lane + 64 * m;
         if (c < SW) {
-#pragma unroll
             for (int b = 0; b < NB; ++b) {
                 g1[m] += part[b * SW + c];
                 g2[m] += part[(NB + b) * SW + c];
@@ -204,6 +185,33 @@ __device__ __forceinline__ void window_norms(Amp amp, int SW, int TH, int TW, in
             }
         }
     }
+}
+
+template <int CPL, int NTH = NT, class Amp>
+__device__ __forceinline__ void window_norms(Amp amp, int SW, int TH, int TW, int NLh, int NLw, double t2, float *nrm,
+                             double *part) {
+    constexpr int NB = NORM_BLOCKS;
+    const int rows_per = (TH + NB - 1) / NB;
+    #pragma unroll 1
+    for (int t = threadIdx.x; t < NB * SW; t += NTH) {
+        const int b = t / SW, c = t - b * SW;
+        const int r0 = b * rows_per, r1 = min(TH, r0 + rows_per);
+        double a1 = 0.0, a2 = 0.0, b1 = 0.0, b2 = 0.0;
+        int r = r0;
+        for (; r + 1 < r1; r += 2) {
+            const double x = amp(r, c), y = amp(r + 1, c);
+            a1 += x; a2 += x * x;
+            b1 += y; b2 += y * y;
+        }
+        if (r < r1) {
+            const double x = amp(r, c);
+            a1 += x; a2 += x * x;
+        }
+        part[b * SW + c] = a1 + b1;
+        part[(NB + b) * SW + c] = a2 + b2;
+    }
+    __syncthreads();
+    window_norms_rows<CPL, NTH>(amp, part, NB, SW, TH, TW, NLh, NLw, t2, nrm);
 }
 
 // ---------------------------------------------------------------------------

@@ -169,6 +169,35 @@ __global__ void k_gather(const void *img, int64_t pitch, const int32_t *sd, cons
     }
 }
 
+// No-sync bounds check: copy the start arrays ([4][n] out), replacing every
+// window whose chips leave their image by window (0, 0) and flagging it
+template <class I>
+__global__ void k_sanitize_starts(const I *rd, const I *ra, const I *sd, const I *sa, int n, int64_t ref_h,
+                                  int64_t ref_w, int64_t sec_h, int64_t sec_w, int wh, int ww, int sh, int sw,
+                                  I *out, int32_t *bad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t r0 = rd[i], a0 = ra[i], s0 = sd[i], t0 = sa[i];
+    const bool ok = r0 >= 0 && a0 >= 0 && r0 + wh <= ref_h && a0 + ww <= ref_w && s0 >= 0 && t0 >= 0 &&
+                    s0 + sh <= sec_h && t0 + sw <= sec_w && ref_h >= wh && ref_w >= ww && sec_h >= sh && sec_w >= sw;
+    out[i] = ok ? I(r0) : I(0);
+    out[n + i] = ok ? I(a0) : I(0);
+    out[2 * n + i] = ok ? I(s0) : I(0);
+    out[3 * n + i] = ok ? I(t0) : I(0);
+    bad[i] = ok ? 0 : 1;
+}
+
+// NaN outputs for the windows k_sanitize_starts flagged
+template <class T>
+__global__ void k_mark_bad(const int32_t *bad, int n, T *offset, T *snr, T *cov) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !bad[i]) return;
+    const T q = T(__builtin_nanf(""));
+    offset[2 * i] = q; offset[2 * i + 1] = q;
+    snr[i] = q;
+    cov[3 * i] = q; cov[3 * i + 1] = q; cov[3 * i + 2] = q;
+}
+
 // block-wide FP64 sum (256 threads)
 // block sum for blocks of up to 64 * (size of sh) threads
 __device__ __forceinline__ double block_sum256(double v, double *sh) {
@@ -907,6 +936,92 @@ static __global__ void k_subpixel(const int32_t *start, const int32_t *zidx, int
     if (w >= n) return;
     offset[2 * w] = ratio * float(zidx[2 * w]) + float(start[2 * w]) - float(hs_down);
     offset[2 * w + 1] = ratio * float(zidx[2 * w + 1]) + float(start[2 * w + 1]) - float(hs_across);
+}
+
+// ---------------------------------------------------------------------------
+// FP64 sub-pixel refinement of the oversampled-surface peak. No counterpart in
+// the reference, whose offset is the bare argmax on the 1/(surf_ovs*raw_ovs)
+// grid (cuOffset.cu:137-187). The surface is the FFT interpolation of the
+// trimmed Z x Z surface a (spectrum zero-padded to N = Z*ovs with the Nyquist
+// bin on the negative side, real part; cuOverSampler.cpp:141-148), so at any
+// integer (X, Y) of the N x N grid
+//   s(X, Y) = Re (1/Z^2) sum_{r,c} a[r][c] D(r, X) D(c, Y),
+//   D(r, X) = sum_{f=-Z/2}^{Z/2-1} e^{2 pi i f (r ovs - X)/N},
+// which is evaluated in FP64 at the peak (px, py) and its four axis neighbours
+// (wrapping, as the interpolation is periodic); a parabola through each axis'
+// three values gives the vertex d in [-1/2, 1/2], and the refined offset is
+// (p + d)/(ovs * raw_ovs) + start - hs. One wave per window; the trimmed
+// surface is the top-left Z x Z of surf ([n][ch][cw]); LDS: refine_lds(Z).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double parabola_vertex(double fm, double f0, double fp) {
+    const double den = fm - 2.0 * f0 + fp;
+    const double d = den < 0.0 ? 0.5 * (fm - fp) / den : 0.0;
+    return fmin(0.5, fmax(-0.5, d));
+}
+
+__host__ __device__ inline size_t refine_lds(int Z) { return sizeof(double) * (size_t(Z) * Z + 5) + 16 * 9 * size_t(Z); }
+
+template <class T>
+__global__ void __launch_bounds__(64) k_refine(const T *surf, int ch, int cw, int Z, int ovs, double ratio,
+                                               const int32_t *zidx, const int32_t *start, int hs_d, int hs_a,
+                                               const cx<double> *tw, double *out) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    cx<double> *D = reinterpret_cast<cx<double> *>(smem);    // [2 axes][3][Z]
+    cx<double> *B = D + 6 * Z;                                 // [3][Z]
+    double *a = reinterpret_cast<double *>(B + 3 * Z);         // [Z][Z]
+    double *s5 = a + Z * Z;
+    const int w = blockIdx.x, lane = threadIdx.x;
+    const int N = Z * ovs;
+    const T *src = surf + size_t(w) * ch * cw;
+    for (int i = lane; i < Z * Z; i += 64) a[i] = double(src[(i / Z) * cw + i % Z]);
+    const int px = zidx[2 * w], py = zidx[2 * w + 1];
+    // D(r, p - 1 + t) for both axes (e^{+2 pi i k/N} = conj(tw[k])); the table
+    // index f*m mod N advances by m from f = -Z/2 on
+    for (int i = lane; i < 6 * Z; i += 64) {
+        const int axis = i / (3 * Z), j = i - axis * 3 * Z, t = j / Z, r = j - t * Z;
+        const int X = (axis ? py : px) - 1 + t;
+        const int m = ((r * ovs - X) % N + N) % N;
+        int k = int((int64_t(N - Z / 2) * m) % N);
+        cx<double> acc{0.0, 0.0};
+        for (int f = 0; f < Z; ++f) {
+            acc.x += tw[k].x;
+            acc.y -= tw[k].y;
+            k += m;
+            if (k >= N) k -= N;
+        }
+        D[i] = acc;
+    }
+    __syncthreads();
+    // B[t][r] = sum_c a[r][c] D(c, py - 1 + t)
+    for (int i = lane; i < 3 * Z; i += 64) {
+        const int t = i / Z, r = i - t * Z;
+        const cx<double> *dy = D + (3 + t) * Z;
+        cx<double> acc{0.0, 0.0};
+        for (int c = 0; c < Z; ++c) {
+            acc.x += a[r * Z + c] * dy[c].x;
+            acc.y += a[r * Z + c] * dy[c].y;
+        }
+        B[i] = acc;
+    }
+    __syncthreads();
+    // the centre and its four axis neighbours: (tx, ty) = (1,1) (0,1) (2,1) (1,0) (1,2)
+    if (lane < 5) {
+        const int tx = lane == 1 ? 0 : (lane == 2 ? 2 : 1);
+        const int ty = lane == 3 ? 0 : (lane == 4 ? 2 : 1);
+        double acc = 0.0;
+        for (int r = 0; r < Z; ++r) {
+            const cx<double> d = D[tx * Z + r], b = B[ty * Z + r];
+            acc += d.x * b.x - d.y * b.y;
+        }
+        s5[lane] = acc / (double(Z) * Z);
+    }
+    __syncthreads();
+    if (lane == 0) {
+        const double dx = parabola_vertex(s5[1], s5[0], s5[2]);
+        const double dy = parabola_vertex(s5[3], s5[0], s5[4]);
+        out[2 * w] = (double(px) + dx) * ratio + double(start[2 * w]) - double(hs_d);
+        out[2 * w + 1] = (double(py) + dy) * ratio + double(start[2 * w + 1]) - double(hs_a);
+    }
 }
 
 }  // namespace ampcor

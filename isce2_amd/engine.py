@@ -83,12 +83,15 @@ class Engine:
         return self.sizes
 
     def run_windows(self, ref, sec, ref_d, ref_a, sec_d, sec_a, *, mode=MODE_AUTO, chunk=0,
-                    debug=False, stream=None, out=None, nosync=False):
+                    debug=False, refine=False, stream=None, out=None, nosync=False):
         """Run the per-window pipeline; every tensor argument lives on this engine's device.
 
         ``ref``/``sec`` are complex64 (or float32) 2-D tensors; start arrays are
         int32 1-D tensors of length n. Returns a dict of device tensors
         ``offset [n,2]``, ``snr [n]``, ``cov [n,3]`` (+ debug stage outputs).
+        ``refine`` adds ``offset_fp64 [n,2]`` (float64): the offset refined by
+        an FP64 parabola through the oversampled surface's peak (an extension;
+        ``offset`` stays the reference's grid value).
         ``out`` may supply any of these tensors. ``nosync`` makes the call free
         of host synchronization (device-side bounds check with NaN outputs for
         bad windows, no FP64 re-run), as needed under graph capture.
@@ -118,22 +121,27 @@ class Engine:
                   _ptr(offset), _ptr(snr), _ptr(cov)]
         if nosync:
             mode = int(mode) | FLAG_NOSYNC
-        if debug:
-            for k in ("maxloc", "start", "shift", "zoom_idx"):
-                res[k] = o[k] if k in o else torch.empty((n, 2), dtype=torch.int32, device=dev)
-            for k in ("maxval", "margin_coarse", "margin_fine"):
-                res[k] = o[k] if k in o else torch.full((n,), float("nan"), dtype=torch.float32, device=dev)
+        if refine:
+            res["offset_fp64"] = o["offset_fp64"] if "offset_fp64" in o else \
+                torch.empty((n, 2), dtype=torch.float64, device=dev)
+        if debug or refine:
+            if debug:
+                for k in ("maxloc", "start", "shift", "zoom_idx"):
+                    res[k] = o[k] if k in o else torch.empty((n, 2), dtype=torch.int32, device=dev)
+                for k in ("maxval", "margin_coarse", "margin_fine"):
+                    res[k] = o[k] if k in o else torch.full((n,), float("nan"), dtype=torch.float32, device=dev)
+            g = res.get
             code = self.lib.ampcor_run_windows_debug(
-                *common, _ptr(res["maxloc"]), _ptr(res["start"]), _ptr(res["shift"]),
-                _ptr(res["zoom_idx"]), _ptr(res["maxval"]), _ptr(res["margin_coarse"]),
-                _ptr(res["margin_fine"]), int(chunk), int(mode), st)
+                *common, _ptr(g("maxloc")), _ptr(g("start")), _ptr(g("shift")),
+                _ptr(g("zoom_idx")), _ptr(g("maxval")), _ptr(g("margin_coarse")),
+                _ptr(g("margin_fine")), _ptr(g("offset_fp64")), int(chunk), int(mode), st)
         else:
             code = self.lib.ampcor_run_windows(*common, int(chunk), int(mode), st)
         check(code, self._h)
         return res
 
     def run_windows_from_host(self, ref, sec, ref_d, ref_a, sec_d, sec_a, *, mode=MODE_AUTO, chunk=0,
-                              bands=8, threads=None):
+                              bands=8, threads=None, refine=False):
         """Run on host-resident images, overlapping their upload with the computation.
 
         ``ref``/``sec`` are 2-D NumPy arrays (memory maps included) or CPU
@@ -151,7 +159,7 @@ class Engine:
         FP64 offset, and SNR and covariance come from the FP64 run only where
         the raw-surface margin was small (so both methods return the same
         outputs). Returns host arrays ``offset [n,2]``,
-        ``snr [n]``, ``cov [n,3]``.
+        ``snr [n]``, ``cov [n,3]`` (and ``offset_fp64 [n,2]`` with ``refine``).
         """
         if self.params is None:
             raise RuntimeError("setup() must be called first")
@@ -182,6 +190,8 @@ class Engine:
         out = dict(offset=torch.empty((n, 2), dtype=torch.float32, device=dev),
                    snr=torch.empty(n, dtype=torch.float32, device=dev),
                    cov=torch.empty((n, 3), dtype=torch.float32, device=dev))
+        if refine:
+            out["offset_fp64"] = torch.empty((n, 2), dtype=torch.float64, device=dev)
         if n == 0:
             return {k: v.cpu().numpy() for k, v in out.items()}
         # the engine's bounds rule, checked here so that the bands run without host syncs
@@ -232,7 +242,7 @@ class Engine:
             comp.wait_event(events[b])
             if i1 > i0:
                 self.run_windows(d_ref, d_sec, *(s[i0:i1] for s in st), mode=mode, chunk=chunk, debug=rerun,
-                                 nosync=True, out={k: v[i0:i1] for k, v in out.items()})
+                                 refine=refine, nosync=True, out={k: v[i0:i1] for k, v in out.items()})
         if rerun:
             mc, mf = out.pop("margin_coarse").cpu().numpy(), out.pop("margin_fine").cpu().numpy()
             coarse = ~(mc >= self.tau[0])
@@ -242,13 +252,15 @@ class Engine:
                 self.set_precision(2)
                 try:
                     o = self.run_windows(d_ref, d_sec, *(s.index_select(0, t).contiguous() for s in st),
-                                         mode=MODE_STAGED)
+                                         mode=MODE_STAGED, refine=refine)
                 finally:
                     self.set_precision(1)
                 # the same rule as the engine's own re-run: a window whose raw-surface
                 # argmax is safe keeps its FP32 SNR and covariance (its raw peak cannot
                 # change) and takes only the FP64 offset; the others take every output
                 out["offset"].index_copy_(0, t, o["offset"])
+                if refine:
+                    out["offset_fp64"].index_copy_(0, t, o["offset_fp64"])
                 keep = torch.as_tensor(coarse[sel], device=dev)
                 for k in ("snr", "cov"):
                     cur = out[k].index_select(0, t)

@@ -525,14 +525,47 @@ def _run_batch(ref_img, sec_img, ref_d, ref_a, sec_d, sec_a, cfg, d, keep):
         surf = oversample_r2r(adj, d["surf_out"], d["surf_out"])
     zidx, zval = maxloc2d(surf)
     off = subpixel_offset(start, zidx, cfg.surf_ovs, ovs, cfg.hs_down, cfg.hs_across)
+    refined = None
+    if not cfg.surf_method:
+        refined = refine_offset(surf, zidx, start, cfg.surf_ovs, ovs, cfg.hs_down, cfg.hs_across)
     if keep:
         st.update(r_referenceBatchOverSampledSubMean=r_ref_os, c_secondaryBatchZoomIn=zoom_chip,
                   r_secondaryBatchOverSampled=r_sec_os, r_corrBatchZoomIn=corr_os,
                   r_corrBatchZoomInAdjust=adj, r_corrBatchZoomInOverSampled=surf)
     res = dict(offset=off, snr=snr, cov=cov, maxloc=loc, maxval=peak, start=start, shift=shift,
                zoom_idx=zidx, zoom_val=zval)
+    if refined is not None:
+        res["offset_fp64"] = refined
     res.update(st)
     return res
+
+
+# ---------------------------------------------------------------------------
+# FP64 sub-pixel refinement (no counterpart in the reference, whose offset is the
+# bare argmax on the 1/(surf_ovs*raw_ovs) grid, cuOffset.cu:137-187)
+# ---------------------------------------------------------------------------
+def parabolic_vertex(fm, f0, fp):
+    """Vertex of the parabola through (-1, fm), (0, f0), (1, fp), clamped to
+    [-1/2, 1/2]; 0 where the three values do not bend down."""
+    fm, f0, fp = (np.asarray(v, np.float64) for v in (fm, f0, fp))
+    den = fm - 2.0 * f0 + fp
+    with np.errstate(divide="ignore", invalid="ignore"):
+        d = np.where(den < 0.0, 0.5 * (fm - fp) / den, 0.0)
+    return np.clip(d, -0.5, 0.5)
+
+
+def refine_offset(surf, zidx, start, surf_ovs, raw_ovs, hs_down, hs_across):
+    """Offset refined by a parabola through the oversampled surface's peak and
+    its two neighbours along each axis, in FP64: ``(idx + d)/(surf_ovs*raw_ovs)
+    + start - hs``. The FFT-interpolated surface is periodic over its side, so
+    the neighbours of an edge peak wrap around."""
+    n, h, w = surf.shape
+    k = np.arange(n)
+    px, py = zidx[:, 0], zidx[:, 1]
+    dx = parabolic_vertex(surf[k, (px - 1) % h, py], surf[k, px, py], surf[k, (px + 1) % h, py])
+    dy = parabolic_vertex(surf[k, px, (py - 1) % w], surf[k, px, py], surf[k, px, (py + 1) % w])
+    r = 1.0 / (surf_ovs * raw_ovs)
+    return np.stack([(px + dx) * r + start[:, 0] - hs_down, (py + dy) * r + start[:, 1] - hs_across], axis=1)
 
 
 def gather_chips_batch(chips, start, h, w):

@@ -208,7 +208,9 @@ def test_launch_options_do_not_change_results(eng, scene1):
     rt, stt = torch.from_numpy(ref).to(dev), torch.from_numpy(sec).to(dev)
     st = [torch.as_tensor(np.asarray(s, np.int32)).to(dev) for s in starts]
     res = {}
+    eng.set_option("fine_kernel", 1)          # fine_threads shapes the whole-template fine kernel
     for mode, name, vals in ((MODE_FUSED, "fine_threads", (512, 1024)),
+                             (MODE_FUSED, "fine_kernel", (1, 2)),
                              (MODE_STAGED, "fft_block_elems", (1024, 2048, 4096, 8192)),
                              (MODE_STAGED, "fft_cols_width", (0, 16, 8))):
         for v in vals:
@@ -216,10 +218,11 @@ def test_launch_options_do_not_change_results(eng, scene1):
             r = eng.run_windows(rt, stt, *st, mode=mode, debug=True)
             torch.cuda.synchronize()
             res[(name, v)] = {k: t.cpu().numpy() for k, t in r.items()}
-        eng.set_option(name, {"fft_block_elems": 2048, "fine_threads": 1024, "fft_cols_width": 8}[name])
+        eng.set_option(name, {"fft_block_elems": 2048, "fine_threads": 1024, "fft_cols_width": 8,
+                              "fine_kernel": 2}[name])
     base_f, base_s = res[("fine_threads", 1024)], res[("fft_block_elems", 2048)]
     for (name, v), r in res.items():
-        base = base_f if name == "fine_threads" else base_s
+        base = base_f if name in ("fine_threads", "fine_kernel") else base_s
         for k in ("maxloc", "zoom_idx", "offset"):
             assert np.array_equal(r[k], base[k]), (name, v, k)
         np.testing.assert_allclose(r["snr"], base["snr"], rtol=1e-5)

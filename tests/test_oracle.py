@@ -167,6 +167,54 @@ def test_subpixel_offset_formula():
     np.testing.assert_array_equal(off, [[16 + 4 - 20, 3 + 40 / 32 - 20]])
 
 
+def test_parabolic_vertex_closed_form():
+    """A parabola sampled at -1, 0, 1 gives back its vertex exactly; values that
+    do not bend down give 0; the result is clamped to [-1/2, 1/2]."""
+    x0 = rng.uniform(-0.5, 0.5, 50)
+    a = rng.uniform(0.1, 3.0, 50)
+    c = rng.uniform(-1, 1, 50)
+    f = lambda x: c - a * (x - x0) ** 2  # noqa: E731
+    np.testing.assert_allclose(O.parabolic_vertex(f(-1.0), f(0.0), f(1.0)), x0, atol=1e-12)
+    assert O.parabolic_vertex(1.0, 1.0, 1.0) == 0.0 and O.parabolic_vertex(2.0, 1.0, 2.0) == 0.0
+    assert O.parabolic_vertex(0.0, 0.1, 5.0) == 0.0                 # bends up
+    assert O.parabolic_vertex(0.0, 1.0, 1.5) == 0.5                 # vertex at 1.5, clamped
+    assert O.parabolic_vertex(1.5, 1.0, 0.0) == -0.5
+
+
+def test_refine_offset_quadratic_surface():
+    """A separable quadratic peak anywhere on the 256x256 grid (an edge peak
+    wrapping around) is located exactly."""
+    X = np.arange(256)[:, None]
+    Y = np.arange(256)[None, :]
+    for x0, y0 in [(100.3, 57.8), (0.2, 255.4), (255.45, 3.0)]:
+        dx = (X - x0 + 128) % 256 - 128          # periodic distance
+        dy = (Y - y0 + 128) % 256 - 128
+        s = (1.0 - 0.003 * dx ** 2 - 0.002 * dy ** 2)[None]
+        zidx, _ = O.maxloc2d(s)
+        r = O.refine_offset(s, zidx, np.array([[3, 5]]), 16, 2, 4, 4)
+        want_x = (x0 if x0 < 255.5 else x0 - 256) / 32 + 3 - 4
+        want_y = (y0 if y0 < 255.5 else y0 - 256) / 32 + 5 - 4
+        # an edge peak's vertex can lie half a cell beyond the grid: compare modulo the side
+        gx = (r[0, 0] - 3 + 4) * 32
+        gy = (r[0, 1] - 5 + 4) * 32
+        assert min(abs(gx - x0), abs(gx - x0 + 256), abs(gx - x0 - 256)) < 1e-9, (r, want_x)
+        assert min(abs(gy - y0), abs(gy - y0 + 256), abs(gy - y0 - 256)) < 1e-9, (r, want_y)
+
+
+def test_refine_offset_band_limited_peak():
+    """A 16x16 sampled 2-D cosine is band-limited, so its FFT-oversampled
+    surface is the continuous cosine: the refined peak lands on the cosine's
+    maximum (input-sample position times 16) within 1e-3 of a fine cell,
+    where the bare argmax is off by up to half a cell."""
+    i = np.arange(16)
+    for x0, y0 in [(7.23, 8.61), (3.05, 12.97), (10.5, 0.49)]:
+        a = (np.cos(2 * np.pi * (i[:, None] - x0) / 16) + np.cos(2 * np.pi * (i[None, :] - y0) / 16))[None]
+        surf = O.oversample_r2r(a, 256, 256)
+        zidx, _ = O.maxloc2d(surf)
+        r = O.refine_offset(surf, zidx, np.zeros((1, 2), int), 16, 1, 0, 0)
+        assert abs(r[0, 0] * 16 - x0 * 16) < 1e-3 and abs(r[0, 1] * 16 - y0 * 16) < 1e-3, (r * 16, x0 * 16, y0 * 16)
+
+
 # ---------------------------------------------------------------------------
 # oversampling and deramp
 # ---------------------------------------------------------------------------
