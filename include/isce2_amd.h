@@ -137,6 +137,29 @@ int ampcor_run_windows_debug(ampcor_ctx *ctx,
                        double *out_offset_fp64,
                        int32_t chunk, int32_t mode, void *stream);
 
+/* Allocation-free variant of ampcor_run_windows. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * It calls no hipMalloc/hipFree and no allocating library routine: the fused
+ * kernels' intermediates live in one file-scope device array of 256 MiB per
+ * GPU (each GPU the library runs on holds it until the process ends), taken in
+ * chunks of up to ~1,800 windows at window 64 (per window: 145 KB of
+ * oversampled chips plus a few stage values). Any n is accepted; windows run
+ * chunk after chunk. The tables the kernels read are built by ampcor_setup.
+ * Covers the configurations of the fused kernels only (ampcor_sizes.fused == 1;
+ * otherwise AMPCOR_E_UNSUPPORTED). FP32 kernels only: the context's precision
+ * setting is not applied (no FP64 re-run). No host synchronization: window
+ * bounds are checked on the device, and a window whose chips leave their image
+ * gets NaN offset/SNR/covariance. Every call on one GPU shares that GPU's
+ * array, so calls that overlap on one GPU (any context, any stream) race:
+ * serialize them. Outputs equal ampcor_run_windows' under precision 0. */
+int ampcor_run_windows_noalloc(ampcor_ctx *ctx,
+                       const void *ref_img, int64_t ref_h, int64_t ref_w, int64_t ref_pitch,
+                       const void *sec_img, int64_t sec_h, int64_t sec_w, int64_t sec_pitch,
+                       const int32_t *ref_d, const int32_t *ref_a,
+                       const int32_t *sec_d, const int32_t *sec_a, int32_t n,
+                       float *out_offset, float *out_snr, float *out_cov, void *stream);
+
 /* Precision control. No counterpart in the reference (it runs FP32 only).
  *   mode 0: FP32 kernels only.
  *   mode 1 (default): FP32 kernels; windows whose raw-surface or oversampled-

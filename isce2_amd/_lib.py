@@ -26,7 +26,7 @@ EXPORTS = ("ampcor_create", "ampcor_destroy", "ampcor_setup", "ampcor_run_window
            "ampcor_run_windows_debug", "ampcor_set_precision", "ampcor_precision_stats",
            "ampcor_profile", "ampcor_profile_report", "ampcor_last_error", "ampcor_version",
            "ampcor_set_fft_radices", "ampcor_set_option", "ampcor_profile_only",
-           "ampcor_start_pixels", "ampcor_check_range")
+           "ampcor_start_pixels", "ampcor_check_range", "ampcor_run_windows_noalloc")
 
 
 class Params(C.Structure):
@@ -65,6 +65,7 @@ def load() -> C.CDLL:
     run_args = [vp, vp, i64, i64, i64, vp, i64, i64, i64, vp, vp, vp, vp, i32, vp, vp, vp]
     lib.ampcor_run_windows.argtypes = run_args + [i32, i32, vp]
     lib.ampcor_run_windows_debug.argtypes = run_args + [vp, vp, vp, vp, vp, vp, vp, vp, i32, i32, vp]
+    lib.ampcor_run_windows_noalloc.argtypes = run_args + [vp]
     lib.ampcor_set_precision.argtypes = [vp, i32, C.c_float, C.c_float]
     lib.ampcor_precision_stats.argtypes = [vp, C.POINTER(i64), C.POINTER(i64)]
     lib.ampcor_set_fft_radices.argtypes = [vp, i32, vp, i32]
@@ -82,7 +83,8 @@ def load() -> C.CDLL:
     for name in ("ampcor_create", "ampcor_destroy", "ampcor_setup", "ampcor_run_windows",
                  "ampcor_run_windows_debug", "ampcor_profile", "ampcor_profile_report",
                  "ampcor_set_precision", "ampcor_precision_stats", "ampcor_set_fft_radices",
-                 "ampcor_set_option", "ampcor_profile_only", "ampcor_start_pixels", "ampcor_check_range"):
+                 "ampcor_set_option", "ampcor_profile_only", "ampcor_start_pixels", "ampcor_check_range",
+                 "ampcor_run_windows_noalloc"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib

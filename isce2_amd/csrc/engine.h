@@ -66,6 +66,10 @@ struct FusedLaunch {
     double *refined = nullptr;     // [n][2] refined offsets (absolute), or null
 };
 
+// Address of this device's copy of the allocation-free entry's scratch array
+// (taken once per device, with that device current).
+void *noalloc_scratch_init(int device);
+
 struct Engine {
     int device = 0;
     ampcor_params p{};
@@ -159,6 +163,10 @@ struct Engine {
     void run_fused(const RunArgs &a);
     FusedTables fused_tables();
     void launch_fused(const FusedTables &tb, const FusedLaunch &c, int n, int w0, hipStream_t st);
+    // allocation-free fused run over the device's static scratch array
+    void *scratch_ptr = nullptr;
+    void run_fused_noalloc(const RunArgs &a);
+    void run_noalloc(const RunArgs &a);
 };
 
 // FFT schedule for length n (radices in pass order; direct DFT if a prime > 17 remains)

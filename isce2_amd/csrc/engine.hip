@@ -703,6 +703,32 @@ void Engine::run(const RunArgs &in) {
     HIP_OK(hipGetLastError());
 }
 
+// ampcor_run_windows_noalloc: the fused kernels over the device's static scratch
+// array, FP32 only, bounds checked on the device, no host synchronization; any
+// step that would allocate (a missing table) throws instead.
+void Engine::run_noalloc(const RunArgs &a) {
+    if (!configured) throw Error(AMPCOR_E_ARG, "ampcor_setup has not been called");
+    if (a.n < 0) throw Error(AMPCOR_E_ARG, "negative window count");
+    if (a.n == 0) return;
+    if (!a.ref || !a.sec || !a.ref_d || !a.ref_a || !a.sec_d || !a.sec_a || !a.offset || !a.snr || !a.cov)
+        throw Error(AMPCOR_E_ARG, "null pointer argument");
+    if (a.ref_pitch < a.ref_w || a.sec_pitch < a.sec_w) throw Error(AMPCOR_E_ARG, "pitch smaller than width");
+    if (!sz.fused)
+        throw Error(AMPCOR_E_UNSUPPORTED, "the allocation-free entry runs the fused kernels only, which do not "
+                                          "cover this configuration");
+    if (a.ref_h < p.window_h || a.ref_w < p.window_w || a.sec_h < sz.search_h || a.sec_w < sz.search_w)
+        throw Error(AMPCOR_E_RANGE, "image smaller than one chip");
+    HIP_OK(hipSetDevice(device));
+    struct Restore {
+        bool &f, &g;
+        bool v, w;
+        ~Restore() { f = v; g = w; }
+    } restore{no_alloc, profiling, no_alloc, profiling};
+    no_alloc = true;
+    profiling = false;
+    run_fused_noalloc(a);
+}
+
 }  // namespace ampcor
 
 // ---------------------------------------------------------------------------
@@ -744,9 +770,28 @@ int ampcor_create(int32_t device, ampcor_ctx **out) {
             throw ampcor::Error(AMPCOR_E_ARG, "device " + std::to_string(device) + " not available (" +
                                                   std::to_string(count) + " devices)");
         HIP_OK(hipSetDevice(device));
+        // the allocation-free entry's scratch address on this device (once per device)
+        void *scratch = ampcor::noalloc_scratch_init(device);
         auto *c = new ampcor_ctx();
         c->eng.device = device;
+        c->eng.scratch_ptr = scratch;
         *out = c;
+    });
+}
+
+int ampcor_run_windows_noalloc(ampcor_ctx *ctx, const void *ref_img, int64_t ref_h, int64_t ref_w,
+                               int64_t ref_pitch, const void *sec_img, int64_t sec_h, int64_t sec_w,
+                               int64_t sec_pitch, const int32_t *ref_d, const int32_t *ref_a, const int32_t *sec_d,
+                               const int32_t *sec_a, int32_t n, float *out_offset, float *out_snr, float *out_cov,
+                               void *stream) {
+    if (!ctx) return AMPCOR_E_ARG;
+    return guarded(ctx, [&] {
+        ampcor::Engine::RunArgs a{ref_img, ref_h, ref_w, ref_pitch, sec_img, sec_h, sec_w, sec_pitch,
+                                  ref_d, ref_a, sec_d, sec_a, n, out_offset, out_snr, out_cov,
+                                  nullptr, nullptr, nullptr, nullptr, nullptr,
+                                  0, AMPCOR_MODE_FUSED, static_cast<hipStream_t>(stream)};
+        a.nosync = true;
+        ctx->eng.run_noalloc(a);
     });
 }
 

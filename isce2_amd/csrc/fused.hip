@@ -878,4 +878,97 @@ void Engine::run_fused(const RunArgs &a) {
     if (precision == 1 && !a.nosync) rerun_fp64(a, c.mc, c.mf, c.start, c.shift);
 }
 
+// ---------------------------------------------------------------------------
+// Allocation-free entry (ampcor_run_windows_noalloc). Every intermediate of the
+// fused kernels lives in one file-scope device array; each GPU holds its own
+// copy until the process ends, and every call on that GPU shares it (calls that
+// overlap on one GPU race). Its address on each device is taken once, at context
+// creation, with that device current.
+// ---------------------------------------------------------------------------
+constexpr size_t kScratchBytes = size_t(256) << 20;   // 256 MiB per GPU
+// In host code `scratch` names a host-side stand-in, never the device array:
+// host code uses the address hipGetSymbolAddress returns for each device.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, void *> g_scratch;                      // device -> address of that device's copy
+
+size_t al256(size_t x) { return (x + 255) & ~size_t(255); }
+}  // namespace
+
+void *noalloc_scratch_init(int device) {
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch.find(device);
+    if (it != g_scratch.end()) return it->second;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        (void)hipGetLastError();
+        throw Error(AMPCOR_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    }
+    g_scratch[device] = ptr;
+    return ptr;
+}
+
+// Per-window bytes of the carve-up below: sanitized starts, bad flags, stage
+// outputs, margins, template mean, oversampled template and search amplitudes.
+static size_t noalloc_window_bytes(int T2, int P) {
+    return 16 + 4 + 8 * 4 + 4 + 8 + 16 + size_t(T2) * T2 * 4 + size_t(P) * P * 4;
+}
+
+void Engine::run_fused_noalloc(const RunArgs &a) {
+    if (!scratch_ptr) throw Error(AMPCOR_E_DEVICE, "the no-alloc scratch of this device is not set up");
+    const int W = p.window_w, Zc = sz.zoom_in_w;
+    const int T2 = 2 * W, P = 2 * Zc;
+    const size_t per = noalloc_window_bytes(T2, P);
+    // 11 arrays, each padded to 256 bytes
+    const size_t cap = (kScratchBytes - 11 * 256) / per;
+    if (cap < 1)
+        throw Error(AMPCOR_E_UNSUPPORTED, "one window needs " + std::to_string(per) + " bytes, more than the " +
+                                              std::to_string(kScratchBytes) + "-byte no-alloc scratch");
+    const int m = int(std::min<size_t>({cap, size_t(a.n), 65535}));
+    const FusedTables tb = fused_tables();                 // built by ampcor_setup; never allocates here
+    unsigned char *s = static_cast<unsigned char *>(scratch_ptr);
+    auto take = [&](size_t bytes) {
+        void *q = s;
+        s += al256(bytes);
+        return q;
+    };
+    int32_t *st4 = static_cast<int32_t *>(take(16 * size_t(m)));
+    int32_t *bad = static_cast<int32_t *>(take(4 * size_t(m)));
+    FusedLaunch c{};
+    c.ref = a.ref; c.ref_pitch = a.ref_pitch; c.sec = a.sec; c.sec_pitch = a.sec_pitch;
+    c.ref_d = st4; c.ref_a = st4 + m; c.sec_d = st4 + 2 * m; c.sec_a = st4 + 3 * m;
+    c.loc = static_cast<int32_t *>(take(8 * size_t(m)));
+    c.start = static_cast<int32_t *>(take(8 * size_t(m)));
+    c.shift = static_cast<int32_t *>(take(8 * size_t(m)));
+    c.zidx = static_cast<int32_t *>(take(8 * size_t(m)));
+    c.val = static_cast<float *>(take(4 * size_t(m)));
+    c.mc = static_cast<float *>(take(4 * size_t(m)));
+    c.mf = static_cast<float *>(take(4 * size_t(m)));
+    c.tmean = static_cast<double *>(take(16 * size_t(m)));
+    c.tamp = static_cast<float *>(take(size_t(m) * T2 * T2 * 4));
+    c.samp = static_cast<float *>(take(size_t(m) * P * P * 4));
+    if (size_t(s - static_cast<unsigned char *>(scratch_ptr)) > kScratchBytes)
+        throw Error(AMPCOR_E_UNSUPPORTED, "no-alloc scratch carve-up exceeds the array");
+    for (int w0 = 0; w0 < a.n; w0 += m) {
+        const int n = std::min(m, a.n - w0);
+        // this chunk's starts, bounds-checked on the device (out-of-image windows get NaN outputs)
+        hipLaunchKernelGGL(k_sanitize_starts<int32_t>, dim3(idivup(n, 256)), dim3(256), 0, a.stream, a.ref_d + w0,
+                           a.ref_a + w0, a.sec_d + w0, a.sec_a + w0, n, a.ref_h, a.ref_w, a.sec_h, a.sec_w,
+                           p.window_h, p.window_w, sz.search_h, sz.search_w, st4, bad);
+        if (n < m) {            // the sanitized arrays of a short last chunk start at the same offsets
+            c.ref_a = st4 + n; c.sec_d = st4 + 2 * n; c.sec_a = st4 + 3 * n;
+        }
+        c.offset = a.offset + 2 * size_t(w0);
+        c.snr = a.snr + w0;
+        c.cov = a.cov + 3 * size_t(w0);
+        launch_fused(tb, c, n, 0, a.stream);
+        hipLaunchKernelGGL(k_mark_bad<float>, dim3(idivup(n, 256)), dim3(256), 0, a.stream, bad, n, c.offset, c.snr,
+                           c.cov);
+        HIP_OK(hipGetLastError());
+    }
+}
+
 }  // namespace ampcor
