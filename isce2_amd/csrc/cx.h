@@ -23,12 +23,20 @@ template <class T> __host__ __device__ __forceinline__ cx<T> cmulc(cx<T> a, cx<T
     return {a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x};
 }
 template <class T> __host__ __device__ __forceinline__ cx<T> conj(cx<T> a) { return {a.x, -a.y}; }
+// acc + conj(a) * b as two chained FMAs per component (the accumulator is the addend)
+template <class T> __device__ __forceinline__ cx<T> cmac_conj(cx<T> acc, cx<T> a, cx<T> b) {
+    return {fma(a.y, b.y, fma(a.x, b.x, acc.x)), fma(-a.y, b.x, fma(a.x, b.y, acc.y))};
+}
 // multiply by -i (dir=-1) or +i (dir=+1)
 template <class T, int DIR> __host__ __device__ __forceinline__ cx<T> mul_i(cx<T> a) {
     return DIR > 0 ? cx<T>{-a.y, a.x} : cx<T>{a.y, -a.x};
 }
+// Hardware square root (v_sqrt_f32, within 1 ulp). The default sqrtf expands to a
+// correctly rounded sequence of ~20 instructions; amplitudes |z| use this one.
+__device__ __forceinline__ float fast_sqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 template <class T> __device__ __forceinline__ T amag(cx<T> a);
-template <> __device__ __forceinline__ float amag<float>(cx<float> a) { return sqrtf(a.x * a.x + a.y * a.y); }
+template <> __device__ __forceinline__ float amag<float>(cx<float> a) { return fast_sqrt(a.x * a.x + a.y * a.y); }
 template <> __device__ __forceinline__ double amag<double>(cx<double> a) { return sqrt(a.x * a.x + a.y * a.y); }
 
 __host__ __device__ __forceinline__ int idivup(int a, int b) { return (a + b - 1) / b; }

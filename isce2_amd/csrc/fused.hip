@@ -76,26 +76,51 @@ __global__ void __launch_bounds__(NT, (K >= 9 ? 2 : 4)) k_coarse(CoarseArgs a, c
     // their loads overlap the search chip's
     constexpr int TI = W * W / NT;
     static_assert(W * W % NT == 0, "template pixels divide evenly over the workgroup");
+    static_assert(NT % L == 0 && NT % W == 0, "fixed column per thread in the gathers");
     float2 traw[TI];
 #pragma unroll
     for (int k = 0; k < TI; ++k) {
         const int i = threadIdx.x + k * NT, r = i / W, x = i - r * W;
         traw[k] = pixel_raw(a.ref, a.mode, (rd + r) * a.ref_pitch + ra + x);
     }
-#pragma unroll 8
-    for (int i = threadIdx.x; i < nS * L; i += NT) {
-        const int j = i / L, x = i - j * L;
-        cf v = {0.f, 0.f};
-        if (x < SW) {
-            v.x = pixel_amp(a.sec, a.mode, (sd + 2 * j) * a.sec_pitch + sa + x);
-            if (2 * j + 1 < SH) v.y = pixel_amp(a.sec, a.mode, (sd + 2 * j + 1) * a.sec_pitch + sa + x);
+    {
+        // every thread keeps one column x of the search chip and walks its row
+        // pairs with a running pointer
+        const int x = threadIdx.x % L;
+        const int j0 = threadIdx.x / L;
+        constexpr int JS = NT / L;
+        const bool in = x < SW;
+        const int64_t step = int64_t(2 * JS) * a.sec_pitch;
+        int64_t off = (sd + 2 * j0) * a.sec_pitch + sa + (in ? x : 0);
+        auto put = [&](auto amp_at) {
+#pragma unroll 4
+            for (int j = j0; j < nS; j += JS, off += step) {
+                cf v = {0.f, 0.f};
+                if (in) {
+                    v.x = amp_at(off);
+                    if (2 * j + 1 < SH) v.y = amp_at(off + a.sec_pitch);
+                }
+                S[j * P + x] = v;
+            }
+        };
+        if (a.mode == 2) {
+            const float *img = static_cast<const float *>(a.sec);
+            put([&](int64_t o) { return fabsf(img[o]); });
+        } else {
+            const float2 *img = static_cast<const float2 *>(a.sec);
+            put([&](int64_t o) {
+                const float2 z = img[o];
+                return fast_sqrt(z.x * z.x + z.y * z.y);
+            });
         }
-        S[j * P + x] = v;
     }
+    // template amplitudes, held for the chunk loads below
+    float tamp[TI];
     double sum = 0.0, sum2 = 0.0;
 #pragma unroll
     for (int k = 0; k < TI; ++k) {
-        const double v = raw_amp(traw[k], a.mode);
+        tamp[k] = raw_amp(traw[k], a.mode);
+        const double v = tamp[k];
         sum += v;
         sum2 += v * v;
     }
@@ -118,16 +143,19 @@ __global__ void __launch_bounds__(NT, (K >= 9 ? 2 : 4)) k_coarse(CoarseArgs a, c
     for (int k = 0; k < K; ++k) q[k] = {0.f, 0.f};
     #pragma unroll 1
     for (int c0 = 0; c0 < W; c0 += TC) {
-        // template rows [c0, c0 + TC) minus the FP64 mean, packed in pairs
+        // template rows [c0, c0 + TC) minus the FP64 mean, packed in pairs, from
+        // the amplitudes held in registers; zero beyond column W
 #pragma unroll
-        for (int i = threadIdx.x; i < nTc * L; i += NT) {
-            const int j = i / L, x = i - j * L;
-            cf v = {0.f, 0.f};
-            if (x < W) {
-                v.x = float(double(pixel_amp(a.ref, a.mode, (rd + c0 + 2 * j) * a.ref_pitch + ra + x)) - mean);
-                v.y = float(double(pixel_amp(a.ref, a.mode, (rd + c0 + 2 * j + 1) * a.ref_pitch + ra + x)) - mean);
+        for (int k = 0; k < TI; ++k) {
+            const int i = threadIdx.x + k * NT, r = i / W, x = i - r * W;
+            if (r >= c0 && r < c0 + TC) {
+                float *slot = reinterpret_cast<float *>(Tb + ((r - c0) >> 1) * P + x) + (r & 1);
+                *slot = float(double(tamp[k]) - mean);
             }
-            Tb[j * P + x] = v;
+        }
+        for (int i = threadIdx.x; i < nTc * (L - W); i += NT) {
+            const int j = i / (L - W), x = W + i - j * (L - W);
+            Tb[j * P + x] = cf{0.f, 0.f};
         }
         __syncthreads();
         if (c0 == 0) {
@@ -211,16 +239,21 @@ __global__ void __launch_bounds__(NT, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversam
             C[r * PC + c] = pixel_cx(a.img, a.mode, (r0 + r) * a.pitch + c0 + c);
         }
     };
-    // deramp (float2's operator*= (float2.h:71-75): the imaginary part uses the new real part)
+    // deramp (float2's operator*= (float2.h:71-75): the imaginary part uses the new real part);
+    // the pixel's rotation factor is formed in FP64 and applied in FP32, as the chip is
     auto apply_deramp = [&]() {
         for (int i = threadIdx.x; i < N * N; i += NT) {
             const int r = i / N, c = i - r * N;
-            const cx<double> f = cmul(colf[c], rowf[r]);
+            const cx<double> f64 = cmul(colf[c], rowf[r]);
+            const float fx = float(f64.x), fy = float(f64.y);
             const cf z = C[r * PC + c];
-            const double nx = double(z.x) * f.x - double(z.y) * f.y;
-            C[r * PC + c] = {float(nx), float(double(z.y) * f.x + nx * f.y)};
+            const float nx = z.x * fx - z.y * fy;
+            C[r * PC + c] = {nx, z.y * fx + nx * fy};
         }
     };
+    double *psum = red + 16;             // [NT/64][2] per-wave FP64 sums of |.| and |.|^2 (MEAN)
+    static_assert(16 + 2 * (NT / 64) <= 32, "wave slots fit the reduction scratch");
+    if (MEAN && threadIdx.x < 2 * (NT / 64)) psum[threadIdx.x] = 0.0;
     gather();
     for (int i = threadIdx.x; i < N; i += NT) hsh[i] = a.hsh[i];
     __syncthreads();
@@ -253,16 +286,30 @@ __global__ void __launch_bounds__(NT, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversam
         __syncthreads();
     }
     float *out = a.out + size_t(win) * 4 * N * N;
-    double sum = 0.0, sum2 = 0.0;
+    // FP64 sums of |.| and |.|^2: each emit reduces its own over the wave and
+    // adds them to the wave's LDS slot, so no FP64 accumulator stays live
+    // through the transforms (the reference-chip kernel is held to 64 VGPRs)
     auto emit = [&](int p, int q) {      // |C| -> output pixels (2r+p, 2c+q)
+        double s1 = 0.0, s2 = 0.0;
         for (int i = threadIdx.x; i < N * N; i += NT) {
             const int r = i / N, c = i - r * N;
             const cf v = C[r * PC + c];
-            const float amp = sqrtf(v.x * v.x + v.y * v.y);
+            const float amp = fast_sqrt(v.x * v.x + v.y * v.y);
             out[size_t(2 * p + q) * N * N + r * N + c] = amp;     // whole lines per phase
             if (MEAN) {
-                sum += amp;
-                sum2 += double(amp) * amp;
+                s1 += amp;
+                s2 += double(amp) * amp;
+            }
+        }
+        if (MEAN) {
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                s1 += __shfl_xor(s1, o);
+                s2 += __shfl_xor(s2, o);
+            }
+            if ((threadIdx.x & 63) == 0) {
+                psum[2 * (threadIdx.x >> 6)] += s1;
+                psum[2 * (threadIdx.x >> 6) + 1] += s2;
             }
         }
     };
@@ -303,8 +350,13 @@ __global__ void __launch_bounds__(NT, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversam
     emit(1, 1);
     if (MEAN) {
         // FP64 mean and sum of squared deviations (sum2 - sum^2 / count)
-        sum = block_sum(sum, red);
-        sum2 = block_sum(sum2, red);
+        __syncthreads();
+        double sum = 0.0, sum2 = 0.0;
+#pragma unroll
+        for (int w = 0; w < NT / 64; ++w) {
+            sum += psum[2 * w];
+            sum2 += psum[2 * w + 1];
+        }
         const double cnt = double(4 * N * N);
         if (threadIdx.x == 0) {
             a.mean[2 * win] = sum / cnt;
@@ -434,7 +486,7 @@ __device__ __forceinline__ void fine_surface(const FineArgs &a, const float *cor
 #pragma unroll
         for (int k1 = 0; k1 < 16; ++k1) {
             const cf g = G[k1 * SO + threadIdx.x];
-            bq += sqrtf(g.x * g.x + g.y * g.y);
+            bq += fast_sqrt(g.x * g.x + g.y * g.y);
         }
         keep = !(bq * (1.0f + 0x1p-10f) + 0x1p-11f * amag < amax);
     }

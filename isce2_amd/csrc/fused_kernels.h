@@ -22,7 +22,7 @@ using cf = cx<float>;
 __device__ __forceinline__ float pixel_amp(const void *img, int mode, int64_t off) {
     if (mode == 2) return fabsf(static_cast<const float *>(img)[off]);
     const float2 z = static_cast<const float2 *>(img)[off];
-    return sqrtf(z.x * z.x + z.y * z.y);
+    return fast_sqrt(z.x * z.x + z.y * z.y);
 }
 
 // one pixel as loaded (real images: (value, 0)); raw_amp gives what pixel_amp gives
@@ -32,13 +32,13 @@ __device__ __forceinline__ float2 pixel_raw(const void *img, int mode, int64_t o
 }
 
 __device__ __forceinline__ float raw_amp(float2 z, int mode) {
-    return mode == 2 ? fabsf(z.x) : sqrtf(z.x * z.x + z.y * z.y);
+    return mode == 2 ? fabsf(z.x) : fast_sqrt(z.x * z.x + z.y * z.y);
 }
 
 __device__ __forceinline__ cf pixel_cx(const void *img, int mode, int64_t off) {
     if (mode == 2) return {static_cast<const float *>(img)[off], 0.f};
     const float2 z = static_cast<const float2 *>(img)[off];
-    if (mode == 1) return {sqrtf(z.x * z.x + z.y * z.y), 0.f};
+    if (mode == 1) return {fast_sqrt(z.x * z.x + z.y * z.y), 0.f};
     return {z.x, z.y};
 }
 
@@ -170,7 +170,9 @@ __device__ __forceinline__ void window_norms_rows(Amp amp, const double *part, i
             if (v < NLw) {
                 const double s1 = ph1 - pl1, s2 = ph2 - pl2;
                 const double norm2 = (s2 - s1 * s1 * inv_n) * t2;
-                nrm[u * NLw + v] = float(1.0 / sqrt(norm2 + double(FLT_EPSILON)));
+                // the cancellation is resolved in FP64; the reciprocal square root of
+                // the FP32 result is the hardware one (within 1 ulp)
+                nrm[u * NLw + v] = __builtin_amdgcn_rsqf(float(norm2 + double(FLT_EPSILON)));
             }
         }
         if (u + 1 < u_end) {
@@ -291,11 +293,12 @@ __device__ __forceinline__ void mac_rows(cf (&q)[K], const cf *S, int Sp, const 
                 // one branch-free form for every lane: conj(t) s, or for slot 0 the
                 // component-wise product (t.x s.x, t.y s.y)
                 const float c0 = tv.x, c1 = dc ? 0.f : tv.y, c2 = dc ? tv.y : tv.x, c3 = dc ? 0.f : -tv.y;
+                // two chained FMAs per component (the accumulator is the addend)
 #pragma unroll
                 for (int k = 0; k < K; ++k) {
                     const cf s = sw[i + k];
-                    q[k].x += c0 * s.x + c1 * s.y;
-                    q[k].y += c2 * s.y + c3 * s.x;
+                    q[k].x = fmaf(c1, s.y, fmaf(c0, s.x, q[k].x));
+                    q[k].y = fmaf(c3, s.x, fmaf(c2, s.y, q[k].y));
                 }
             }
         }

@@ -162,7 +162,7 @@ __global__ void k_gather(const void *img, int64_t pitch, const int32_t *sd, cons
         } else {
             const float2 z = static_cast<const float2 *>(img)[off];
             v = {T(z.x), T(z.y)};
-            if (mode == 1) v = {T(sqrtf(z.x * z.x + z.y * z.y)), T(0)};
+            if (mode == 1) v = {T(fast_sqrt(z.x * z.x + z.y * z.y)), T(0)};
         }
         o[i] = v;
         if (amp) amp[size_t(win) * tot + i] = amag(v);
@@ -360,7 +360,7 @@ __global__ void __launch_bounds__(64) k_mac_lags(const cx<T> *tspec, int th, con
         tw[0] = r - u0 < th ? ts[size_t(r - u0) * H1] : cx<T>{T(0), T(0)};
         const cx<T> sv = ss[size_t(r) * H1];
 #pragma unroll
-        for (int j = 0; j < K; ++j) acc[j] = acc[j] + cmulc(tw[j], sv);
+        for (int j = 0; j < K; ++j) acc[j] = cmac_conj(acc[j], tw[j], sv);
     }
 #pragma unroll
     for (int j = 0; j < K; ++j) {
@@ -888,7 +888,7 @@ __global__ void __launch_bounds__(256) k_surface(const T *corr, int ch, int cw, 
 #pragma unroll
         for (int k1 = 0; k1 < 16; ++k1) {
             const cx<T> g = G[k1 * 256 + threadIdx.x];
-            bq += sqrt(g.x * g.x + g.y * g.y);
+            bq += ampcor::amag(g);
         }
         keep = !(bq * T(1.0 + 0x1p-10) + T(0x1p-11) * amag < amax);
     }
