@@ -216,8 +216,8 @@ struct OsArgs {
     double *mean;                     // [n][2]: mean and sum of squared deviations of |.| (MEAN kernels)
 };
 
-// chip (pitch N+1), shift table, FP64 deramp row/column phase factors, reduction scratch
-__host__ __device__ inline int os_lds_bytes(int N) { return al16(N * (N + 1) * 8 + N * 8) + 2 * N * 16 + 32 * 8; }
+// chip (pitch N+1), shift table, deramp row/column rotation factors, reduction scratch
+__host__ __device__ inline int os_lds_bytes(int N) { return al16(N * (N + 1) * 8 + N * 8) + 2 * N * 8 + 32 * 8; }
 
 template <int R1, int R2, bool MEAN>
 __global__ void __launch_bounds__(NT, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversample(OsArgs a, const cf *tw, int w0) {
@@ -226,8 +226,8 @@ __global__ void __launch_bounds__(NT, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversam
     static_assert(2 * N <= NT, "one thread per deramp row and column factor");
     cf *C = reinterpret_cast<cf *>(smem);
     cf *hsh = C + N * PC;
-    cx<double> *rowf = reinterpret_cast<cx<double> *>(smem + al16(N * PC * 8 + N * 8));   // e^{i r phy}
-    cx<double> *colf = rowf + N;                                                         // e^{i c phx}
+    cf *rowf = reinterpret_cast<cf *>(smem + al16(N * PC * 8 + N * 8));   // e^{i r phy}
+    cf *colf = rowf + N;                                                  // e^{i c phx}
     double *red = reinterpret_cast<double *>(colf + N);
     const int win = w0 + xcd_window(blockIdx.x, gridDim.x);
     int64_t r0 = a.sd[win], c0 = a.sa[win];
@@ -240,15 +240,14 @@ __global__ void __launch_bounds__(NT, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversam
         }
     };
     // deramp (float2's operator*= (float2.h:71-75): the imaginary part uses the new real part);
-    // the pixel's rotation factor is formed in FP64 and applied in FP32, as the chip is
+    // the phases and their sines and cosines are FP64, the factors and the rotation FP32
     auto apply_deramp = [&]() {
         for (int i = threadIdx.x; i < N * N; i += NT) {
             const int r = i / N, c = i - r * N;
-            const cx<double> f64 = cmul(colf[c], rowf[r]);
-            const float fx = float(f64.x), fy = float(f64.y);
+            const cf f = cmul(colf[c], rowf[r]);
             const cf z = C[r * PC + c];
-            const float nx = z.x * fx - z.y * fy;
-            C[r * PC + c] = {nx, z.y * fx + nx * fy};
+            const float nx = z.x * f.x - z.y * f.y;
+            C[r * PC + c] = {nx, z.y * f.x + nx * f.y};
         }
     };
     double *psum = red + 16;             // [NT/64][2] per-wave FP64 sums of |.| and |.|^2 (MEAN)
@@ -276,10 +275,11 @@ __global__ void __launch_bounds__(NT, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversam
         axr = block_sum(axr, red); axi = block_sum(axi, red);
         const double phy = atan2(ayi, ayr), phx = atan2(axi, axr);
         // the phase c*phx + r*phy is separable: one FP64 sincos per row and per column
-        if (threadIdx.x < N) sincos(double(threadIdx.x) * phy, &rowf[threadIdx.x].y, &rowf[threadIdx.x].x);
-        else if (threadIdx.x - N < N) {
-            const int c = threadIdx.x - N;
-            sincos(double(c) * phx, &colf[c].y, &colf[c].x);
+        if (threadIdx.x < 2 * N) {
+            const int k = threadIdx.x < N ? threadIdx.x : threadIdx.x - N;
+            double sn, cs;
+            sincos(double(k) * (threadIdx.x < N ? phy : phx), &sn, &cs);
+            (threadIdx.x < N ? rowf : colf)[k] = {float(cs), float(sn)};
         }
         __syncthreads();
         apply_deramp();
@@ -289,17 +289,23 @@ __global__ void __launch_bounds__(NT, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversam
     // FP64 sums of |.| and |.|^2: each emit reduces its own over the wave and
     // adds them to the wave's LDS slot, so no FP64 accumulator stays live
     // through the transforms (the reference-chip kernel is held to 64 VGPRs)
-    auto emit = [&](int p, int q) {      // |C| -> output pixels (2r+p, 2c+q)
+    static_assert(N % 4 == 0, "four output columns per store");
+    auto emit = [&](int p, int q) {      // |C| -> output pixels (2r+p, 2c+q), four columns per thread
         double s1 = 0.0, s2 = 0.0;
-        for (int i = threadIdx.x; i < N * N; i += NT) {
-            const int r = i / N, c = i - r * N;
-            const cf v = C[r * PC + c];
-            const float amp = fast_sqrt(v.x * v.x + v.y * v.y);
-            out[size_t(2 * p + q) * N * N + r * N + c] = amp;     // whole lines per phase
-            if (MEAN) {
-                s1 += amp;
-                s2 += double(amp) * amp;
+        float *plane = out + size_t(2 * p + q) * N * N;           // whole lines per phase
+        for (int i = threadIdx.x; i < N * N / 4; i += NT) {
+            const int r = (4 * i) / N, c = 4 * i - r * N;
+            float amp[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const cf v = C[r * PC + c + m];
+                amp[m] = fast_sqrt(v.x * v.x + v.y * v.y);
+                if (MEAN) {
+                    s1 += amp[m];
+                    s2 += double(amp[m]) * amp[m];
+                }
             }
+            *reinterpret_cast<float4 *>(plane + r * N + c) = make_float4(amp[0], amp[1], amp[2], amp[3]);
         }
         if (MEAN) {
 #pragma unroll
@@ -696,23 +702,38 @@ __global__ void __launch_bounds__(NTH, 2) k_fine2(FineArgs a, const cf *tw, int 
     for (int ch = 0; ch < NCH; ++ch) {
         const int c0 = ch * TC;
         const int first = ch ? KEEP : 0;     // first pair slot loaded in this chunk
-        // search rows [c0 + 2 first, c0 + SROWS) into pair slots [first, SRP)
-#pragma unroll 4
-        for (int i = threadIdx.x; i < (SRP - first) * L; i += NTH) {
-            const int jj = first + i / L, x = i - (i / L) * L;
-            const int r = c0 + 2 * jj;
-            SR[jj * PP + x] = {sval(r, x), sval(r + 1, x)};
+        // search rows [c0 + 2 first, c0 + SROWS) into pair slots [first, SRP), eight
+        // columns per work item: four 16-byte loads (even and odd columns of both rows)
+        for (int i = threadIdx.x; i < (SRP - first) * (L / 8); i += NTH) {
+            const int jj = first + i / (L / 8), x0 = 8 * (i - (i / (L / 8)) * (L / 8));
+            const int r = c0 + 2 * jj;      // even: planes 0/1 hold row r, planes 2/3 row r + 1
+            const float *b = sg + (r >> 1) * Q + (x0 >> 1);
+            const float4 e0 = *reinterpret_cast<const float4 *>(b);
+            const float4 o0 = *reinterpret_cast<const float4 *>(b + Q * Q);
+            const float4 e1 = *reinterpret_cast<const float4 *>(b + 2 * Q * Q);
+            const float4 o1 = *reinterpret_cast<const float4 *>(b + 3 * Q * Q);
+            cf *d = SR + jj * PP + x0;
+            d[0] = {e0.x, e1.x}; d[1] = {o0.x, o1.x}; d[2] = {e0.y, e1.y}; d[3] = {o0.y, o1.y};
+            d[4] = {e0.z, e1.z}; d[5] = {o0.z, o1.z}; d[6] = {e0.w, e1.w}; d[7] = {o0.w, o1.w};
         }
         // template rows [c0, c0 + TC) minus the FP64 mean, zero beyond column T
-#pragma unroll 4
-        for (int i = threadIdx.x; i < TCP * L; i += NTH) {
-            const int j = i / L, x = i - j * L;
-            cf v = {0.f, 0.f};
-            if (x < T) {
-                const int r = c0 + 2 * j;
-                v = {float(double(tval(r, x)) - mean), float(double(tval(r + 1, x)) - mean)};
+        for (int i = threadIdx.x; i < TCP * (L / 8); i += NTH) {
+            const int j = i / (L / 8), x0 = 8 * (i - j * (L / 8));
+            cf *d = TB + j * PP + x0;
+            if (x0 < T) {
+                const float *b = tp + ((c0 >> 1) + j) * TQ + (x0 >> 1);
+                const float4 e0 = *reinterpret_cast<const float4 *>(b);
+                const float4 o0 = *reinterpret_cast<const float4 *>(b + TQ * TQ);
+                const float4 e1 = *reinterpret_cast<const float4 *>(b + 2 * TQ * TQ);
+                const float4 o1 = *reinterpret_cast<const float4 *>(b + 3 * TQ * TQ);
+                auto m = [&](float v) { return float(double(v) - mean); };
+                d[0] = {m(e0.x), m(e1.x)}; d[1] = {m(o0.x), m(o1.x)}; d[2] = {m(e0.y), m(e1.y)};
+                d[3] = {m(o0.y), m(o1.y)}; d[4] = {m(e0.z), m(e1.z)}; d[5] = {m(o0.z), m(o1.z)};
+                d[6] = {m(e0.w), m(e1.w)}; d[7] = {m(o0.w), m(o1.w)};
+            } else {
+#pragma unroll
+                for (int k = 0; k < 8; ++k) d[k] = cf{0.f, 0.f};
             }
-            TB[j * PP + x] = v;
         }
         __syncthreads();
         // column sums of the new rows below T; the first and last NS rows kept

@@ -19,28 +19,41 @@ pytestmark = pytest.mark.gpu
 
 
 def _oracle(groups):
+    """The oracle over each class at once: the class's chips stacked into one tall
+    reference and one tall secondary image, windows at (i*W, 0) and (i*S, 0)."""
     out = {}
     for (W, hs), g in groups.items():
         cfg = O.AmpcorConfig(window_h=W, window_w=W, hs_down=hs, hs_across=hs, deramp=1)
-        ref = g["ref"].cpu().numpy()
-        sec = g["sec"].cpu().numpy()
-        for j, i in enumerate(g["idx"]):
-            z = np.zeros(1, np.int64)
-            out[int(i)] = O.run_windows(ref[j], sec[j], z, z, z, z, cfg)
+        S = W + 2 * hs
+        m = len(g["idx"])
+        ref = g["ref"].cpu().numpy().reshape(m * W, W)
+        sec = g["sec"].cpu().numpy().reshape(m * S, S)
+        i = np.arange(m, dtype=np.int64)
+        z = np.zeros(m, np.int64)
+        o = O.run_windows(ref, sec, i * W, z, i * S, z, cfg)
+        for j, p in enumerate(g["idx"]):
+            out[int(p)] = {k: v[j:j + 1] for k, v in o.items()}
     return out
 
 
 @pytest.fixture(scope="module")
 def groups():
-    # 2 pairs per class on average; W=256 pairs dominate the oracle's time
-    return chip_pairs_uneven_grouped(n=30, seed=5, device=torch.device("cuda", 0))
+    # 300 pairs: at least 21 in each of the 12 (window, half search) classes
+    g = chip_pairs_uneven_grouped(n=300, seed=5, device=torch.device("cuda", 0))
+    assert len(g) == 12 and min(len(v["idx"]) for v in g.values()) >= 16
+    return g
 
 
-def test_uneven_batch_equals_oracle(groups):
+@pytest.fixture(scope="module")
+def oracle_out(groups):
+    return _oracle(groups)
+
+
+def test_uneven_batch_equals_oracle(groups, oracle_out):
     b = ChipPairBatch(groups, device=0, graph=True)
     b.run()
     r = b.results()
-    o = _oracle(groups)
+    o = oracle_out
     bad = []
     for i, oi in o.items():
         same = (np.array_equal(r["maxloc"][i], oi["maxloc"][0]) and np.array_equal(r["zoom_idx"][i], oi["zoom_idx"][0])
@@ -54,11 +67,11 @@ def test_uneven_batch_equals_oracle(groups):
     assert not bad, bad
 
 
-def test_forced_fp64_rerun_equals_oracle(groups):
+def test_forced_fp64_rerun_equals_oracle(groups, oracle_out):
     """tau = inf sends every pair through the per-class FP64 re-run (own stream and
     context per class, overlapping): results equal the oracle's, twice in a row."""
     b = ChipPairBatch(groups, device=0, graph=True, tau=float("inf"))
-    o = _oracle(groups)
+    o = oracle_out
     for rep in range(2):
         b.run()
         r = b.results()
