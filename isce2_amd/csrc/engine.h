@@ -152,6 +152,9 @@ struct Engine {
     int fft_block_elems = 2048;
     // columns per workgroup of the FP32 staged column FFTs (0: from fft_block_elems)
     int fft_cols_width = 8;
+    // staged x2 chip oversampling: 1 = quadrant-padded inverse 2-D FFT,
+    // 2 = separable half-sample interpolation per axis (equal in exact arithmetic)
+    int oversampler = 2;
     int precision = 1;
     float tau_coarse = 2e-6f, tau_fine = 2e-6f;
     long long prec_checked = 0, prec_rerun = 0;

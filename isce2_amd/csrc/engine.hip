@@ -302,6 +302,28 @@ void fft2_chip(Engine &e, const cx<T> *src, int ih, int iw, const int32_t *start
     fft_cols<T, DIR>(e.plan(h), dst, nimg, w, fft_elems<T>(e), st, cols_width<T>(e));
 }
 
+// x2 oversampled amplitude `out` ([nimg][2h][2w]) of h x w chips read from
+// [nimg][ih][iw] sources (cropped at `start`, deramped with `tab` when given),
+// by the separable interpolation (k_os2_cols, then k_os2_rows); `Y` holds the
+// [nimg][h][w] odd rows between the passes
+template <class T>
+void os2_separable(Engine &e, const cx<T> *src, int ih, int iw, const int32_t *start, const cx<double> *tab, int nimg,
+                   int h, int w, cx<T> *Y, T *out, hipStream_t st) {
+    const bool f32 = std::is_same<T, float>::value;
+    const FftPlan &pc = e.plan(h), &pr = e.plan(w);
+    const int elems = fft_elems<T>(e);
+    // up to 16 columns per block (whole 128-byte row segments) within 80 KB of LDS
+    const int cb = std::max(1, std::min(w, std::min(16, (81920 / int(h * sizeof(cx<T>)) - 1) / 2)));
+    hipLaunchKernelGGL(k_os2_cols<T>, dim3(idivup(w, cb), nimg), dim3(256), (2 * size_t(cb) + 1) * h * sizeof(cx<T>),
+                       st, src, ih, iw, start, tab, w, cb, f32 ? pc.desc32 : pc.desc64, Y);
+    // rows per block from the element budget, within 80 KB of LDS
+    const int row_bytes = w * int(2 * sizeof(cx<T>) + sizeof(T));
+    const int g = std::max(1, std::min(elems / w, (81920 - w * int(sizeof(cx<T>))) / row_bytes));
+    hipLaunchKernelGGL(k_os2_rows<T>, dim3(idivup(nimg * 2 * h, g)), dim3(256),
+                       (2 * size_t(g) + 1) * w * sizeof(cx<T>) + size_t(g) * w * sizeof(T), st, src, ih, iw, start, tab,
+                       Y, nimg, h, g, f32 ? pr.desc32 : pr.desc64, out);
+}
+
 // start[i], shift[i] = the given values for windows f0 <= i < n
 __global__ void k_given_starts(const int32_t *gs, const int32_t *gh, int f0, int n, int32_t *start, int32_t *shift) {
     const int i = f0 + blockIdx.x * blockDim.x + threadIdx.x;
@@ -366,6 +388,12 @@ void Engine::run_staged_t(const RunArgs &a) {
     cx<double> *dtab = p.deramp == 1 ? buf<cx<double>>(tn + "dtab", size_t(nb) * (std::max(W_h + W_w,
                                                                  sz.zoom_in_h + sz.zoom_in_w))) : nullptr;
     const int OSh = sz.os_search_h, OSw = sz.os_search_w;   // oversampled zoom chip (fixed by the interpolation)
+    // the separable oversampler covers the x2 interpolation of even-sided chips (an
+    // odd side loses its middle spectrum line to the quadrant padding, which does
+    // not factor into the half-sample form)
+    const bool sep_os = oversampler == 2 && p.raw_ovs == 2 && sz.os_win_h == 2 * W_h && sz.os_win_w == 2 * W_w &&
+                        OSh == 2 * sz.zoom_in_h && OSw == 2 * sz.zoom_in_w && W_h % 2 == 0 && W_w % 2 == 0 &&
+                        sz.zoom_in_h % 2 == 0 && sz.zoom_in_w % 2 == 0;
     cx<T> *c_sec_os = buf<cx<T>>(tn + "c_sec_os", size_t(nb) * OSh * OSw);
     T *r_sec_os = buf<T>(tn + "r_sec_os", size_t(nb) * OSh * OSw);
     T *corr_os = buf<T>(tn + "corr_os", size_t(nb) * sz.os_corr_h * sz.os_corr_w);
@@ -472,17 +500,27 @@ void Engine::run_staged_t(const RunArgs &a) {
         if (p.deramp == 1)
             hipLaunchKernelGGL(k_deramp_phase<T>, dim3(n), dim3(per_window_threads(W_h * W_w)), 0, st, c_ref, W_h, W_w,
                                (const int32_t *)nullptr, W_h, W_w, dtab);
-        fft2_chip<T, +1>(*this, c_ref, W_h, W_w, nullptr, p.deramp == 1 ? dtab : nullptr, c_ref, n, W_h, W_w, st);
-        fft2_quad<T, -1, 1>(*this, c_ref, n, W_h, W_w, c_ref_os, sz.os_win_h, sz.os_win_w, r_ref_os, st);
+        if (sep_os) {
+            os2_separable<T>(*this, c_ref, W_h, W_w, nullptr, p.deramp == 1 ? dtab : nullptr, n, W_h, W_w, c_ref_os,
+                             r_ref_os, st);
+        } else {
+            fft2_chip<T, +1>(*this, c_ref, W_h, W_w, nullptr, p.deramp == 1 ? dtab : nullptr, c_ref, n, W_h, W_w, st);
+            fft2_quad<T, -1, 1>(*this, c_ref, n, W_h, W_w, c_ref_os, sz.os_win_h, sz.os_win_w, r_ref_os, st);
+        }
         hipLaunchKernelGGL(k_sub_mean<T>, dim3(n), dim3(per_window_threads(sz.os_win_h * sz.os_win_w)), 0, st,
                            r_ref_os, sz.os_win_h * sz.os_win_w, ssd_os);
         // secondary zoom-in crop and oversampling (cuAmpcorChunk.cpp:134-137)
         if (p.deramp == 1)
             hipLaunchKernelGGL(k_deramp_phase<T>, dim3(n), dim3(per_window_threads(sz.zoom_in_h * sz.zoom_in_w)), 0,
                                st, c_sec, S_h, S_w, start, sz.zoom_in_h, sz.zoom_in_w, dtab);
-        fft2_chip<T, +1>(*this, c_sec, S_h, S_w, start, p.deramp == 1 ? dtab : nullptr, c_zoom, n, sz.zoom_in_h,
-                         sz.zoom_in_w, st);
-        fft2_quad<T, -1, 1>(*this, c_zoom, n, sz.zoom_in_h, sz.zoom_in_w, c_sec_os, OSh, OSw, r_sec_os, st);
+        if (sep_os) {
+            os2_separable<T>(*this, c_sec, S_h, S_w, start, p.deramp == 1 ? dtab : nullptr, n, sz.zoom_in_h,
+                             sz.zoom_in_w, c_sec_os, r_sec_os, st);
+        } else {
+            fft2_chip<T, +1>(*this, c_sec, S_h, S_w, start, p.deramp == 1 ? dtab : nullptr, c_zoom, n, sz.zoom_in_h,
+                             sz.zoom_in_w, st);
+            fft2_quad<T, -1, 1>(*this, c_zoom, n, sz.zoom_in_h, sz.zoom_in_w, c_sec_os, OSh, OSw, r_sec_os, st);
+        }
         // oversampled correlation + normalize (cuAmpcorChunk.cpp:148-161)
         if (row_os) {
             row_correlate(r_ref_os, sz.os_win_h, sz.os_win_w, r_sec_os, OSh, OSw, L2w, sz.os_corr_h, sz.os_corr_w,
@@ -853,6 +891,9 @@ int ampcor_set_option(ampcor_ctx *ctx, const char *name, int64_t value) {
         } else if (k == "fine_kernel") {
             if (value != 1 && value != 2) throw ampcor::Error(AMPCOR_E_ARG, "fine_kernel must be 1 or 2");
             ctx->eng.fine_kernel = int(value);
+        } else if (k == "oversampler") {
+            if (value != 1 && value != 2) throw ampcor::Error(AMPCOR_E_ARG, "oversampler must be 1 or 2");
+            ctx->eng.oversampler = int(value);
         } else if (k == "fine_threads") {
             if (value != 512 && value != 1024) throw ampcor::Error(AMPCOR_E_ARG, "fine_threads must be 512 or 1024");
             ctx->eng.fine_threads = int(value);

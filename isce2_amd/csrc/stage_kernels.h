@@ -739,6 +739,109 @@ __global__ void __launch_bounds__(256) k_fft_rows_chip(const cx<T> *src, int ih,
     for (int i = threadIdx.x; i < nr * w; i += blockDim.x) base[i] = res[i];
 }
 
+// ---------------------------------------------------------------------------
+// x2 chip oversampling, separable form. The quadrant-padded inverse 2-D FFT of
+// the reference's oversampler (cuOverSampler.cpp:72-78 with
+// cuArraysPadding.cu:59-103) is the tensor product of one 1-D interpolation
+// per axis, because the padding and both transforms factor by axis. Along an
+// axis of N samples, output sample 2m is input sample m, and output sample
+// 2m+1 is (1/N) FFT_-(FFT_+(x) e^{-i pi k_s/N})[m] with k_s = k for k < N/2 and
+// k - N otherwise (the padding puts bin N/2 among the negative frequencies).
+// Column pass (k_os2_cols): the odd output rows Y of each chip. Row pass
+// (k_os2_rows): every output row (chip row m or Y row m), |.| at even columns
+// and |.| of its interpolation at odd columns. Chips are read from
+// [nimg][ih][iw] sources, cropped at `start` when given and deramped with
+// `tab` when given, as in k_fft_rows_chip.
+// ---------------------------------------------------------------------------
+template <class T>
+__device__ __forceinline__ cx<T> half_shift(int k, int n) {
+    const int ks = k < n / 2 ? k : k - n;
+    double s, c;
+    sincospi(-double(ks) / double(n), &s, &c);
+    return {T(c / n), T(s / n)};
+}
+
+template <class T>
+__device__ __forceinline__ cx<T> chip_pixel(const cx<T> *src, int ih, int iw, const int32_t *start,
+                                            const cx<double> *tab, int win, int h, int w, int r, int c) {
+    cx<T> v = crop_origin(src, ih, iw, start, win)[r * iw + c];
+    if (tab) {
+        const cx<double> *rowf = tab + size_t(win) * (h + w);
+        v = deramp_apply(v, rowf[r], rowf[h + c]);
+    }
+    return v;
+}
+
+// `cb` chip columns per block (blockIdx.x), one chip per blockIdx.y; d.n = h;
+// Y: [nimg][h][w]
+template <class T>
+__global__ void __launch_bounds__(256) k_os2_cols(const cx<T> *src, int ih, int iw, const int32_t *start,
+                                                  const cx<double> *tab, int w, int cb, FftDesc d, cx<T> *Y) {
+    extern __shared__ unsigned char smem_raw[];
+    const int h = d.n;
+    cx<T> *a = reinterpret_cast<cx<T> *>(smem_raw);
+    cx<T> *b = a + cb * h;
+    cx<T> *sh = b + cb * h;
+    const int win = blockIdx.y, c0 = blockIdx.x * cb, nc = min(cb, w - c0);
+    for (int i = threadIdx.x; i < h * nc; i += blockDim.x) {
+        const int r = i / nc, c = i - r * nc;
+        a[r * cb + c] = chip_pixel(src, ih, iw, start, tab, win, h, w, r, c0 + c);
+    }
+    for (int k = threadIdx.x; k < h; k += blockDim.x) sh[k] = half_shift<T>(k, h);
+    __syncthreads();
+    cx<T> *X = fft_lds<T, +1>(a, b, d, nc, cb, 1);
+    cx<T> *other = X == a ? b : a;
+    for (int i = threadIdx.x; i < h * nc; i += blockDim.x) {
+        const int k = i / nc, c = i - k * nc;
+        X[k * cb + c] = cmul(X[k * cb + c], sh[k]);
+    }
+    __syncthreads();
+    const cx<T> *res = fft_lds<T, -1>(X, other, d, nc, cb, 1);
+    cx<T> *y = Y + size_t(win) * h * w + c0;
+    for (int i = threadIdx.x; i < h * nc; i += blockDim.x) {
+        const int r = i / nc, c = i - r * nc;
+        y[size_t(r) * w + c] = res[r * cb + c];
+    }
+}
+
+// `g` output rows per block over the nimg * 2h rows; d.n = w; out: [nimg][2h][2w]
+// real, |.| (OUT 1) of the oversampled chip
+template <class T>
+__global__ void __launch_bounds__(256) k_os2_rows(const cx<T> *src, int ih, int iw, const int32_t *start,
+                                                  const cx<double> *tab, const cx<T> *Y, int nimg, int h, int g,
+                                                  FftDesc d, T *out) {
+    extern __shared__ unsigned char smem_raw[];
+    const int w = d.n;
+    cx<T> *a = reinterpret_cast<cx<T> *>(smem_raw);
+    cx<T> *b = a + g * w;
+    cx<T> *sh = b + g * w;
+    T *amp = reinterpret_cast<T *>(sh + w);
+    const int t0 = blockIdx.x * g;
+    const int nr = min(g, nimg * 2 * h - t0);
+    for (int i = threadIdx.x; i < nr * w; i += blockDim.x) {
+        const int k = i / w, c = i - k * w;
+        const int t = t0 + k, win = t / (2 * h), tr = t - win * 2 * h, m = tr >> 1;
+        const cx<T> v = (tr & 1) ? Y[(size_t(win) * h + m) * w + c] : chip_pixel(src, ih, iw, start, tab, win, h, w, m, c);
+        a[i] = v;
+        amp[i] = amag(v);
+    }
+    for (int k = threadIdx.x; k < w; k += blockDim.x) sh[k] = half_shift<T>(k, w);
+    __syncthreads();
+    cx<T> *X = fft_lds<T, +1>(a, b, d, nr, 1, w);
+    cx<T> *other = X == a ? b : a;
+    for (int i = threadIdx.x; i < nr * w; i += blockDim.x) {
+        const int c = i % w;
+        X[i] = cmul(X[i], sh[c]);
+    }
+    __syncthreads();
+    const cx<T> *res = fft_lds<T, -1>(X, other, d, nr, 1, w);
+    T *o = out + size_t(t0) * 2 * w;
+    for (int i = threadIdx.x; i < nr * w; i += blockDim.x) {
+        const int k = i / w, c = i - k * w;
+        *reinterpret_cast<cx<T> *>(o + size_t(k) * 2 * w + 2 * c) = cx<T>{amp[i], amag(res[i])};
+    }
+}
+
 template <class T>
 __global__ void k_trim_complex(const T *in, int ih, int iw, cx<T> *out, int z) {
     const int win = blockIdx.y;
