@@ -153,7 +153,9 @@ struct Engine {
     // columns per workgroup of the FP32 staged column FFTs (0: from fft_block_elems)
     int fft_cols_width = 8;
     // staged x2 chip oversampling: 1 = quadrant-padded inverse 2-D FFT,
-    // 2 = separable half-sample interpolation per axis (equal in exact arithmetic)
+    // 2 = separable half-sample interpolation per axis (equal in exact
+    // arithmetic), in-place transforms for the common chip sides; 3 = the
+    // separable form with the general out-of-place transforms only
     int oversampler = 2;
     int precision = 1;
     float tau_coarse = 2e-6f, tau_fine = 2e-6f;

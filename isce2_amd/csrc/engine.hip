@@ -306,12 +306,79 @@ void fft2_chip(Engine &e, const cx<T> *src, int ih, int iw, const int32_t *start
 // [nimg][ih][iw] sources (cropped at `start`, deramped with `tab` when given),
 // by the separable interpolation (k_os2_cols, then k_os2_rows); `Y` holds the
 // [nimg][h][w] odd rows between the passes
+// in-place kernels for the chip sides of the common windows (32, 64, 128, 256
+// and their zoom chips, 8 pixels wider); false for other sides
+template <class T>
+const cx<T> *tw_of(Engine &e, int n) { return static_cast<const cx<T> *>(sizeof(T) == 4 ? e.plan(n).tw32.ptr : e.plan(n).tw64.ptr); }
+
+template <class T, int R1, int R2>
+void os2_cols_launch(Engine &e, const cx<T> *src, int ih, int iw, const int32_t *start, const cx<double> *tab,
+                     int nimg, int w, cx<T> *Y, hipStream_t st) {
+    constexpr int N = R1 * R2, CB = 16;
+    const size_t lds = (size_t(N) * (CB + 1) + N) * sizeof(cx<T>);
+    hipLaunchKernelGGL((k_os2_cols_f<T, R1, R2, CB>), dim3(idivup(w, CB), nimg), dim3(256), lds, st, src, ih, iw,
+                       start, tab, w, tw_of<T>(e, N), Y);
+}
+
+template <class T, int R1, int R2>
+void os2_rows_launch(Engine &e, const cx<T> *src, int ih, int iw, const int32_t *start, const cx<double> *tab,
+                     const cx<T> *Y, int nimg, int h, T *out, hipStream_t st) {
+    constexpr int N = R1 * R2, G = sizeof(T) == 4 ? 16 : 8;
+    const size_t lds = (size_t(G) * (N + 1) + N) * sizeof(cx<T>) + size_t(G) * N * sizeof(T);
+    hipLaunchKernelGGL((k_os2_rows_f<T, R1, R2, G>), dim3(idivup(nimg * 2 * h, G)), dim3(256), lds, st, src, ih, iw,
+                       start, tab, Y, nimg, h, tw_of<T>(e, N), out);
+}
+
+template <class T>
+bool os2_cols_fast(Engine &e, const cx<T> *src, int ih, int iw, const int32_t *start, const cx<double> *tab,
+                   int nimg, int h, int w, cx<T> *Y, hipStream_t st) {
+    switch (h) {
+        case 32: os2_cols_launch<T, 8, 4>(e, src, ih, iw, start, tab, nimg, w, Y, st); return true;
+        case 40: os2_cols_launch<T, 8, 5>(e, src, ih, iw, start, tab, nimg, w, Y, st); return true;
+        case 64: os2_cols_launch<T, 8, 8>(e, src, ih, iw, start, tab, nimg, w, Y, st); return true;
+        case 72: os2_cols_launch<T, 8, 9>(e, src, ih, iw, start, tab, nimg, w, Y, st); return true;
+        case 128: os2_cols_launch<T, 16, 8>(e, src, ih, iw, start, tab, nimg, w, Y, st); return true;
+        case 136: os2_cols_launch<T, 8, 17>(e, src, ih, iw, start, tab, nimg, w, Y, st); return true;
+        case 256: os2_cols_launch<T, 16, 16>(e, src, ih, iw, start, tab, nimg, w, Y, st); return true;
+        case 264: os2_cols_launch<T, 8, 33>(e, src, ih, iw, start, tab, nimg, w, Y, st); return true;
+        default: return false;
+    }
+}
+
+template <class T>
+bool os2_rows_fast(Engine &e, const cx<T> *src, int ih, int iw, const int32_t *start, const cx<double> *tab,
+                   const cx<T> *Y, int nimg, int h, int w, T *out, hipStream_t st) {
+    switch (w) {
+        case 32: os2_rows_launch<T, 8, 4>(e, src, ih, iw, start, tab, Y, nimg, h, out, st); return true;
+        case 40: os2_rows_launch<T, 8, 5>(e, src, ih, iw, start, tab, Y, nimg, h, out, st); return true;
+        case 64: os2_rows_launch<T, 8, 8>(e, src, ih, iw, start, tab, Y, nimg, h, out, st); return true;
+        case 72: os2_rows_launch<T, 8, 9>(e, src, ih, iw, start, tab, Y, nimg, h, out, st); return true;
+        case 128: os2_rows_launch<T, 16, 8>(e, src, ih, iw, start, tab, Y, nimg, h, out, st); return true;
+        case 136: os2_rows_launch<T, 8, 17>(e, src, ih, iw, start, tab, Y, nimg, h, out, st); return true;
+        case 256: os2_rows_launch<T, 16, 16>(e, src, ih, iw, start, tab, Y, nimg, h, out, st); return true;
+        case 264: os2_rows_launch<T, 8, 33>(e, src, ih, iw, start, tab, Y, nimg, h, out, st); return true;
+        default: return false;
+    }
+}
+
 template <class T>
 void os2_separable(Engine &e, const cx<T> *src, int ih, int iw, const int32_t *start, const cx<double> *tab, int nimg,
                    int h, int w, cx<T> *Y, T *out, hipStream_t st) {
     const bool f32 = std::is_same<T, float>::value;
     const FftPlan &pc = e.plan(h), &pr = e.plan(w);
     const int elems = fft_elems<T>(e);
+    if (e.oversampler == 2) {
+        const bool cols = os2_cols_fast<T>(e, src, ih, iw, start, tab, nimg, h, w, Y, st);
+        if (cols && os2_rows_fast<T>(e, src, ih, iw, start, tab, Y, nimg, h, w, out, st)) return;
+        if (cols) {
+            const int g = std::max(1, std::min(elems / w, (81920 - w * int(sizeof(cx<T>))) /
+                                                              (w * int(2 * sizeof(cx<T>) + sizeof(T)))));
+            hipLaunchKernelGGL(k_os2_rows<T>, dim3(idivup(nimg * 2 * h, g)), dim3(256),
+                               (2 * size_t(g) + 1) * w * sizeof(cx<T>) + size_t(g) * w * sizeof(T), st, src, ih, iw,
+                               start, tab, Y, nimg, h, g, f32 ? pr.desc32 : pr.desc64, out);
+            return;
+        }
+    }
     // up to 16 columns per block (whole 128-byte row segments) within 80 KB of LDS
     const int cb = std::max(1, std::min(w, std::min(16, (81920 / int(h * sizeof(cx<T>)) - 1) / 2)));
     hipLaunchKernelGGL(k_os2_cols<T>, dim3(idivup(w, cb), nimg), dim3(256), (2 * size_t(cb) + 1) * h * sizeof(cx<T>),
@@ -391,7 +458,7 @@ void Engine::run_staged_t(const RunArgs &a) {
     // the separable oversampler covers the x2 interpolation of even-sided chips (an
     // odd side loses its middle spectrum line to the quadrant padding, which does
     // not factor into the half-sample form)
-    const bool sep_os = oversampler == 2 && p.raw_ovs == 2 && sz.os_win_h == 2 * W_h && sz.os_win_w == 2 * W_w &&
+    const bool sep_os = oversampler >= 2 && p.raw_ovs == 2 && sz.os_win_h == 2 * W_h && sz.os_win_w == 2 * W_w &&
                         OSh == 2 * sz.zoom_in_h && OSw == 2 * sz.zoom_in_w && W_h % 2 == 0 && W_w % 2 == 0 &&
                         sz.zoom_in_h % 2 == 0 && sz.zoom_in_w % 2 == 0;
     cx<T> *c_sec_os = buf<cx<T>>(tn + "c_sec_os", size_t(nb) * OSh * OSw);
@@ -892,7 +959,7 @@ int ampcor_set_option(ampcor_ctx *ctx, const char *name, int64_t value) {
             if (value != 1 && value != 2) throw ampcor::Error(AMPCOR_E_ARG, "fine_kernel must be 1 or 2");
             ctx->eng.fine_kernel = int(value);
         } else if (k == "oversampler") {
-            if (value != 1 && value != 2) throw ampcor::Error(AMPCOR_E_ARG, "oversampler must be 1 or 2");
+            if (value < 1 || value > 3) throw ampcor::Error(AMPCOR_E_ARG, "oversampler must be 1, 2 or 3");
             ctx->eng.oversampler = int(value);
         } else if (k == "fine_threads") {
             if (value != 512 && value != 1024) throw ampcor::Error(AMPCOR_E_ARG, "fine_threads must be 512 or 1024");

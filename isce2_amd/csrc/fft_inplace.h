@@ -59,6 +59,83 @@ __device__ __forceinline__ void dft9(cx<T> *x) {
     }
 }
 
+// cos and sin of 2 pi m / R for m in [0, R), evaluated at compile time (Taylor
+// series after reduction to [-pi, pi]; 30 terms reach double precision there)
+namespace trig_detail {
+constexpr double kPi = 3.14159265358979323846;
+constexpr double reduce(double x) {
+    while (x > kPi) x -= 2 * kPi;
+    while (x < -kPi) x += 2 * kPi;
+    return x;
+}
+constexpr double csin(double x) {
+    x = reduce(x);
+    double term = x, sum = x;
+    for (int n = 1; n < 30; ++n) {
+        term *= -x * x / double((2 * n) * (2 * n + 1));
+        sum += term;
+    }
+    return sum;
+}
+constexpr double ccos(double x) {
+    x = reduce(x);
+    double term = 1.0, sum = 1.0;
+    for (int n = 1; n < 30; ++n) {
+        term *= -x * x / double((2 * n - 1) * (2 * n));
+        sum += term;
+    }
+    return sum;
+}
+}  // namespace trig_detail
+
+template <int R>
+struct RootTable {
+    double c[R], s[R];
+    constexpr RootTable() : c(), s() {
+        for (int m = 0; m < R; ++m) {
+            c[m] = trig_detail::ccos(2 * trig_detail::kPi * m / R);
+            s[m] = trig_detail::csin(2 * trig_detail::kPi * m / R);
+        }
+    }
+};
+
+// odd-length DFT by the conjugate symmetry of its kernel (any odd R): with
+// a_r = x_r + x_{R-r}, b_r = x_r - x_{R-r} (r = 1..M, M = (R-1)/2),
+//   X_q, X_{R-q} = x_0 + sum_r a_r cos(2 pi rq/R) +- DIR i sum_r b_r sin(2 pi rq/R)
+// (M^2 real multiply-adds per component instead of an FFT's R log R: cheap for
+// the radices 11..33 used here)
+template <class T, int DIR, int R>
+__device__ __forceinline__ void dft_oddc(cx<T> *x) {
+    constexpr int M = (R - 1) / 2;
+    constexpr RootTable<R> tg{};
+    cx<T> a[M + 1], b[M + 1];
+    const cx<T> x0 = x[0];
+    cx<T> sum = x0;
+#pragma unroll
+    for (int r = 1; r <= M; ++r) {
+        a[r] = x[r] + x[R - r];
+        b[r] = x[r] - x[R - r];
+        sum = sum + a[r];
+    }
+    x[0] = sum;
+#pragma unroll
+    for (int q = 1; q <= M; ++q) {
+        cx<T> A = x0, B = {T(0), T(0)};
+#pragma unroll
+        for (int r = 1; r <= M; ++r) {
+            const int m = (r * q) % R;
+            const T c = T(tg.c[m]), sn = T(tg.s[m]);
+            A.x = fma(a[r].x, c, A.x);
+            A.y = fma(a[r].y, c, A.y);
+            B.x = fma(b[r].x, sn, B.x);
+            B.y = fma(b[r].y, sn, B.y);
+        }
+        const cx<T> iB = {-T(DIR) * B.y, T(DIR) * B.x};
+        x[q] = A + iB;
+        x[R - q] = A - iB;
+    }
+}
+
 template <class T, int DIR, int R>
 __device__ __forceinline__ void dft_r(cx<T> *x) {
     if constexpr (R == 2) dft2<T, DIR>(x[0], x[1]);
@@ -68,6 +145,7 @@ __device__ __forceinline__ void dft_r(cx<T> *x) {
     else if constexpr (R == 8) dft8<T, DIR>(x);
     else if constexpr (R == 9) dft9<T, DIR>(x);
     else if constexpr (R == 16) dft16<T, DIR>(x);
+    else if constexpr (R % 2 == 1) dft_oddc<T, DIR, R>(x);
     else static_assert(R == 2, "unsupported radix");
 }
 

@@ -4,6 +4,7 @@
 #pragma once
 #include <cfloat>
 #include "cx.h"
+#include "fft_inplace.h"
 #include "fft_lds.h"
 
 namespace ampcor {
@@ -839,6 +840,85 @@ __global__ void __launch_bounds__(256) k_os2_rows(const cx<T> *src, int ih, int 
     for (int i = threadIdx.x; i < nr * w; i += blockDim.x) {
         const int k = i / w, c = i - k * w;
         *reinterpret_cast<cx<T> *>(o + size_t(k) * 2 * w + 2 * c) = cx<T>{amp[i], amag(res[i])};
+    }
+}
+
+// The same two passes for chip sides N = R1 * R2 with the transforms in place
+// (fft_nat2dr, then fft_dr2nat: the spectrum stays in digit-reversed order,
+// where the shift factors are applied), one chip line per LDS row of odd
+// pitch. Shift factors in that order:
+template <class T, int R1, int R2>
+__device__ __forceinline__ void half_shift_dr(cx<T> *sh) {
+    constexpr int N = R1 * R2;
+    for (int q = threadIdx.x; q < N; q += blockDim.x) {
+        const int k1 = q / R2, k2 = q - k1 * R2;     // storage q holds bin k1 + R1 k2
+        sh[q] = half_shift<T>(k1 + R1 * k2, N);
+    }
+}
+
+// CB chip columns per block (blockIdx.x), one chip per blockIdx.y; chip height
+// R1 * R2; Y: [nimg][R1 * R2][w]
+template <class T, int R1, int R2, int CB>
+__global__ void __launch_bounds__(256) k_os2_cols_f(const cx<T> *src, int ih, int iw, const int32_t *start,
+                                                    const cx<double> *tab, int w, const cx<T> *tw, cx<T> *Y) {
+    constexpr int N = R1 * R2, PC = CB + 1;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T> *C = reinterpret_cast<cx<T> *>(smem_raw);   // [N][PC]: column c at C[c + r * PC]
+    cx<T> *sh = C + N * PC;
+    const int win = blockIdx.y, c0 = blockIdx.x * CB, nc = min(CB, w - c0);
+    for (int i = threadIdx.x; i < N * CB; i += 256) {
+        const int r = i / CB, c = i - r * CB;
+        if (c < nc) C[r * PC + c] = chip_pixel(src, ih, iw, start, tab, win, N, w, r, c0 + c);
+    }
+    half_shift_dr<T, R1, R2>(sh);
+    __syncthreads();
+    fft_nat2dr<T, +1, R1, R2, 256>(C, nc, 1, PC, tw);
+    for (int i = threadIdx.x; i < N * nc; i += 256) {
+        const int q = i / nc, c = i - q * nc;
+        C[q * PC + c] = cmul(C[q * PC + c], sh[q]);
+    }
+    __syncthreads();
+    fft_dr2nat<T, -1, R1, R2, 256>(C, nc, 1, PC, tw);
+    cx<T> *y = Y + size_t(win) * N * w + c0;
+    for (int i = threadIdx.x; i < N * CB; i += 256) {
+        const int r = i / CB, c = i - r * CB;
+        if (c < nc) y[size_t(r) * w + c] = C[r * PC + c];
+    }
+}
+
+// G output rows per block over the nimg * 2h rows; chip width R1 * R2;
+// out: [nimg][2h][2 R1 R2] amplitudes
+template <class T, int R1, int R2, int G>
+__global__ void __launch_bounds__(256) k_os2_rows_f(const cx<T> *src, int ih, int iw, const int32_t *start,
+                                                    const cx<double> *tab, const cx<T> *Y, int nimg, int h,
+                                                    const cx<T> *tw, T *out) {
+    constexpr int N = R1 * R2, PR = N + 1;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T> *C = reinterpret_cast<cx<T> *>(smem_raw);   // [G][PR]
+    cx<T> *sh = C + G * PR;
+    T *amp = reinterpret_cast<T *>(sh + N);            // [G][N]
+    const int t0 = blockIdx.x * G;
+    const int nr = min(G, nimg * 2 * h - t0);
+    for (int i = threadIdx.x; i < nr * N; i += 256) {
+        const int k = i / N, c = i - k * N;
+        const int t = t0 + k, win = t / (2 * h), tr = t - win * 2 * h, m = tr >> 1;
+        const cx<T> v = (tr & 1) ? Y[(size_t(win) * h + m) * N + c] : chip_pixel(src, ih, iw, start, tab, win, h, N, m, c);
+        C[k * PR + c] = v;
+        amp[i] = amag(v);
+    }
+    half_shift_dr<T, R1, R2>(sh);
+    __syncthreads();
+    fft_nat2dr<T, +1, R1, R2, 256>(C, nr, PR, 1, tw);
+    for (int i = threadIdx.x; i < nr * N; i += 256) {
+        const int k = i / N, q = i - k * N;
+        C[k * PR + q] = cmul(C[k * PR + q], sh[q]);
+    }
+    __syncthreads();
+    fft_dr2nat<T, -1, R1, R2, 256>(C, nr, PR, 1, tw);
+    T *o = out + size_t(t0) * 2 * N;
+    for (int i = threadIdx.x; i < nr * N; i += 256) {
+        const int k = i / N, c = i - k * N;
+        *reinterpret_cast<cx<T> *>(o + size_t(k) * 2 * N + 2 * c) = cx<T>{amp[i], amag(C[k * PR + c])};
     }
 }
 

@@ -1,16 +1,18 @@
 """Staged x2 chip oversampling: separable half-sample form against the FP64 oracle.
 
 The staged pipeline oversamples even-sided chips by one half-sample
-interpolation per axis (option ``oversampler`` 2, the default) instead of the
-quadrant-padded inverse 2-D FFT the oracle restates (option 1). The two are
+interpolation per axis (option ``oversampler`` 2, the default: in-place
+transforms for the chip sides of windows 32/64/128/256 and their zoom chips;
+option 3: the general out-of-place transforms) instead of the
+quadrant-padded inverse 2-D FFT the oracle restates (option 1). They are
 equal in exact arithmetic, so:
 
 * the FP64 staged pipeline with the separable form equals the oracle on every
   integer result and offset (square and non-square even windows, deramp 0
   and 1, windows 32 to 128);
-* in FP32 the two forms give the same coarse peaks (the coarse stage does not
-  oversample) and the same fine peaks except where the fine argmax margin is
-  below the near-tie threshold.
+* in FP32 the separable forms give the 2-D form's coarse peaks (the coarse
+  stage does not oversample) and its fine peaks except where the fine argmax
+  margin is below the near-tie threshold.
 """
 import numpy as np
 import pytest
@@ -50,19 +52,21 @@ def _starts(cfg, shape, n=6):
     return rd, ra, sd, sa
 
 
+@pytest.mark.parametrize("form", [2, 3])
 @pytest.mark.parametrize("case", CASES)
-def test_fp64_separable_equals_oracle(eng, scene, case):
+def test_fp64_separable_equals_oracle(eng, scene, case, form):
     wh, ww, hd, ha, deramp = case
     ref, sec = scene
     cfg = O.AmpcorConfig(window_h=wh, window_w=ww, hs_down=hd, hs_across=ha, deramp=deramp)
-    eng.set_option("oversampler", 2)
+    eng.set_option("oversampler", form)
     eng.set_precision(2)
     try:
         g, o = run_both(eng, ref, sec, cfg, _starts(cfg, ref.shape), MODE_STAGED)
     finally:
         eng.set_precision(1)
+        eng.set_option("oversampler", 2)
     c = compare(g, o)
-    print(f"\n[separable fp64 {case}] {c}", flush=True)
+    print(f"\n[separable fp64 form={form} {case}] {c}", flush=True)
     assert c["n"] >= 16
     assert c["maxloc_mismatch"] == 0 and c["zoom_mismatch"] == 0, c
     assert c["offset_max_err"] == 0.0, c
@@ -82,7 +86,7 @@ def test_fp32_forms_agree_up_to_ties(eng, scene, case):
     out = {}
     eng.set_precision(0)
     try:
-        for form in (1, 2):
+        for form in (1, 2, 3):
             eng.set_option("oversampler", form)
             r = eng.run_windows(ref_t, sec_t, *st, mode=MODE_STAGED, debug=True)
             torch.cuda.synchronize()
@@ -90,12 +94,13 @@ def test_fp32_forms_agree_up_to_ties(eng, scene, case):
     finally:
         eng.set_option("oversampler", 2)
         eng.set_precision(1)
-    a, b = out[1], out[2]
-    assert np.array_equal(a["maxloc"], b["maxloc"])
-    flip = np.any(a["zoom_idx"] != b["zoom_idx"], axis=1)
-    print(f"\n[fp32 forms {case}] n={len(flip)} fine_flips={int(flip.sum())} "
-          f"max_flip_margin={float(np.maximum(a['margin_fine'], b['margin_fine'])[flip].max()) if flip.any() else None}",
-          flush=True)
-    assert np.all(np.minimum(a["margin_fine"], b["margin_fine"])[flip] < TAU)
-    same = ~flip
-    assert np.array_equal(a["offset"][same], b["offset"][same])
+    a = out[1]
+    for form in (2, 3):
+        b = out[form]
+        assert np.array_equal(a["maxloc"], b["maxloc"])
+        flip = np.any(a["zoom_idx"] != b["zoom_idx"], axis=1)
+        print(f"\n[fp32 forms 1 vs {form} {case}] n={len(flip)} fine_flips={int(flip.sum())} max_flip_margin="
+              f"{float(np.maximum(a['margin_fine'], b['margin_fine'])[flip].max()) if flip.any() else None}", flush=True)
+        assert np.all(np.minimum(a["margin_fine"], b["margin_fine"])[flip] < TAU)
+        same = ~flip
+        assert np.array_equal(a["offset"][same], b["offset"][same])
