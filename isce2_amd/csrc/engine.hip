@@ -391,6 +391,65 @@ void os2_separable(Engine &e, const cx<T> *src, int ih, int iw, const int32_t *s
                        Y, nimg, h, g, f32 ? pr.desc32 : pr.desc64, out);
 }
 
+// Row-spectrum transform lengths with an in-place kernel (k_rowspec_f, FP32):
+// L = R1 * R2, listed by L. The correlation is the same for any L of at least
+// the search width; these cover the uneven windows' search widths with little
+// padding and prime factors <= 17 (which the lag-row inverse needs).
+struct RowLen { int L, R1, R2; };
+constexpr RowLen kRowLens[] = {{72, 8, 9},    {80, 16, 5},   {104, 8, 13},  {128, 16, 8},  {144, 16, 9},
+                               {168, 8, 21},  {195, 13, 15}, {256, 16, 16}, {272, 16, 17}, {289, 17, 17},
+                               {325, 13, 25}, {528, 16, 33}};
+
+// smallest listed length in [sw, sw + 48], or 0
+int row_len_fast(int sw) {
+    for (const RowLen &r : kRowLens)
+        if (r.L >= sw && r.L <= sw + 48) return r.L;
+    return 0;
+}
+
+template <int R1, int R2>
+void rowspec_launch(Engine &e, const float *img, int nimg, int h, int w, cx<float> *spec, hipStream_t st) {
+    constexpr int L = R1 * R2, G = 16;
+    hipLaunchKernelGGL((k_rowspec_f<float, R1, R2, G>), dim3(idivup(nimg * ((h + 1) / 2), G)), dim3(256),
+                       size_t(G) * (L + 1) * sizeof(cx<float>), st, img, nimg, h, w, tw_of<float>(e, L), spec);
+}
+
+void rowspec_fast(Engine &e, int L, const float *img, int nimg, int h, int w, cx<float> *spec, hipStream_t st) {
+    switch (L) {
+        case 72: rowspec_launch<8, 9>(e, img, nimg, h, w, spec, st); break;
+        case 80: rowspec_launch<16, 5>(e, img, nimg, h, w, spec, st); break;
+        case 104: rowspec_launch<8, 13>(e, img, nimg, h, w, spec, st); break;
+        case 128: rowspec_launch<16, 8>(e, img, nimg, h, w, spec, st); break;
+        case 144: rowspec_launch<16, 9>(e, img, nimg, h, w, spec, st); break;
+        case 168: rowspec_launch<8, 21>(e, img, nimg, h, w, spec, st); break;
+        case 195: rowspec_launch<13, 15>(e, img, nimg, h, w, spec, st); break;
+        case 256: rowspec_launch<16, 16>(e, img, nimg, h, w, spec, st); break;
+        case 272: rowspec_launch<16, 17>(e, img, nimg, h, w, spec, st); break;
+        case 289: rowspec_launch<17, 17>(e, img, nimg, h, w, spec, st); break;
+        case 325: rowspec_launch<13, 25>(e, img, nimg, h, w, spec, st); break;
+        case 528: rowspec_launch<16, 33>(e, img, nimg, h, w, spec, st); break;
+        default: throw Error(AMPCOR_E_ARG, "no in-place row transform of length " + std::to_string(L));
+    }
+}
+
+// per-window copies of a chunk's outputs (pairs of int32, one T -> float); a
+// null destination or source skips that output
+template <class T>
+__global__ void k_chunk_out(int n, const int32_t *loc, int32_t *o_loc, const int32_t *start, int32_t *o_start,
+                            const int32_t *shift, int32_t *o_shift, const int32_t *zidx, int32_t *o_zidx, const T *val,
+                            float *o_val) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    auto pair = [&](const int32_t *s, int32_t *d) {
+        if (s && d) { d[2 * i] = s[2 * i]; d[2 * i + 1] = s[2 * i + 1]; }
+    };
+    pair(loc, o_loc);
+    pair(start, o_start);
+    pair(shift, o_shift);
+    pair(zidx, o_zidx);
+    if (val && o_val) o_val[i] = float(val[i]);
+}
+
 // start[i], shift[i] = the given values for windows f0 <= i < n
 __global__ void k_given_starts(const int32_t *gs, const int32_t *gh, int f0, int n, int32_t *start, int32_t *shift) {
     const int i = f0 + blockIdx.x * blockDim.x + threadIdx.x;
@@ -470,18 +529,22 @@ void Engine::run_staged_t(const RunArgs &a) {
     // for shapes whose row pairs outnumber the transform rows.
     auto fits_rows = [&](int th, int sh, int Lh) { return (th + 1) / 2 + (sh + 1) / 2 <= Lh; };
     const bool row_raw = fits_rows(W_h, S_h, L1h), row_os = fits_rows(sz.os_win_h, OSh, L2h);
+    // row transform lengths: an in-place length for the FP32 passes where one is listed
+    const bool f32 = sizeof(T) == 4;
+    const int R1w = f32 && row_len_fast(S_w) ? row_len_fast(S_w) : L1w;
+    const int R2w = f32 && row_len_fast(OSw) ? row_len_fast(OSw) : L2w;
     size_t zmax = 0;
     if (!row_raw) zmax = size_t(L1h) * L1w;
     if (!row_os) zmax = std::max(zmax, size_t(L2h) * L2w);
     cx<T> *zc = zmax ? buf<cx<T>>(tn + "zc", size_t(nb) * zmax) : nullptr;
     size_t spec_elems = 0, q_elems = 0;
     if (row_raw) {
-        spec_elems = std::max(spec_elems, size_t(W_h + S_h) * (L1w / 2 + 1));
-        q_elems = std::max(q_elems, size_t(sz.raw_corr_h) * L1w);
+        spec_elems = std::max(spec_elems, size_t(W_h + S_h) * (R1w / 2 + 1));
+        q_elems = std::max(q_elems, size_t(sz.raw_corr_h) * R1w);
     }
     if (row_os) {
-        spec_elems = std::max(spec_elems, size_t(sz.os_win_h + OSh) * (L2w / 2 + 1));
-        q_elems = std::max(q_elems, size_t(sz.os_corr_h) * L2w);
+        spec_elems = std::max(spec_elems, size_t(sz.os_win_h + OSh) * (R2w / 2 + 1));
+        q_elems = std::max(q_elems, size_t(sz.os_corr_h) * R2w);
     }
     cx<T> *rspec = spec_elems ? buf<cx<T>>(tn + "rspec", size_t(nb) * spec_elems) : nullptr;
     cx<T> *rq = q_elems ? buf<cx<T>>(tn + "rq", size_t(nb) * q_elems) : nullptr;
@@ -493,10 +556,15 @@ void Engine::run_staged_t(const RunArgs &a) {
         const int g = std::max(1, fft_elems<T>(*this) / L);
         const size_t lds = 2 * size_t(g) * L * sizeof(cx<T>);
         cx<T> *tspec = rspec, *sspec = rspec + size_t(n) * th * H1;
-        hipLaunchKernelGGL(k_rowspec<T>, dim3(idivup(n * ((th + 1) / 2), g)), dim3(256), lds, st, tm, n, th, tw, g, d,
-                           tspec);
-        hipLaunchKernelGGL(k_rowspec<T>, dim3(idivup(n * ((sh + 1) / 2), g)), dim3(256), lds, st, sr, n, sh, sw, g, d,
-                           sspec);
+        if (sizeof(T) == 4 && row_len_fast(sw) == L) {
+            rowspec_fast(*this, L, reinterpret_cast<const float *>(tm), n, th, tw, reinterpret_cast<cx<float> *>(tspec), st);
+            rowspec_fast(*this, L, reinterpret_cast<const float *>(sr), n, sh, sw, reinterpret_cast<cx<float> *>(sspec), st);
+        } else {
+            hipLaunchKernelGGL(k_rowspec<T>, dim3(idivup(n * ((th + 1) / 2), g)), dim3(256), lds, st, tm, n, th, tw, g,
+                               d, tspec);
+            hipLaunchKernelGGL(k_rowspec<T>, dim3(idivup(n * ((sh + 1) / 2), g)), dim3(256), lds, st, sr, n, sh, sw, g,
+                               d, sspec);
+        }
         // lags per thread: 17 once there is more than one group of 9 (each group
         // re-reads every spectrum; the sums themselves do not depend on the grouping)
         if (NL > 9)
@@ -538,7 +606,7 @@ void Engine::run_staged_t(const RunArgs &a) {
                                ssd_raw);
             // raw correlation (cuAmpcorChunk.cpp:48-52)
             if (row_raw) {
-                row_correlate(r_ref, W_h, W_w, r_sec, S_h, S_w, L1w, sz.raw_corr_h, sz.raw_corr_w, corr, n);
+                row_correlate(r_ref, W_h, W_w, r_sec, S_h, S_w, R1w, sz.raw_corr_h, sz.raw_corr_w, corr, n);
             } else {
                 hipLaunchKernelGGL(k_pack_corr<T>, grid2(L1h * L1w, n), dim3(256), 0, st, r_ref, W_h, W_w, r_sec,
                                    S_h, S_w, zc, L1h, L1w);
@@ -590,7 +658,7 @@ void Engine::run_staged_t(const RunArgs &a) {
         }
         // oversampled correlation + normalize (cuAmpcorChunk.cpp:148-161)
         if (row_os) {
-            row_correlate(r_ref_os, sz.os_win_h, sz.os_win_w, r_sec_os, OSh, OSw, L2w, sz.os_corr_h, sz.os_corr_w,
+            row_correlate(r_ref_os, sz.os_win_h, sz.os_win_w, r_sec_os, OSh, OSw, R2w, sz.os_corr_h, sz.os_corr_w,
                           corr_os, n);
         } else {
             hipLaunchKernelGGL(k_pack_corr<T>, grid2(L2h * L2w, n), dim3(256), 0, st, r_ref_os, sz.os_win_h,
@@ -634,11 +702,13 @@ void Engine::run_staged_t(const RunArgs &a) {
                                Z, p.surf_ovs, 1.0 / double(p.surf_ovs * p.raw_ovs), zidx, start, p.hs_down,
                                p.hs_across, static_cast<const cx<double> *>(plan(So).tw64.ptr),
                                a.refined + 2 * size_t(w0));
-        if (a.maxloc && !fine_only) HIP_OK(hipMemcpyAsync(a.maxloc + 2 * size_t(w0), loc, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToDevice, st));
-        if (a.start) HIP_OK(hipMemcpyAsync(a.start + 2 * size_t(w0), start, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToDevice, st));
-        if (a.shift) HIP_OK(hipMemcpyAsync(a.shift + 2 * size_t(w0), shift, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToDevice, st));
-        if (a.zoom_idx) HIP_OK(hipMemcpyAsync(a.zoom_idx + 2 * size_t(w0), zidx, sizeof(int32_t) * 2 * n, hipMemcpyDeviceToDevice, st));
-        if (a.maxval && !fine_only) hipLaunchKernelGGL(k_to_float<T>, dim3(idivup(n, 256)), dim3(256), 0, st, val, a.maxval + w0, n);
+        // the chunk's integer results and peak value to the caller's arrays, one launch
+        {
+            auto at = [&](int32_t *o) { return o ? o + 2 * size_t(w0) : nullptr; };
+            hipLaunchKernelGGL(k_chunk_out<T>, dim3(idivup(n, 256)), dim3(256), 0, st, n, fine_only ? nullptr : loc,
+                               at(a.maxloc), start, at(a.start), shift, at(a.shift), zidx, at(a.zoom_idx),
+                               fine_only ? nullptr : val, a.maxval ? a.maxval + w0 : nullptr);
+        }
         HIP_OK(hipGetLastError());
     }
 }

@@ -320,6 +320,39 @@ __global__ void __launch_bounds__(256) k_rowspec(const T *img, int nimg, int h, 
     }
 }
 
+// The same half spectra with the row transform in place (fft_nat2dr, length
+// L = R1 * R2, read back from digit-reversed order), G row pairs per block of
+// odd LDS pitch
+template <class T, int R1, int R2, int G>
+__global__ void __launch_bounds__(256) k_rowspec_f(const T *img, int nimg, int h, int w, const cx<T> *tw,
+                                                   cx<T> *spec) {
+    constexpr int L = R1 * R2, PR = L + 1, H1 = L / 2 + 1;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T> *C = reinterpret_cast<cx<T> *>(smem_raw);   // [G][PR]
+    const int np = (h + 1) / 2;
+    const int t0 = blockIdx.x * G;
+    const int nr = min(G, nimg * np - t0);
+    for (int i = threadIdx.x; i < nr * L; i += 256) {
+        const int k = i / L, x = i - k * L;
+        const int t = t0 + k, win = t / np, r = 2 * (t - win * np);
+        const T *im = img + size_t(win) * h * w;
+        C[k * PR + x] = {x < w ? im[r * w + x] : T(0), (x < w && r + 1 < h) ? im[(r + 1) * w + x] : T(0)};
+    }
+    __syncthreads();
+    fft_nat2dr<T, -1, R1, R2, 256>(C, nr, PR, 1, tw);
+    for (int i = threadIdx.x; i < nr * H1; i += 256) {
+        const int k = i / H1, kk = i - k * H1;
+        const int t = t0 + k, win = t / np, j = t - win * np;
+        const cx<T> za = C[k * PR + dr_pos<R1, R2>(kk)], bc = conj(C[k * PR + dr_pos<R1, R2>(kk ? L - kk : 0)]);
+        cx<T> *sp = spec + size_t(win) * h * H1;
+        sp[size_t(2 * j) * H1 + kk] = (za + bc) * T(0.5);
+        if (2 * j + 1 < h) {
+            const cx<T> dd = (za - bc) * T(0.5);
+            sp[size_t(2 * j + 1) * H1 + kk] = {dd.y, -dd.x};
+        }
+    }
+}
+
 // the inverse row FFT of the lag rows and k_crop_real in one pass: rows t of
 // length d.n (t = window * nl + lag) -> out[t][v] = Re, v < nlw; `g` rows per block
 template <class T>
@@ -977,12 +1010,6 @@ __global__ void k_sinc(const T *in, int nin, T *out, int nout, const int32_t *sh
         }
         o[ox * nout + oy] = T(acc / wsum);
     }
-}
-
-template <class T>
-__global__ void k_to_float(const T *in, float *out, int n) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = float(in[i]);
 }
 
 // final offset (cuOffset.cu:137-146)
