@@ -320,13 +320,21 @@ void os2_cols_launch(Engine &e, const cx<T> *src, int ih, int iw, const int32_t 
                        start, tab, w, tw_of<T>(e, N), Y);
 }
 
+// output rows per block of k_os2_rows_f: 16 where the second factor is a long
+// odd DFT (its pass has G * R1 work items), else 8 (measured on window-256 pairs)
+template <class T, int R2>
+constexpr int os2_rows_g() { return sizeof(T) == 4 && R2 >= 17 ? 16 : 8; }
+
+template <class T>
+int os2_rows_per_block(int w) { return sizeof(T) == 4 && (w == 136 || w == 264) ? 16 : 8; }
+
 template <class T, int R1, int R2>
 void os2_rows_launch(Engine &e, const cx<T> *src, int ih, int iw, const int32_t *start, const cx<double> *tab,
-                     const cx<T> *Y, int nimg, int h, T *out, hipStream_t st) {
-    constexpr int N = R1 * R2, G = sizeof(T) == 4 ? 16 : 8;
+                     const cx<T> *Y, int nimg, int h, T *out, double *part, hipStream_t st) {
+    constexpr int N = R1 * R2, G = os2_rows_g<T, R2>();
     const size_t lds = (size_t(G) * (N + 1) + N) * sizeof(cx<T>) + size_t(G) * N * sizeof(T);
     hipLaunchKernelGGL((k_os2_rows_f<T, R1, R2, G>), dim3(idivup(nimg * 2 * h, G)), dim3(256), lds, st, src, ih, iw,
-                       start, tab, Y, nimg, h, tw_of<T>(e, N), out);
+                       start, tab, Y, nimg, h, tw_of<T>(e, N), out, part);
 }
 
 template <class T>
@@ -345,31 +353,40 @@ bool os2_cols_fast(Engine &e, const cx<T> *src, int ih, int iw, const int32_t *s
     }
 }
 
-template <class T>
-bool os2_rows_fast(Engine &e, const cx<T> *src, int ih, int iw, const int32_t *start, const cx<double> *tab,
-                   const cx<T> *Y, int nimg, int h, int w, T *out, hipStream_t st) {
+// chip widths with an in-place row kernel
+bool os2_rows_has(int w) {
     switch (w) {
-        case 32: os2_rows_launch<T, 8, 4>(e, src, ih, iw, start, tab, Y, nimg, h, out, st); return true;
-        case 40: os2_rows_launch<T, 8, 5>(e, src, ih, iw, start, tab, Y, nimg, h, out, st); return true;
-        case 64: os2_rows_launch<T, 8, 8>(e, src, ih, iw, start, tab, Y, nimg, h, out, st); return true;
-        case 72: os2_rows_launch<T, 8, 9>(e, src, ih, iw, start, tab, Y, nimg, h, out, st); return true;
-        case 128: os2_rows_launch<T, 16, 8>(e, src, ih, iw, start, tab, Y, nimg, h, out, st); return true;
-        case 136: os2_rows_launch<T, 8, 17>(e, src, ih, iw, start, tab, Y, nimg, h, out, st); return true;
-        case 256: os2_rows_launch<T, 16, 16>(e, src, ih, iw, start, tab, Y, nimg, h, out, st); return true;
-        case 264: os2_rows_launch<T, 8, 33>(e, src, ih, iw, start, tab, Y, nimg, h, out, st); return true;
+        case 32: case 40: case 64: case 72: case 128: case 136: case 256: case 264: return true;
         default: return false;
     }
 }
 
 template <class T>
+bool os2_rows_fast(Engine &e, const cx<T> *src, int ih, int iw, const int32_t *start, const cx<double> *tab,
+                   const cx<T> *Y, int nimg, int h, int w, T *out, double *part, hipStream_t st) {
+    switch (w) {
+        case 32: os2_rows_launch<T, 8, 4>(e, src, ih, iw, start, tab, Y, nimg, h, out, part, st); return true;
+        case 40: os2_rows_launch<T, 8, 5>(e, src, ih, iw, start, tab, Y, nimg, h, out, part, st); return true;
+        case 64: os2_rows_launch<T, 8, 8>(e, src, ih, iw, start, tab, Y, nimg, h, out, part, st); return true;
+        case 72: os2_rows_launch<T, 8, 9>(e, src, ih, iw, start, tab, Y, nimg, h, out, part, st); return true;
+        case 128: os2_rows_launch<T, 16, 8>(e, src, ih, iw, start, tab, Y, nimg, h, out, part, st); return true;
+        case 136: os2_rows_launch<T, 8, 17>(e, src, ih, iw, start, tab, Y, nimg, h, out, part, st); return true;
+        case 256: os2_rows_launch<T, 16, 16>(e, src, ih, iw, start, tab, Y, nimg, h, out, part, st); return true;
+        case 264: os2_rows_launch<T, 8, 33>(e, src, ih, iw, start, tab, Y, nimg, h, out, part, st); return true;
+        default: return false;
+    }
+}
+
+// `part`: block sums for k_mean_parts (in-place row kernel only; null for none)
+template <class T>
 void os2_separable(Engine &e, const cx<T> *src, int ih, int iw, const int32_t *start, const cx<double> *tab, int nimg,
-                   int h, int w, cx<T> *Y, T *out, hipStream_t st) {
+                   int h, int w, cx<T> *Y, T *out, hipStream_t st, double *part = nullptr) {
     const bool f32 = std::is_same<T, float>::value;
     const FftPlan &pc = e.plan(h), &pr = e.plan(w);
     const int elems = fft_elems<T>(e);
     if (e.oversampler == 2) {
         const bool cols = os2_cols_fast<T>(e, src, ih, iw, start, tab, nimg, h, w, Y, st);
-        if (cols && os2_rows_fast<T>(e, src, ih, iw, start, tab, Y, nimg, h, w, out, st)) return;
+        if (cols && os2_rows_fast<T>(e, src, ih, iw, start, tab, Y, nimg, h, w, out, part, st)) return;
         if (cols) {
             const int g = std::max(1, std::min(elems / w, (81920 - w * int(sizeof(cx<T>))) /
                                                               (w * int(2 * sizeof(cx<T>) + sizeof(T)))));
@@ -408,26 +425,29 @@ int row_len_fast(int sw) {
 }
 
 template <int R1, int R2>
-void rowspec_launch(Engine &e, const float *img, int nimg, int h, int w, cx<float> *spec, hipStream_t st) {
-    constexpr int L = R1 * R2, G = 16;
+void rowspec_launch(Engine &e, const float *img, int nimg, int h, int w, const double *mean, cx<float> *spec,
+                    hipStream_t st) {
+    constexpr int L = R1 * R2, G = 8;
     hipLaunchKernelGGL((k_rowspec_f<float, R1, R2, G>), dim3(idivup(nimg * ((h + 1) / 2), G)), dim3(256),
-                       size_t(G) * (L + 1) * sizeof(cx<float>), st, img, nimg, h, w, tw_of<float>(e, L), spec);
+                       size_t(G) * (L + 1) * sizeof(cx<float>), st, img, nimg, h, w, tw_of<float>(e, L), mean, spec);
 }
 
-void rowspec_fast(Engine &e, int L, const float *img, int nimg, int h, int w, cx<float> *spec, hipStream_t st) {
+// `mean`: per-image mean subtracted on load (null for none)
+void rowspec_fast(Engine &e, int L, const float *img, int nimg, int h, int w, const double *mean, cx<float> *spec,
+                  hipStream_t st) {
     switch (L) {
-        case 72: rowspec_launch<8, 9>(e, img, nimg, h, w, spec, st); break;
-        case 80: rowspec_launch<16, 5>(e, img, nimg, h, w, spec, st); break;
-        case 104: rowspec_launch<8, 13>(e, img, nimg, h, w, spec, st); break;
-        case 128: rowspec_launch<16, 8>(e, img, nimg, h, w, spec, st); break;
-        case 144: rowspec_launch<16, 9>(e, img, nimg, h, w, spec, st); break;
-        case 168: rowspec_launch<8, 21>(e, img, nimg, h, w, spec, st); break;
-        case 195: rowspec_launch<13, 15>(e, img, nimg, h, w, spec, st); break;
-        case 256: rowspec_launch<16, 16>(e, img, nimg, h, w, spec, st); break;
-        case 272: rowspec_launch<16, 17>(e, img, nimg, h, w, spec, st); break;
-        case 289: rowspec_launch<17, 17>(e, img, nimg, h, w, spec, st); break;
-        case 325: rowspec_launch<13, 25>(e, img, nimg, h, w, spec, st); break;
-        case 528: rowspec_launch<16, 33>(e, img, nimg, h, w, spec, st); break;
+        case 72: rowspec_launch<8, 9>(e, img, nimg, h, w, mean, spec, st); break;
+        case 80: rowspec_launch<16, 5>(e, img, nimg, h, w, mean, spec, st); break;
+        case 104: rowspec_launch<8, 13>(e, img, nimg, h, w, mean, spec, st); break;
+        case 128: rowspec_launch<16, 8>(e, img, nimg, h, w, mean, spec, st); break;
+        case 144: rowspec_launch<16, 9>(e, img, nimg, h, w, mean, spec, st); break;
+        case 168: rowspec_launch<8, 21>(e, img, nimg, h, w, mean, spec, st); break;
+        case 195: rowspec_launch<13, 15>(e, img, nimg, h, w, mean, spec, st); break;
+        case 256: rowspec_launch<16, 16>(e, img, nimg, h, w, mean, spec, st); break;
+        case 272: rowspec_launch<16, 17>(e, img, nimg, h, w, mean, spec, st); break;
+        case 289: rowspec_launch<17, 17>(e, img, nimg, h, w, mean, spec, st); break;
+        case 325: rowspec_launch<13, 25>(e, img, nimg, h, w, mean, spec, st); break;
+        case 528: rowspec_launch<16, 33>(e, img, nimg, h, w, mean, spec, st); break;
         default: throw Error(AMPCOR_E_ARG, "no in-place row transform of length " + std::to_string(L));
     }
 }
@@ -533,6 +553,15 @@ void Engine::run_staged_t(const RunArgs &a) {
     const bool f32 = sizeof(T) == 4;
     const int R1w = f32 && row_len_fast(S_w) ? row_len_fast(S_w) : L1w;
     const int R2w = f32 && row_len_fast(OSw) ? row_len_fast(OSw) : L2w;
+    // the oversampled template's mean from the row kernel's block sums, subtracted
+    // as the row spectra load it (no k_sub_mean pass), where every piece applies:
+    // FP32, separable in-place rows, whole-chip blocks, the in-place row spectra,
+    // and the normalization that needs no mean-free template image (search <= 1024)
+    const bool fuse_mean = f32 && sep_os && oversampler == 2 && os2_rows_has(W_w) && os2_rows_has(W_h) &&
+                           sz.os_win_h % os2_rows_per_block<T>(W_w) == 0 && row_os && R2w != L2w && OSw <= 1024;
+    double *os_part = fuse_mean ? buf<double>(tn + "os_part", size_t(nb) * 2 * (sz.os_win_h / os2_rows_per_block<T>(W_w)))
+                                : nullptr;
+    double *os_mean = fuse_mean ? buf<double>(tn + "os_mean", nb) : nullptr;
     size_t zmax = 0;
     if (!row_raw) zmax = size_t(L1h) * L1w;
     if (!row_os) zmax = std::max(zmax, size_t(L2h) * L2w);
@@ -548,8 +577,9 @@ void Engine::run_staged_t(const RunArgs &a) {
     }
     cx<T> *rspec = spec_elems ? buf<cx<T>>(tn + "rspec", size_t(nb) * spec_elems) : nullptr;
     cx<T> *rq = q_elems ? buf<cx<T>>(tn + "rq", size_t(nb) * q_elems) : nullptr;
+    // tmean: the template's mean, subtracted as its rows load (FP32 in-place row path), or null
     auto row_correlate = [&](const T *tm, int th, int tw, const T *sr, int sh, int sw, int L, int NL, int NLw,
-                             T *out, int n) {
+                             T *out, int n, const double *tmean) {
         const int H1 = L / 2 + 1;
         const FftPlan &pl = plan(L);
         const FftDesc &d = sizeof(T) == 4 ? pl.desc32 : pl.desc64;
@@ -557,8 +587,10 @@ void Engine::run_staged_t(const RunArgs &a) {
         const size_t lds = 2 * size_t(g) * L * sizeof(cx<T>);
         cx<T> *tspec = rspec, *sspec = rspec + size_t(n) * th * H1;
         if (sizeof(T) == 4 && row_len_fast(sw) == L) {
-            rowspec_fast(*this, L, reinterpret_cast<const float *>(tm), n, th, tw, reinterpret_cast<cx<float> *>(tspec), st);
-            rowspec_fast(*this, L, reinterpret_cast<const float *>(sr), n, sh, sw, reinterpret_cast<cx<float> *>(sspec), st);
+            rowspec_fast(*this, L, reinterpret_cast<const float *>(tm), n, th, tw, tmean,
+                         reinterpret_cast<cx<float> *>(tspec), st);
+            rowspec_fast(*this, L, reinterpret_cast<const float *>(sr), n, sh, sw, nullptr,
+                         reinterpret_cast<cx<float> *>(sspec), st);
         } else {
             hipLaunchKernelGGL(k_rowspec<T>, dim3(idivup(n * ((th + 1) / 2), g)), dim3(256), lds, st, tm, n, th, tw, g,
                                d, tspec);
@@ -606,7 +638,7 @@ void Engine::run_staged_t(const RunArgs &a) {
                                ssd_raw);
             // raw correlation (cuAmpcorChunk.cpp:48-52)
             if (row_raw) {
-                row_correlate(r_ref, W_h, W_w, r_sec, S_h, S_w, R1w, sz.raw_corr_h, sz.raw_corr_w, corr, n);
+                row_correlate(r_ref, W_h, W_w, r_sec, S_h, S_w, R1w, sz.raw_corr_h, sz.raw_corr_w, corr, n, nullptr);
             } else {
                 hipLaunchKernelGGL(k_pack_corr<T>, grid2(L1h * L1w, n), dim3(256), 0, st, r_ref, W_h, W_w, r_sec,
                                    S_h, S_w, zc, L1h, L1w);
@@ -637,13 +669,17 @@ void Engine::run_staged_t(const RunArgs &a) {
                                (const int32_t *)nullptr, W_h, W_w, dtab);
         if (sep_os) {
             os2_separable<T>(*this, c_ref, W_h, W_w, nullptr, p.deramp == 1 ? dtab : nullptr, n, W_h, W_w, c_ref_os,
-                             r_ref_os, st);
+                             r_ref_os, st, fuse_mean ? os_part : nullptr);
         } else {
             fft2_chip<T, +1>(*this, c_ref, W_h, W_w, nullptr, p.deramp == 1 ? dtab : nullptr, c_ref, n, W_h, W_w, st);
             fft2_quad<T, -1, 1>(*this, c_ref, n, W_h, W_w, c_ref_os, sz.os_win_h, sz.os_win_w, r_ref_os, st);
         }
-        hipLaunchKernelGGL(k_sub_mean<T>, dim3(n), dim3(per_window_threads(sz.os_win_h * sz.os_win_w)), 0, st,
-                           r_ref_os, sz.os_win_h * sz.os_win_w, ssd_os);
+        if (fuse_mean)
+            hipLaunchKernelGGL(k_mean_parts, dim3(idivup(n, 256)), dim3(256), 0, st, os_part,
+                               sz.os_win_h / os2_rows_per_block<T>(W_w), sz.os_win_h * sz.os_win_w, n, os_mean, ssd_os);
+        else
+            hipLaunchKernelGGL(k_sub_mean<T>, dim3(n), dim3(per_window_threads(sz.os_win_h * sz.os_win_w)), 0, st,
+                               r_ref_os, sz.os_win_h * sz.os_win_w, ssd_os);
         // secondary zoom-in crop and oversampling (cuAmpcorChunk.cpp:134-137)
         if (p.deramp == 1)
             hipLaunchKernelGGL(k_deramp_phase<T>, dim3(n), dim3(per_window_threads(sz.zoom_in_h * sz.zoom_in_w)), 0,
@@ -659,7 +695,7 @@ void Engine::run_staged_t(const RunArgs &a) {
         // oversampled correlation + normalize (cuAmpcorChunk.cpp:148-161)
         if (row_os) {
             row_correlate(r_ref_os, sz.os_win_h, sz.os_win_w, r_sec_os, OSh, OSw, R2w, sz.os_corr_h, sz.os_corr_w,
-                          corr_os, n);
+                          corr_os, n, fuse_mean ? os_mean : nullptr);
         } else {
             hipLaunchKernelGGL(k_pack_corr<T>, grid2(L2h * L2w, n), dim3(256), 0, st, r_ref_os, sz.os_win_h,
                                sz.os_win_w, r_sec_os, OSh, OSw, zc, L2h, L2w);

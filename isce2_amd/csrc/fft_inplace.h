@@ -137,6 +137,47 @@ __device__ __forceinline__ void dft_oddc(cx<T> *x) {
 }
 
 template <class T, int DIR, int R>
+__device__ __forceinline__ void dft_r(cx<T> *x);
+
+// smallest odd factor >= 3 of an odd R (R itself when prime)
+template <int R>
+constexpr int odd_factor() {
+    for (int a = 3; a * a <= R; a += 2)
+        if (R % a == 0) return a;
+    return R;
+}
+
+// DFT of composite length R = A * B in registers: n = B n1 + n2, k = k1 + A k2;
+// A-point DFTs over stride B, twiddles w_R^{n2 k1}, B-point DFTs, natural order out
+template <class T, int DIR, int A, int B>
+__device__ __forceinline__ void dft_nested(cx<T> *x) {
+    constexpr int R = A * B;
+    constexpr RootTable<R> tg{};
+    cx<T> y[R];
+#pragma unroll
+    for (int n2 = 0; n2 < B; ++n2) {
+        cx<T> v[A];
+#pragma unroll
+        for (int n1 = 0; n1 < A; ++n1) v[n1] = x[B * n1 + n2];
+        dft_r<T, DIR, A>(v);
+#pragma unroll
+        for (int k1 = 0; k1 < A; ++k1) {
+            const int m = (n2 * k1) % R;
+            y[n2 * A + k1] = m == 0 ? v[k1] : cmul(v[k1], cx<T>{T(tg.c[m]), T(DIR) * T(tg.s[m])});
+        }
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < A; ++k1) {
+        cx<T> v[B];
+#pragma unroll
+        for (int n2 = 0; n2 < B; ++n2) v[n2] = y[n2 * A + k1];
+        dft_r<T, DIR, B>(v);
+#pragma unroll
+        for (int k2 = 0; k2 < B; ++k2) x[k1 + A * k2] = v[k2];
+    }
+}
+
+template <class T, int DIR, int R>
 __device__ __forceinline__ void dft_r(cx<T> *x) {
     if constexpr (R == 2) dft2<T, DIR>(x[0], x[1]);
     else if constexpr (R == 3) dft3<T, DIR>(x);
@@ -145,7 +186,8 @@ __device__ __forceinline__ void dft_r(cx<T> *x) {
     else if constexpr (R == 8) dft8<T, DIR>(x);
     else if constexpr (R == 9) dft9<T, DIR>(x);
     else if constexpr (R == 16) dft16<T, DIR>(x);
-    else if constexpr (R % 2 == 1) dft_oddc<T, DIR, R>(x);
+    else if constexpr (R % 2 == 1 && odd_factor<R>() == R) dft_oddc<T, DIR, R>(x);
+    else if constexpr (R % 2 == 1) dft_nested<T, DIR, odd_factor<R>(), R / odd_factor<R>()>(x);
     else static_assert(R == 2, "unsupported radix");
 }
 

@@ -325,18 +325,28 @@ __global__ void __launch_bounds__(256) k_rowspec(const T *img, int nimg, int h, 
 // odd LDS pitch
 template <class T, int R1, int R2, int G>
 __global__ void __launch_bounds__(256) k_rowspec_f(const T *img, int nimg, int h, int w, const cx<T> *tw,
-                                                   cx<T> *spec) {
+                                                   const double *mean, cx<T> *spec) {
     constexpr int L = R1 * R2, PR = L + 1, H1 = L / 2 + 1;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *C = reinterpret_cast<cx<T> *>(smem_raw);   // [G][PR]
+    __shared__ T rowmu[G];                             // the pair's image mean, subtracted on load when given
     const int np = (h + 1) / 2;
     const int t0 = blockIdx.x * G;
     const int nr = min(G, nimg * np - t0);
+    if (mean) {
+        if (threadIdx.x < nr) rowmu[threadIdx.x] = T(mean[(t0 + threadIdx.x) / np]);
+        __syncthreads();
+    }
     for (int i = threadIdx.x; i < nr * L; i += 256) {
         const int k = i / L, x = i - k * L;
         const int t = t0 + k, win = t / np, r = 2 * (t - win * np);
         const T *im = img + size_t(win) * h * w;
-        C[k * PR + x] = {x < w ? im[r * w + x] : T(0), (x < w && r + 1 < h) ? im[(r + 1) * w + x] : T(0)};
+        cx<T> v = {x < w ? im[r * w + x] : T(0), (x < w && r + 1 < h) ? im[(r + 1) * w + x] : T(0)};
+        if (mean) {
+            const T mu = rowmu[k];
+            v = {x < w ? v.x - mu : T(0), (x < w && r + 1 < h) ? v.y - mu : T(0)};
+        }
+        C[k * PR + x] = v;
     }
     __syncthreads();
     fft_nat2dr<T, -1, R1, R2, 256>(C, nr, PR, 1, tw);
@@ -920,11 +930,13 @@ __global__ void __launch_bounds__(256) k_os2_cols_f(const cx<T> *src, int ih, in
 }
 
 // G output rows per block over the nimg * 2h rows; chip width R1 * R2;
-// out: [nimg][2h][2 R1 R2] amplitudes
+// out: [nimg][2h][2 R1 R2] amplitudes. With `part` (G dividing 2h, so that a
+// block holds rows of one chip only), each block also writes the FP64 sums of
+// its amplitudes and of their squares to part[2 * block], part[2 * block + 1].
 template <class T, int R1, int R2, int G>
 __global__ void __launch_bounds__(256) k_os2_rows_f(const cx<T> *src, int ih, int iw, const int32_t *start,
                                                     const cx<double> *tab, const cx<T> *Y, int nimg, int h,
-                                                    const cx<T> *tw, T *out) {
+                                                    const cx<T> *tw, T *out, double *part) {
     constexpr int N = R1 * R2, PR = N + 1;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *C = reinterpret_cast<cx<T> *>(smem_raw);   // [G][PR]
@@ -949,10 +961,38 @@ __global__ void __launch_bounds__(256) k_os2_rows_f(const cx<T> *src, int ih, in
     __syncthreads();
     fft_dr2nat<T, -1, R1, R2, 256>(C, nr, PR, 1, tw);
     T *o = out + size_t(t0) * 2 * N;
+    double s1 = 0.0, s2 = 0.0;
     for (int i = threadIdx.x; i < nr * N; i += 256) {
         const int k = i / N, c = i - k * N;
-        *reinterpret_cast<cx<T> *>(o + size_t(k) * 2 * N + 2 * c) = cx<T>{amp[i], amag(C[k * PR + c])};
+        const T a0 = amp[i], a1 = amag(C[k * PR + c]);
+        *reinterpret_cast<cx<T> *>(o + size_t(k) * 2 * N + 2 * c) = cx<T>{a0, a1};
+        s1 += double(a0) + double(a1);
+        s2 += double(a0) * double(a0) + double(a1) * double(a1);
     }
+    if (part) {
+        __shared__ double red[16];
+        s1 = block_sum256(s1, red);
+        s2 = block_sum256(s2, red);
+        if (threadIdx.x == 0) {
+            part[2 * blockIdx.x] = s1;
+            part[2 * blockIdx.x + 1] = s2;
+        }
+    }
+}
+
+// mean and sum of squared deviations of each [size] image from nb block sums
+// (k_os2_rows_f's `part`): the values k_sub_mean finds, for the consumer to
+// subtract the mean as it loads the image (k_rowspec_f's `mean`)
+static __global__ void k_mean_parts(const double *part, int nb, int size, int nimg, double *mean, double *ssd) {
+    const int w = blockIdx.x * blockDim.x + threadIdx.x;
+    if (w >= nimg) return;
+    double s1 = 0.0, s2 = 0.0;
+    for (int j = 0; j < nb; ++j) {
+        s1 += part[2 * (size_t(w) * nb + j)];
+        s2 += part[2 * (size_t(w) * nb + j) + 1];
+    }
+    mean[w] = s1 / size;
+    if (ssd) ssd[w] = fmax(s2 - s1 * s1 / size, 0.0);
 }
 
 template <class T>
