@@ -160,6 +160,16 @@ int ampcor_run_windows_noalloc(ampcor_ctx *ctx,
                        const int32_t *sec_d, const int32_t *sec_a, int32_t n,
                        float *out_offset, float *out_snr, float *out_cov, void *stream);
 
+/* Near-tie selection for FP64 re-runs driven by the caller (no counterpart in
+ * the reference; for batch drivers that capture their re-runs in a HIP graph).
+ * From the margins that ampcor_run_windows_debug writes, idx[0..cap) gets the
+ * indices of the first cap windows in order whose margin_coarse < tau_coarse
+ * or margin_fine < tau_fine (NaN counts as below), then, while slots remain,
+ * the first windows that are not. Needs 0 <= cap <= n. Device pointers; one
+ * workgroup on `stream`, no host synchronization, no allocation (capturable). */
+int ampcor_select_near_ties(ampcor_ctx *ctx, const float *margin_coarse, const float *margin_fine, int32_t n,
+                            float tau_coarse, float tau_fine, int32_t cap, int32_t *idx, void *stream);
+
 /* Precision control. No counterpart in the reference (it runs FP32 only).
  *   mode 0: FP32 kernels only.
  *   mode 1 (default): FP32 kernels; windows whose raw-surface or oversampled-

@@ -26,7 +26,8 @@ EXPORTS = ("ampcor_create", "ampcor_destroy", "ampcor_setup", "ampcor_run_window
            "ampcor_run_windows_debug", "ampcor_set_precision", "ampcor_precision_stats",
            "ampcor_profile", "ampcor_profile_report", "ampcor_last_error", "ampcor_version",
            "ampcor_set_fft_radices", "ampcor_set_option", "ampcor_profile_only",
-           "ampcor_start_pixels", "ampcor_check_range", "ampcor_run_windows_noalloc")
+           "ampcor_start_pixels", "ampcor_check_range", "ampcor_run_windows_noalloc",
+           "ampcor_select_near_ties")
 
 
 class Params(C.Structure):
@@ -77,6 +78,7 @@ def load() -> C.CDLL:
                                         vp, vp, vp, vp, vp, vp]
     lib.ampcor_check_range.argtypes = [C.POINTER(Params), vp, vp, vp, vp, i32, i64, i64, i64, i64,
                                        C.POINTER(i32), C.POINTER(i32)]
+    lib.ampcor_select_near_ties.argtypes = [vp, vp, vp, i32, C.c_float, C.c_float, i32, vp, vp]
     lib.ampcor_last_error.argtypes = [vp]
     lib.ampcor_last_error.restype = C.c_char_p
     lib.ampcor_version.restype = C.c_char_p
@@ -84,7 +86,7 @@ def load() -> C.CDLL:
                  "ampcor_run_windows_debug", "ampcor_profile", "ampcor_profile_report",
                  "ampcor_set_precision", "ampcor_precision_stats", "ampcor_set_fft_radices",
                  "ampcor_set_option", "ampcor_profile_only", "ampcor_start_pixels", "ampcor_check_range",
-                 "ampcor_run_windows_noalloc"):
+                 "ampcor_run_windows_noalloc", "ampcor_select_near_ties"):
         getattr(lib, name).restype = C.c_int
     _lib = lib
     return lib

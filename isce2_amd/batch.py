@@ -9,13 +9,19 @@ with start pixels ``(i*W, 0)`` and ``(i*S, 0)``. Each class has its own engine
 context (resident plans and buffers) and a schedule chosen by the autotuner
 (fused kernels or staged pipeline, chunk, FFT radix orders).
 
-One step = every class's launches on one stream in no-sync mode; with
-``graph=True`` the step is captured once into a HIP graph
-(``torch.cuda.CUDAGraph``) and replayed. Under precision 1 the windows whose
-argmax margins fall below the thresholds are re-run through the FP64 pipeline
-after the step: the margins of every class come back in one device-to-host
-copy, and each class re-runs on its own stream and engine context (so the
-graph's buffers are never touched and the classes' re-runs overlap on the GPU).
+One step = every class's launches in no-sync mode; with ``graph=True`` the
+step is captured once into a HIP graph (``torch.cuda.CUDAGraph``) and
+replayed. Under precision 1 the windows whose argmax margins fall below the
+thresholds are re-run through the FP64 pipeline inside the same step: right
+after a class's launches, its near ties are picked on the device (a stable
+sort puts them first, in window order) and up to ``cap`` of them (ceil(m/64),
+at least 8) run through the class's FP64 engine context on its own stream,
+whose results are scattered into the class's outputs. The FP64 chains thus
+overlap the other classes' FP32 launches, and nothing waits on the host. The
+capacity's spare slots re-run the first windows that are not near ties, whose
+integer results FP64 leaves as they are. After the step one small copy
+brings the near-tie counts back; a class with more than ``cap`` near ties has
+the rest re-run from the host.
 """
 from __future__ import annotations
 
@@ -54,8 +60,10 @@ class _Class:
             self.out[k] = torch.full((m,), float("nan"), dtype=torch.float32, device=dev)
         self.sched = Schedule(mode=MODE_FUSED if self.sizes["fused"] else MODE_STAGED,
                               chunk=2048 if self.sizes["fused"] else 256)
-        self.rerun_eng = None      # FP64 re-run context and stream, made on first use
+        self.rerun_eng = None      # FP64 re-run context and stream (precision 1)
         self.rerun_stream = None
+        self.overflow_eng = None   # FP64 context of the host-driven re-runs (never captured)
+        self.cap = min(m, max(8, -(-m // 64)))            # FP64 re-runs inside the step
 
     def args(self):
         return (self.ref, self.sec, *self.starts)
@@ -90,6 +98,12 @@ class ChipPairBatch:
                 c.sched.apply(c.eng)
                 c.eng.setup(c.params)
             c.eng.set_precision(2 if precision == 2 else 0)
+        if precision == 1:
+            for c in self.classes:
+                c.rerun_eng = Engine(device)
+                c.rerun_eng.setup(c.params)
+                c.rerun_eng.set_precision(2)
+                c.rerun_stream = torch.cuda.Stream(device=self.dev)
         self.graph = None
         self.stream = torch.cuda.Stream(device=self.dev)
         self.rerun_count = 0
@@ -107,9 +121,29 @@ class ChipPairBatch:
             g2[k] = dict(ref=ref.to(dev), sec=sec.to(dev), idx=np.asarray(idx))
         return cls(g2, device=device, **kw)
 
+    _RERUN_KEYS = ("offset", "snr", "cov", "maxloc", "start", "shift", "zoom_idx", "maxval")
+
+    def _rerun_in_step(self, c):
+        """FP64 re-run of the class's first ``cap`` near ties (then non-ties), on the device."""
+        idx = c.rerun_eng.select_near_ties(c.out["margin_coarse"], c.out["margin_fine"], c.cap, self.tau,
+                                           self.tau).long()
+        st = [x.index_select(0, idx).contiguous() for x in c.starts]
+        o = c.rerun_eng.run_windows(c.ref, c.sec, *st, mode=MODE_STAGED, debug=True, nosync=True)
+        for k in self._RERUN_KEYS:
+            c.out[k].index_copy_(0, idx, o[k])
+
     def _step(self, nosync):
+        main = torch.cuda.current_stream(self.dev)
         for c in self.classes:
             c.launch(nosync)
+            if self.precision == 1:
+                s = c.rerun_stream
+                s.wait_stream(main)
+                with torch.cuda.stream(s):
+                    self._rerun_in_step(c)
+        if self.precision == 1:
+            for c in self.classes:
+                main.wait_stream(c.rerun_stream)
 
     def run(self):
         """One pass over every pair; results stay on the device until :meth:`results`."""
@@ -127,36 +161,38 @@ class ChipPairBatch:
             else:
                 self._step(True)
             if self.precision == 1:
-                self._rerun_near_ties()
+                self._rerun_overflow()
         self.stream.synchronize()
 
-    def _rerun_near_ties(self):
-        self.rerun_count = 0
+    def _rerun_overflow(self):
+        """Host-driven FP64 re-run of the near ties beyond each class's in-step capacity."""
         mc = torch.cat([c.out["margin_coarse"] for c in self.classes]).cpu().numpy()
         mf = torch.cat([c.out["margin_fine"] for c in self.classes]).cpu().numpy()
         near = ~(mc >= self.tau) | ~(mf >= self.tau)
+        self.rerun_count = 0
         main = torch.cuda.current_stream(self.dev)
         used = []
         o0 = 0
         for c in self.classes:
-            sel = np.nonzero(near[o0:o0 + c.m])[0]
+            pos = np.nonzero(near[o0:o0 + c.m])[0]
             o0 += c.m
-            if len(sel) == 0:
+            self.rerun_count += len(pos)
+            if len(pos) <= c.cap:
                 continue
-            self.rerun_count += len(sel)
-            if c.rerun_eng is None:
-                c.rerun_eng = Engine(self.device)
-                c.rerun_eng.setup(c.params)
-                c.rerun_eng.set_precision(2)
-                c.rerun_stream = torch.cuda.Stream(device=self.dev)
+            sel = torch.as_tensor(pos[c.cap:], dtype=torch.int64).to(self.dev)
+            # a context of its own: the in-step context's buffers are held by the graph
+            # at the capacity's size and must not grow
+            if c.overflow_eng is None:
+                c.overflow_eng = Engine(self.device)
+                c.overflow_eng.setup(c.params)
+                c.overflow_eng.set_precision(2)
             s = c.rerun_stream
             s.wait_stream(main)
             with torch.cuda.stream(s):
-                t = torch.as_tensor(sel, dtype=torch.int64).to(self.dev)
-                st = [x.index_select(0, t).contiguous() for x in c.starts]
-                o = c.rerun_eng.run_windows(c.ref, c.sec, *st, mode=MODE_STAGED, debug=True)
-                for k in ("offset", "snr", "cov", "maxloc", "start", "shift", "zoom_idx", "maxval"):
-                    c.out[k].index_copy_(0, t, o[k])
+                st = [x.index_select(0, sel).contiguous() for x in c.starts]
+                o = c.overflow_eng.run_windows(c.ref, c.sec, *st, mode=MODE_STAGED, debug=True)
+                for k in self._RERUN_KEYS:
+                    c.out[k].index_copy_(0, sel, o[k])
             used.append(s)
         for s in used:
             main.wait_stream(s)
@@ -175,5 +211,6 @@ class ChipPairBatch:
         self.graph = None
         for c in self.classes:
             c.eng.close()
-            if c.rerun_eng is not None:
-                c.rerun_eng.close()
+            for e in (c.rerun_eng, c.overflow_eng):
+                if e is not None:
+                    e.close()

@@ -104,6 +104,9 @@ struct Engine {
     // set while a call runs on a stream under HIP graph capture: anything that
     // would allocate device memory or copy synchronously throws instead
     bool no_alloc = false;
+    // set once a stream capture has recorded launches on this context: a captured
+    // graph holds the addresses of its buffers, so none may move (grow) afterwards
+    bool captured = false;
     void forbid_in_nosync(const char *what) const {
         if (no_alloc)
             throw Error(AMPCOR_E_ARG, std::string("a call under stream capture would ") + what +
@@ -114,7 +117,12 @@ struct Engine {
     template <class U> U *buf(const std::string &name, size_t count) {
         auto &b = bufs[name];
         if (!b) { forbid_in_nosync("allocate a device buffer"); b.reset(new DevBuf()); }
-        if (b->bytes < count * sizeof(U)) forbid_in_nosync("grow a device buffer");
+        if (b->bytes < count * sizeof(U)) {
+            forbid_in_nosync("grow a device buffer");
+            if (captured)
+                throw Error(AMPCOR_E_ARG, "a graph captured from this context holds its buffers, and this call "
+                                          "needs a larger one: run it on another context");
+        }
         b->reserve(count * sizeof(U));
         return b->as<U>();
     }

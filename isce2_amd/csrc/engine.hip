@@ -871,6 +871,7 @@ void Engine::run(const RunArgs &in) {
     if (capturing && profiling) throw Error(AMPCOR_E_ARG, "disable ampcor_profile before capturing");
     struct Restore { bool &f; bool v; ~Restore() { f = v; } } restore{no_alloc, no_alloc};
     no_alloc = capturing;
+    if (capturing) captured = true;
 
     if (a.refined && (p.surf_method != 0 || sz.zoom > 64))
         throw Error(AMPCOR_E_UNSUPPORTED, "the FP64 peak refinement needs FFT surface oversampling and a zoom "
@@ -987,6 +988,20 @@ int ampcor_create(int32_t device, ampcor_ctx **out) {
         c->eng.device = device;
         c->eng.scratch_ptr = scratch;
         *out = c;
+    });
+}
+
+int ampcor_select_near_ties(ampcor_ctx *ctx, const float *margin_coarse, const float *margin_fine, int32_t n,
+                            float tau_coarse, float tau_fine, int32_t cap, int32_t *idx, void *stream) {
+    if (!ctx) return AMPCOR_E_ARG;
+    return guarded(ctx, [&] {
+        if (n < 0 || cap < 0 || cap > n) throw ampcor::Error(AMPCOR_E_ARG, "the capacity must lie in [0, n]");
+        if (cap == 0) return;
+        if (!margin_coarse || !margin_fine || !idx) throw ampcor::Error(AMPCOR_E_ARG, "null pointer argument");
+        HIP_OK(hipSetDevice(ctx->eng.device));
+        hipLaunchKernelGGL(ampcor::k_select_near, dim3(1), dim3(1024), 0, static_cast<hipStream_t>(stream),
+                           margin_coarse, margin_fine, n, tau_coarse, tau_fine, cap, idx);
+        HIP_OK(hipGetLastError());
     });
 }
 

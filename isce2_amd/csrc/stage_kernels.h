@@ -980,6 +980,42 @@ __global__ void __launch_bounds__(256) k_os2_rows_f(const cx<T> *src, int ih, in
     }
 }
 
+// idx[0..cap) = the first cap windows (in order) whose margins are below the
+// thresholds (NaN counts as below), then the first windows that are not, for
+// cap <= n. One workgroup of 1024 threads: per 1024-window slice, a ballot
+// per wave and the wave totals give each flagged window its slot.
+static __global__ void __launch_bounds__(1024) k_select_near(const float *mc, const float *mf, int n, float tc,
+                                                             float tf, int cap, int32_t *idx) {
+    __shared__ int wtot[16];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    int filled = 0;                     // slots taken so far (the same in every thread)
+    for (int pass = 0; pass < 2 && filled < cap; ++pass) {
+        for (int c0 = 0; c0 < n && filled < cap; c0 += blockDim.x) {
+            const int i = c0 + threadIdx.x;
+            bool flag = false;
+            if (i < n) {
+                const bool near = !(mc[i] >= tc) || !(mf[i] >= tf);
+                flag = pass == 0 ? near : !near;
+            }
+            const unsigned long long m = __ballot(flag);
+            const int before = __popcll(m & ((1ull << lane) - 1ull));
+            if (lane == 0) wtot[wv] = __popcll(m);
+            __syncthreads();
+            int off = 0, tot = 0;
+            for (int k = 0; k < nw; ++k) {
+                const int t = wtot[k];
+                if (k < wv) off += t;
+                tot += t;
+            }
+            const int slot = filled + off + before;
+            if (flag && slot < cap) idx[slot] = i;
+            filled += tot;
+            __syncthreads();
+        }
+        if (filled > cap) filled = cap;
+    }
+}
+
 // mean and sum of squared deviations of each [size] image from nb block sums
 // (k_os2_rows_f's `part`): the values k_sub_mean finds, for the consumer to
 // subtract the mean as it loads the image (k_rowspec_f's `mean`)

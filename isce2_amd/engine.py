@@ -276,6 +276,22 @@ class Engine:
             if t >= 0:
                 self.tau[i] = float(t)
 
+    def select_near_ties(self, margin_coarse, margin_fine, cap: int, tau_coarse: float, tau_fine: float,
+                         out=None, stream=None):
+        """int32 [cap] device tensor: the first ``cap`` windows whose margins fall below
+        the thresholds (NaN counts as below), in order, then the first that do not
+        (``ampcor_select_near_ties``; capturable, no host synchronization)."""
+        n = int(margin_coarse.numel())
+        dev = torch.device("cuda", self.device)
+        for t in (margin_coarse, margin_fine):
+            if t.dtype != torch.float32 or t.device != dev or not t.is_contiguous() or t.numel() != n:
+                raise ValueError("margins must be contiguous float32 device tensors of equal length")
+        idx = out if out is not None else torch.empty(cap, dtype=torch.int32, device=dev)
+        st = C.c_void_p(stream.cuda_stream if stream is not None else torch.cuda.current_stream(dev).cuda_stream)
+        check(self.lib.ampcor_select_near_ties(self._h, _ptr(margin_coarse), _ptr(margin_fine), n, float(tau_coarse),
+                                               float(tau_fine), int(cap), _ptr(idx), st), self._h)
+        return idx
+
     def set_option(self, name: str, value: int):
         """Launch-shape option (``fine_threads``: 512 or 1024)."""
         check(self.lib.ampcor_set_option(self._h, name.encode(), int(value)), self._h)

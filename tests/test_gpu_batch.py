@@ -12,6 +12,7 @@ import torch
 
 from isce2_amd.autotune import Autotuner
 from isce2_amd.batch import ChipPairBatch
+from isce2_amd.engine import MODE_STAGED
 from isce2_amd.synth import chip_pairs_uneven_grouped
 from oracle import ampcor_oracle as O
 
@@ -126,3 +127,35 @@ def test_shift_recovery_and_autotuned_schedule():
     assert np.median(err) < 0.05
     base.close()
     tuned.close()
+
+
+@pytest.mark.parametrize("n,cap", [(1, 1), (37, 8), (700, 11), (2500, 40), (3000, 3000)])
+def test_select_near_ties_matches_numpy(n, cap):
+    """The device selection of the in-step FP64 re-runs: near ties first (NaN margins
+    count as near), in window order, then the first windows that are not."""
+    from isce2_amd.engine import Engine
+    rng = np.random.default_rng(n)
+    mc = rng.uniform(0, 1e-5, n).astype(np.float32)
+    mf = rng.uniform(0, 1e-5, n).astype(np.float32)
+    mc[rng.uniform(size=n) < 0.02] = np.nan
+    e = Engine(0)
+    try:
+        for tau in (2e-6, 0.0, float("inf")):
+            got = e.select_near_ties(torch.from_numpy(mc).cuda(), torch.from_numpy(mf).cuda(), cap, tau, tau)
+            near = ~(mc >= tau) | ~(mf >= tau)
+            want = np.concatenate([np.nonzero(near)[0], np.nonzero(~near)[0]])[:cap]
+            assert np.array_equal(got.cpu().numpy(), want), tau
+    finally:
+        e.close()
+
+
+def test_captured_context_refuses_to_grow(groups):
+    """After a step is captured, a larger call on a captured context fails on the host
+    (its buffers are held by the graph) instead of moving them."""
+    b = ChipPairBatch(groups, device=0, graph=True)
+    b.run()
+    c = max(b.classes, key=lambda k: k.W)
+    st = [x.contiguous() for x in c.starts]
+    with pytest.raises(Exception, match="captured"):
+        c.rerun_eng.run_windows(c.ref, c.sec, *st, mode=MODE_STAGED, debug=True, nosync=True)
+    b.close()
