@@ -134,7 +134,9 @@ class ChipPairBatch:
 
     def _step(self, nosync):
         main = torch.cuda.current_stream(self.dev)
-        for c in self.classes:
+        # largest windows first: their FP64 re-run chains, the longest, then overlap the
+        # smaller classes' launches instead of trailing the step
+        for c in sorted(self.classes, key=lambda k: (-k.W, -k.hs)):
             c.launch(nosync)
             if self.precision == 1:
                 s = c.rerun_stream
