@@ -598,8 +598,10 @@ void Engine::run_staged_t(const RunArgs &a) {
                                d, sspec);
         }
         // lags per thread: 17 once there is more than one group of 9 (each group
-        // re-reads every spectrum; the sums themselves do not depend on the grouping)
-        if (NL > 9)
+        // re-reads every spectrum; the sums themselves do not depend on the
+        // grouping), unless that leaves too few workgroups to fill the GPU (the
+        // FP64 re-run of a few hundred windows)
+        if (NL > 9 && n * idivup(H1, 64) >= 2048)
             hipLaunchKernelGGL((k_mac_lags<T, 17>), dim3(idivup(H1, 64), idivup(NL, 17), n), dim3(64), 0, st, tspec,
                                th, sspec, sh, NL, L, rq, T(1.0 / double(L)));
         else
@@ -763,10 +765,82 @@ void Engine::run_staged(const RunArgs &a) {
 template void Engine::run_staged_t<double>(const RunArgs &a);
 
 namespace {
-__global__ void k_take_i32(const int32_t *src, const int32_t *idx, int32_t *dst, int n, int width) {
+// the four start arrays of the m selected windows into st4[0..4m), and, when
+// given, their zoom starts and shifts (pairs) into st4[4m..6m) and st4[6m..8m)
+__global__ void k_take_starts(const int32_t *rd, const int32_t *ra, const int32_t *sd, const int32_t *sa,
+                              const int32_t *start32, const int32_t *shift32, const int32_t *idx, int m,
+                              int32_t *st4) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n * width) dst[i] = src[size_t(idx[i / width]) * width + i % width];
+    if (i >= m) return;
+    const int w = idx[i];
+    st4[i] = rd[w];
+    st4[m + i] = ra[w];
+    st4[2 * m + i] = sd[w];
+    st4[3 * m + i] = sa[w];
+    if (start32) {
+        st4[4 * m + 2 * i] = start32[2 * w];
+        st4[4 * m + 2 * i + 1] = start32[2 * w + 1];
+        st4[6 * m + 2 * i] = shift32[2 * w];
+        st4[6 * m + 2 * i + 1] = shift32[2 * w + 1];
+    }
 }
+
+// Windows to re-run in FP64, in order: first every window whose coarse margin
+// is below tc (or whose fine margin is below tf when no zoom starts are given:
+// has_start 0), then every other window whose fine margin is below tf (NaN
+// counts as below). cnt = {first kind, both kinds}. One workgroup of 1024
+// threads: per 1024-window slice, a ballot per wave and the wave totals give
+// each selected window its slot.
+__global__ void __launch_bounds__(1024) k_select_rerun(const float *mc, const float *mf, int n, float tc, float tf,
+                                                       int has_start, int32_t *idx, int32_t *cnt) {
+    __shared__ int wtot[16];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    int filled = 0;                     // the same in every thread
+    int first = 0;
+    for (int pass = 0; pass < 2; ++pass) {
+        for (int c0 = 0; c0 < n; c0 += blockDim.x) {
+            const int i = c0 + threadIdx.x;
+            bool flag = false;
+            if (i < n) {
+                const bool c = !(mc[i] >= tc), f = !(mf[i] >= tf);
+                const bool full = c || (f && !has_start);
+                flag = pass == 0 ? full : (!full && f);
+            }
+            const unsigned long long b = __ballot(flag);
+            const int before = __popcll(b & ((1ull << lane) - 1ull));
+            if (lane == 0) wtot[wv] = __popcll(b);
+            __syncthreads();
+            int off = 0, tot = 0;
+            for (int k = 0; k < nw; ++k) {
+                const int t = wtot[k];
+                if (k < wv) off += t;
+                tot += t;
+            }
+            if (flag) idx[filled + off + before] = i;
+            filled += tot;
+            __syncthreads();
+        }
+        if (pass == 0) first = filled;
+    }
+    if (threadIdx.x == 0) {
+        cnt[0] = first;
+        cnt[1] = filled;
+    }
+}
+
+// the first window (lowest index) whose chips leave their images, or n, into *first
+// (initialized to n by the caller's launch of k_set_i32)
+__global__ void k_first_bad(const int32_t *rd, const int32_t *ra, const int32_t *sd, const int32_t *sa, int n,
+                            int64_t ref_h, int64_t ref_w, int64_t sec_h, int64_t sec_w, int wh, int ww, int sh,
+                            int sw, int32_t *first) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t r = rd[i], c = ra[i], s = sd[i], t = sa[i];
+    if (r < 0 || c < 0 || r + wh > ref_h || c + ww > ref_w || s < 0 || t < 0 || s + sh > sec_h || t + sw > sec_w)
+        atomicMin(first, i);
+}
+
+__global__ void k_set_i32(int32_t *p, int v) { *p = v; }
 template <class U>
 __global__ void k_put(const U *src, const int32_t *idx, U *dst, int n, int width) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -785,31 +859,25 @@ __global__ void k_put(const U *src, const int32_t *idx, U *dst, int n, int width
 // stage for all of them: the pass is latency-bound at these window counts).
 void Engine::rerun_fp64(const RunArgs &a, const float *mc, const float *mf, const int32_t *start32,
                         const int32_t *shift32) {
-    std::vector<float> hc(a.n), hf(a.n);
-    HIP_OK(hipMemcpyAsync(hc.data(), mc, sizeof(float) * a.n, hipMemcpyDeviceToHost, a.stream));
-    HIP_OK(hipMemcpyAsync(hf.data(), mf, sizeof(float) * a.n, hipMemcpyDeviceToHost, a.stream));
+    // the windows to re-run, selected on the device: those needing the whole
+    // pipeline first, then those re-run from the zoom stage on (their coarse
+    // results, SNR and covariance stay the FP32 ones); only the two counts
+    // come back to the host
+    int32_t *d_idx = buf<int32_t>("p_idx", a.n);
+    int32_t *d_cnt = buf<int32_t>("p_count", 2);
+    hipLaunchKernelGGL(k_select_rerun, dim3(1), dim3(1024), 0, a.stream, mc, mf, a.n, tau_coarse, tau_fine,
+                       start32 != nullptr, d_idx, d_cnt);
+    int32_t cnt[2];
+    HIP_OK(hipMemcpyAsync(cnt, d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, a.stream));
     HIP_OK(hipStreamSynchronize(a.stream));
-    std::vector<int32_t> full, fine;
-    for (int i = 0; i < a.n; ++i) {
-        const bool c = !(hc[i] >= tau_coarse), f = !(hf[i] >= tau_fine);
-        if (c || (f && !start32)) full.push_back(i);
-        else if (f) fine.push_back(i);
-    }
+    const int m_full = cnt[0], m = cnt[1];
     prec_checked += a.n;
-    prec_rerun += long(full.size() + fine.size());
-    // one FP64 pass over both kinds: the windows needing the whole pipeline
-    // first, then those re-run from the zoom stage on (their coarse results,
-    // SNR and covariance stay the FP32 ones)
-    std::vector<int32_t> idx = full;
-    idx.insert(idx.end(), fine.begin(), fine.end());
-    if (idx.empty()) return;
-    const int m = int(idx.size()), m_full = int(full.size());
-    int32_t *d_idx = buf<int32_t>("p_idx", m);
-    HIP_OK(hipMemcpyAsync(d_idx, idx.data(), sizeof(int32_t) * m, hipMemcpyHostToDevice, a.stream));
+    prec_rerun += m;
+    if (m == 0) return;
+    // one FP64 pass over both kinds
     int32_t *st4 = buf<int32_t>("p_starts", size_t(8) * m);
-    const int32_t *srcs[4] = {a.ref_d, a.ref_a, a.sec_d, a.sec_a};
-    for (int k = 0; k < 4; ++k)
-        hipLaunchKernelGGL(k_take_i32, dim3(idivup(m, 256)), dim3(256), 0, a.stream, srcs[k], d_idx, st4 + k * m, m, 1);
+    hipLaunchKernelGGL(k_take_starts, dim3(idivup(m, 256)), dim3(256), 0, a.stream, a.ref_d, a.ref_a, a.sec_d,
+                       a.sec_a, m_full < m ? start32 : nullptr, m_full < m ? shift32 : nullptr, d_idx, m, st4);
     float *o_off = buf<float>("p_off", size_t(2) * m), *o_snr = buf<float>("p_snr", m);
     float *o_cov = buf<float>("p_cov", size_t(3) * m), *o_val = buf<float>("p_val", m);
     int32_t *o_i = buf<int32_t>("p_ints", size_t(8) * m);
@@ -824,8 +892,6 @@ void Engine::rerun_fp64(const RunArgs &a, const float *mc, const float *mf, cons
     double *o_ref = a.refined ? buf<double>("p_refined", size_t(2) * m) : nullptr;
     sub.refined = o_ref;
     if (m_full < m) {
-        hipLaunchKernelGGL(k_take_i32, dim3(idivup(2 * m, 256)), dim3(256), 0, a.stream, start32, d_idx, st4 + 4 * m, m, 2);
-        hipLaunchKernelGGL(k_take_i32, dim3(idivup(2 * m, 256)), dim3(256), 0, a.stream, shift32, d_idx, st4 + 6 * m, m, 2);
         sub.fine_start = st4 + 4 * m;
         sub.fine_shift = st4 + 6 * m;
         sub.fine_from = m_full;
@@ -892,20 +958,18 @@ void Engine::run(const RunArgs &in) {
                            sz.search_h, sz.search_w, st4, bad);
         a.ref_d = st4; a.ref_a = st4 + a.n; a.sec_d = st4 + 2 * size_t(a.n); a.sec_a = st4 + 3 * size_t(a.n);
     } else {
-        // bounds: every chip must lie inside its image (device-side start arrays are
-        // checked here with one small copy; the caller's checkPixelInImageRange is stricter)
-        std::vector<int32_t> h(4 * size_t(a.n));
-        HIP_OK(hipMemcpyAsync(h.data(), a.ref_d, sizeof(int32_t) * a.n, hipMemcpyDeviceToHost, a.stream));
-        HIP_OK(hipMemcpyAsync(h.data() + a.n, a.ref_a, sizeof(int32_t) * a.n, hipMemcpyDeviceToHost, a.stream));
-        HIP_OK(hipMemcpyAsync(h.data() + 2 * size_t(a.n), a.sec_d, sizeof(int32_t) * a.n, hipMemcpyDeviceToHost, a.stream));
-        HIP_OK(hipMemcpyAsync(h.data() + 3 * size_t(a.n), a.sec_a, sizeof(int32_t) * a.n, hipMemcpyDeviceToHost, a.stream));
+        // bounds: every chip must lie inside its image, checked on the device (the
+        // first failing window comes back in one 4-byte copy; the caller's
+        // checkPixelInImageRange is stricter)
+        int32_t *first = buf<int32_t>("range_first", 1);
+        hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(1), 0, a.stream, first, int(a.n));
+        hipLaunchKernelGGL(k_first_bad, dim3(idivup(a.n, 256)), dim3(256), 0, a.stream, a.ref_d, a.ref_a, a.sec_d,
+                           a.sec_a, int(a.n), a.ref_h, a.ref_w, a.sec_h, a.sec_w, p.window_h, p.window_w,
+                           sz.search_h, sz.search_w, first);
+        int32_t bad_i = 0;
+        HIP_OK(hipMemcpyAsync(&bad_i, first, sizeof(int32_t), hipMemcpyDeviceToHost, a.stream));
         HIP_OK(hipStreamSynchronize(a.stream));
-        for (int i = 0; i < a.n; ++i) {
-            const int64_t rd = h[i], ra = h[a.n + i], sd = h[2 * size_t(a.n) + i], sa = h[3 * size_t(a.n) + i];
-            if (rd < 0 || ra < 0 || rd + p.window_h > a.ref_h || ra + p.window_w > a.ref_w ||
-                sd < 0 || sa < 0 || sd + sz.search_h > a.sec_h || sa + sz.search_w > a.sec_w)
-                throw Error(AMPCOR_E_RANGE, "window " + std::to_string(i) + " reads outside its image");
-        }
+        if (bad_i < a.n) throw Error(AMPCOR_E_RANGE, "window " + std::to_string(bad_i) + " reads outside its image");
     }
     if (precision == 2) run_staged_t<double>(a);
     else if (fused) run_fused(a);
