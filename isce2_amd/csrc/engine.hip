@@ -325,13 +325,17 @@ void os2_cols_launch(Engine &e, const cx<T> *src, int ih, int iw, const int32_t 
 template <class T, int R2>
 constexpr int os2_rows_g() { return sizeof(T) == 4 && R2 >= 17 ? 16 : 8; }
 
+// the same count by chip width (the widths os2_rows_fast runs: 136 = 8 x 17 is the
+// one with a long second factor)
 template <class T>
-int os2_rows_per_block(int w) { return sizeof(T) == 4 && (w == 136 || w == 264) ? 16 : 8; }
+int os2_rows_per_block(int w) { return sizeof(T) == 4 && w == 136 ? 16 : 8; }
 
 template <class T, int R1, int R2>
 void os2_rows_launch(Engine &e, const cx<T> *src, int ih, int iw, const int32_t *start, const cx<double> *tab,
                      const cx<T> *Y, int nimg, int h, T *out, double *part, hipStream_t st) {
     constexpr int N = R1 * R2, G = os2_rows_g<T, R2>();
+    // k_mean_parts reads one block sum per G rows: the width table must agree
+    if (G != os2_rows_per_block<T>(N)) throw Error(AMPCOR_E_ARG, "row block size table out of step");
     const size_t lds = (size_t(G) * (N + 1) + N) * sizeof(cx<T>) + size_t(G) * N * sizeof(T);
     hipLaunchKernelGGL((k_os2_rows_f<T, R1, R2, G>), dim3(idivup(nimg * 2 * h, G)), dim3(256), lds, st, src, ih, iw,
                        start, tab, Y, nimg, h, tw_of<T>(e, N), out, part);
@@ -348,7 +352,7 @@ bool os2_cols_fast(Engine &e, const cx<T> *src, int ih, int iw, const int32_t *s
         case 128: os2_cols_launch<T, 16, 8>(e, src, ih, iw, start, tab, nimg, w, Y, st); return true;
         case 136: os2_cols_launch<T, 8, 17>(e, src, ih, iw, start, tab, nimg, w, Y, st); return true;
         case 256: os2_cols_launch<T, 16, 16>(e, src, ih, iw, start, tab, nimg, w, Y, st); return true;
-        case 264: os2_cols_launch<T, 8, 33>(e, src, ih, iw, start, tab, nimg, w, Y, st); return true;
+        case 264: os2_cols_launch<T, 11, 24>(e, src, ih, iw, start, tab, nimg, w, Y, st); return true;
         default: return false;
     }
 }
@@ -372,7 +376,7 @@ bool os2_rows_fast(Engine &e, const cx<T> *src, int ih, int iw, const int32_t *s
         case 128: os2_rows_launch<T, 16, 8>(e, src, ih, iw, start, tab, Y, nimg, h, out, part, st); return true;
         case 136: os2_rows_launch<T, 8, 17>(e, src, ih, iw, start, tab, Y, nimg, h, out, part, st); return true;
         case 256: os2_rows_launch<T, 16, 16>(e, src, ih, iw, start, tab, Y, nimg, h, out, part, st); return true;
-        case 264: os2_rows_launch<T, 8, 33>(e, src, ih, iw, start, tab, Y, nimg, h, out, part, st); return true;
+        case 264: os2_rows_launch<T, 24, 11>(e, src, ih, iw, start, tab, Y, nimg, h, out, part, st); return true;
         default: return false;
     }
 }

@@ -186,6 +186,8 @@ __device__ __forceinline__ void dft_r(cx<T> *x) {
     else if constexpr (R == 8) dft8<T, DIR>(x);
     else if constexpr (R == 9) dft9<T, DIR>(x);
     else if constexpr (R == 16) dft16<T, DIR>(x);
+    else if constexpr (R % 16 == 0 && R > 16) dft_nested<T, DIR, 16, R / 16>(x);
+    else if constexpr (R % 8 == 0 && R > 16) dft_nested<T, DIR, 8, R / 8>(x);
     else if constexpr (R % 2 == 1 && odd_factor<R>() == R) dft_oddc<T, DIR, R>(x);
     else if constexpr (R % 2 == 1) dft_nested<T, DIR, odd_factor<R>(), R / odd_factor<R>()>(x);
     else static_assert(R == 2, "unsupported radix");
