@@ -128,12 +128,16 @@ __global__ void __launch_bounds__(NT, (K >= 9 ? 2 : 4)) k_coarse(CoarseArgs a, c
     sum2 = block_sum(sum2, red);
     const double mean = sum / double(W * W);
     const double t2 = fmax(sum2 - sum * sum / double(W * W), 0.0);   // sum of squared deviations
-    window_norms<(L + 63) / 64>(
+    // window sums from 16-row block sums down the whole search (the template
+    // chunk's region, not loaded yet, holds them: 2 * ceil(SH/16) * SW doubles)
+    static_assert(W % NORM_BLK == 0, "template height a multiple of the norm block");
+    static_assert(2 * ((L + NORM_BLK - 1) / NORM_BLK) * L <= nTc * P, "block sums fit the template chunk region");
+    window_norms_blk<(L + 63) / 64>(
         [&](int r, int c) -> double {
             const cf v = S[(r >> 1) * P + c];
             return (r & 1) ? v.y : v.x;
         },
-        SW, W, W, NLh, NLw, t2, nrm, reinterpret_cast<double *>(Tb));   // template chunk not loaded yet
+        SH, SW, W, W, NLh, NLw, t2, nrm, reinterpret_cast<double *>(Tb));
     __syncthreads();
 
     const int f = threadIdx.x % H, blk = threadIdx.x / H, u0 = blk * K;
@@ -809,6 +813,8 @@ bool fused_supported(const ampcor_params &p, const ampcor_sizes &s) {
     if (L == 0) return false;
     if (pick_k(s.raw_corr_h, NT / (L / 2)) == 0) return false;
     if (CoarseLds(p.window_w, L, s.search_h, s.raw_corr_h, s.raw_corr_w).total > 160 * 1024) return false;
+    // the window sums' block sums (doubles) live in the template chunk's region (cf)
+    if (2 * ((s.search_h + NORM_BLK - 1) / NORM_BLK) * s.search_w > coarse_tc(p.window_w) / 2 * (L + 1)) return false;
     const int P = 2 * s.zoom_in_w;
     if (FineLds(P, P - 2 * p.window_w + 1, 2 * p.window_w).total > 160 * 1024) return false;
     return true;

@@ -105,29 +105,33 @@ __device__ __forceinline__ double lane63_f64(double x) {
 }
 
 // Lag-row phase of window_norms: part holds, for NB row blocks, the column sums
-// of s ([b][SW]) then of s^2 ([NB + b][SW]) over rows [0, TH); amp(r, c) is read
-// for the slides only (rows u and u + TH, u < NLh - 1).
+// of s ([b][SW]) then of s^2 ([NB + b][SW]). With blk = 0 the blocks cover rows
+// [0, TH) and every wave slides its window down from row 0; with blk > 0 they
+// are blocks of blk rows from row 0 down the whole search (TH a multiple of
+// blk), and a wave starts from the TH / blk blocks at or above its first lag
+// row, sliding at most blk - 1 rows. amp(r, c) is read for the slides only.
 template <int CPL, int NTH, class Amp>
 __device__ __forceinline__ void window_norms_rows(Amp amp, const double *part, int NB, int SW, int TH, int TW,
-                                                  int NLh, int NLw, double t2, float *nrm) {
+                                                  int NLh, int NLw, double t2, float *nrm, int blk = 0) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     constexpr int NW = NTH / 64;
     const int per = (NLh + NW - 1) / NW;
     const int u_beg = wave * per, u_end = min(NLh, u_beg + per);
     if (u_beg >= u_end) return;
+    const int b_lo = blk ? u_beg / blk : 0, nb = blk ? TH / blk : NB;
     double g1[CPL], g2[CPL];
 #pragma unroll
     for (int m = 0; m < CPL; ++m) {
         g1[m] = 0.0; g2[m] = 0.0;
         const int c = lane + 64 * m;
         if (c < SW) {
-            for (int b = 0; b < NB; ++b) {
+            for (int b = b_lo; b < b_lo + nb; ++b) {
                 g1[m] += part[b * SW + c];
                 g2[m] += part[(NB + b) * SW + c];
             }
             // slide down to u_beg (two independent chains)
             double d1 = 0.0, d2 = 0.0, e1 = 0.0, e2 = 0.0;
-            int u = 0;
+            int u = b_lo * blk;
 #pragma unroll 2
             for (; u + 1 < u_beg; u += 2) {
                 const double vn = amp(u + TH, c), vo = amp(u, c);
@@ -187,6 +191,37 @@ __device__ __forceinline__ void window_norms_rows(Amp amp, const double *part, i
             }
         }
     }
+}
+
+// Window sums from column sums of row blocks covering the whole search height
+// (SH rows, blocks of NORM_BLK rows; part holds 2 * ceil(SH / NORM_BLK) * SW
+// doubles), so that no wave slides far to its first lag row. TH must be a
+// multiple of NORM_BLK.
+constexpr int NORM_BLK = 16;
+template <int CPL, int NTH = NT, class Amp>
+__device__ __forceinline__ void window_norms_blk(Amp amp, int SH, int SW, int TH, int TW, int NLh, int NLw,
+                                                 double t2, float *nrm, double *part) {
+    const int NB = (SH + NORM_BLK - 1) / NORM_BLK;
+    #pragma unroll 1
+    for (int t = threadIdx.x; t < NB * SW; t += NTH) {
+        const int b = t / SW, c = t - b * SW;
+        const int r0 = b * NORM_BLK, r1 = min(SH, r0 + NORM_BLK);
+        double a1 = 0.0, a2 = 0.0, b1 = 0.0, b2 = 0.0;
+        int r = r0;
+        for (; r + 1 < r1; r += 2) {
+            const double x = amp(r, c), y = amp(r + 1, c);
+            a1 += x; a2 += x * x;
+            b1 += y; b2 += y * y;
+        }
+        if (r < r1) {
+            const double x = amp(r, c);
+            a1 += x; a2 += x * x;
+        }
+        part[b * SW + c] = a1 + b1;
+        part[(NB + b) * SW + c] = a2 + b2;
+    }
+    __syncthreads();
+    window_norms_rows<CPL, NTH>(amp, part, NB, SW, TH, TW, NLh, NLw, t2, nrm, NORM_BLK);
 }
 
 template <int CPL, int NTH = NT, class Amp>
