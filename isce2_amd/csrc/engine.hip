@@ -330,14 +330,14 @@ constexpr int os2_rows_g() { return sizeof(T) == 4 && R2 >= 17 ? 16 : 8; }
 template <class T>
 int os2_rows_per_block(int w) { return sizeof(T) == 4 && w == 136 ? 16 : 8; }
 
-template <class T, int R1, int R2>
+template <class T, int R1, int R2, int NTH = 256>
 void os2_rows_launch(Engine &e, const cx<T> *src, int ih, int iw, const int32_t *start, const cx<double> *tab,
                      const cx<T> *Y, int nimg, int h, T *out, double *part, hipStream_t st) {
     constexpr int N = R1 * R2, G = os2_rows_g<T, R2>();
     // k_mean_parts reads one block sum per G rows: the width table must agree
     if (G != os2_rows_per_block<T>(N)) throw Error(AMPCOR_E_ARG, "row block size table out of step");
     const size_t lds = (size_t(G) * (N + 1) + N) * sizeof(cx<T>) + size_t(G) * N * sizeof(T);
-    hipLaunchKernelGGL((k_os2_rows_f<T, R1, R2, G>), dim3(idivup(nimg * 2 * h, G)), dim3(256), lds, st, src, ih, iw,
+    hipLaunchKernelGGL((k_os2_rows_f<T, R1, R2, G, NTH>), dim3(idivup(nimg * 2 * h, G)), dim3(NTH), lds, st, src, ih, iw,
                        start, tab, Y, nimg, h, tw_of<T>(e, N), out, part);
 }
 
@@ -376,7 +376,7 @@ bool os2_rows_fast(Engine &e, const cx<T> *src, int ih, int iw, const int32_t *s
         case 128: os2_rows_launch<T, 16, 8>(e, src, ih, iw, start, tab, Y, nimg, h, out, part, st); return true;
         case 136: os2_rows_launch<T, 8, 17>(e, src, ih, iw, start, tab, Y, nimg, h, out, part, st); return true;
         case 256: os2_rows_launch<T, 16, 16>(e, src, ih, iw, start, tab, Y, nimg, h, out, part, st); return true;
-        case 264: os2_rows_launch<T, 24, 11>(e, src, ih, iw, start, tab, Y, nimg, h, out, part, st); return true;
+        case 264: os2_rows_launch<T, 24, 11, 192>(e, src, ih, iw, start, tab, Y, nimg, h, out, part, st); return true;
         default: return false;
     }
 }
@@ -428,11 +428,11 @@ int row_len_fast(int sw) {
     return 0;
 }
 
-template <int R1, int R2>
+template <int R1, int R2, int G = 8, int NTH = 256>
 void rowspec_launch(Engine &e, const float *img, int nimg, int h, int w, const double *mean, cx<float> *spec,
                     hipStream_t st) {
-    constexpr int L = R1 * R2, G = 8;
-    hipLaunchKernelGGL((k_rowspec_f<float, R1, R2, G>), dim3(idivup(nimg * ((h + 1) / 2), G)), dim3(256),
+    constexpr int L = R1 * R2;
+    hipLaunchKernelGGL((k_rowspec_f<float, R1, R2, G, NTH>), dim3(idivup(nimg * ((h + 1) / 2), G)), dim3(NTH),
                        size_t(G) * (L + 1) * sizeof(cx<float>), st, img, nimg, h, w, tw_of<float>(e, L), mean, spec);
 }
 
@@ -451,7 +451,7 @@ void rowspec_fast(Engine &e, int L, const float *img, int nimg, int h, int w, co
         case 272: rowspec_launch<16, 17>(e, img, nimg, h, w, mean, spec, st); break;
         case 289: rowspec_launch<17, 17>(e, img, nimg, h, w, mean, spec, st); break;
         case 325: rowspec_launch<13, 25>(e, img, nimg, h, w, mean, spec, st); break;
-        case 528: rowspec_launch<16, 33>(e, img, nimg, h, w, mean, spec, st); break;
+        case 528: rowspec_launch<16, 33, 7, 256>(e, img, nimg, h, w, mean, spec, st); break;
         default: throw Error(AMPCOR_E_ARG, "no in-place row transform of length " + std::to_string(L));
     }
 }

@@ -323,8 +323,8 @@ __global__ void __launch_bounds__(256) k_rowspec(const T *img, int nimg, int h, 
 // The same half spectra with the row transform in place (fft_nat2dr, length
 // L = R1 * R2, read back from digit-reversed order), G row pairs per block of
 // odd LDS pitch
-template <class T, int R1, int R2, int G>
-__global__ void __launch_bounds__(256) k_rowspec_f(const T *img, int nimg, int h, int w, const cx<T> *tw,
+template <class T, int R1, int R2, int G, int NTH = 256>
+__global__ void __launch_bounds__(NTH) k_rowspec_f(const T *img, int nimg, int h, int w, const cx<T> *tw,
                                                    const double *mean, cx<T> *spec) {
     constexpr int L = R1 * R2, PR = L + 1, H1 = L / 2 + 1;
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -337,7 +337,7 @@ __global__ void __launch_bounds__(256) k_rowspec_f(const T *img, int nimg, int h
         if (threadIdx.x < nr) rowmu[threadIdx.x] = T(mean[(t0 + threadIdx.x) / np]);
         __syncthreads();
     }
-    for (int i = threadIdx.x; i < nr * L; i += 256) {
+    for (int i = threadIdx.x; i < nr * L; i += NTH) {
         const int k = i / L, x = i - k * L;
         const int t = t0 + k, win = t / np, r = 2 * (t - win * np);
         const T *im = img + size_t(win) * h * w;
@@ -349,8 +349,8 @@ __global__ void __launch_bounds__(256) k_rowspec_f(const T *img, int nimg, int h
         C[k * PR + x] = v;
     }
     __syncthreads();
-    fft_nat2dr<T, -1, R1, R2, 256>(C, nr, PR, 1, tw);
-    for (int i = threadIdx.x; i < nr * H1; i += 256) {
+    fft_nat2dr<T, -1, R1, R2, NTH>(C, nr, PR, 1, tw);
+    for (int i = threadIdx.x; i < nr * H1; i += NTH) {
         const int k = i / H1, kk = i - k * H1;
         const int t = t0 + k, win = t / np, j = t - win * np;
         const cx<T> za = C[k * PR + dr_pos<R1, R2>(kk)], bc = conj(C[k * PR + dr_pos<R1, R2>(kk ? L - kk : 0)]);
@@ -933,8 +933,8 @@ __global__ void __launch_bounds__(256) k_os2_cols_f(const cx<T> *src, int ih, in
 // out: [nimg][2h][2 R1 R2] amplitudes. With `part` (G dividing 2h, so that a
 // block holds rows of one chip only), each block also writes the FP64 sums of
 // its amplitudes and of their squares to part[2 * block], part[2 * block + 1].
-template <class T, int R1, int R2, int G>
-__global__ void __launch_bounds__(256) k_os2_rows_f(const cx<T> *src, int ih, int iw, const int32_t *start,
+template <class T, int R1, int R2, int G, int NTH = 256>
+__global__ void __launch_bounds__(NTH) k_os2_rows_f(const cx<T> *src, int ih, int iw, const int32_t *start,
                                                     const cx<double> *tab, const cx<T> *Y, int nimg, int h,
                                                     const cx<T> *tw, T *out, double *part) {
     constexpr int N = R1 * R2, PR = N + 1;
@@ -944,7 +944,7 @@ __global__ void __launch_bounds__(256) k_os2_rows_f(const cx<T> *src, int ih, in
     T *amp = reinterpret_cast<T *>(sh + N);            // [G][N]
     const int t0 = blockIdx.x * G;
     const int nr = min(G, nimg * 2 * h - t0);
-    for (int i = threadIdx.x; i < nr * N; i += 256) {
+    for (int i = threadIdx.x; i < nr * N; i += NTH) {
         const int k = i / N, c = i - k * N;
         const int t = t0 + k, win = t / (2 * h), tr = t - win * 2 * h, m = tr >> 1;
         const cx<T> v = (tr & 1) ? Y[(size_t(win) * h + m) * N + c] : chip_pixel(src, ih, iw, start, tab, win, h, N, m, c);
@@ -953,16 +953,16 @@ __global__ void __launch_bounds__(256) k_os2_rows_f(const cx<T> *src, int ih, in
     }
     half_shift_dr<T, R1, R2>(sh);
     __syncthreads();
-    fft_nat2dr<T, +1, R1, R2, 256>(C, nr, PR, 1, tw);
-    for (int i = threadIdx.x; i < nr * N; i += 256) {
+    fft_nat2dr<T, +1, R1, R2, NTH>(C, nr, PR, 1, tw);
+    for (int i = threadIdx.x; i < nr * N; i += NTH) {
         const int k = i / N, q = i - k * N;
         C[k * PR + q] = cmul(C[k * PR + q], sh[q]);
     }
     __syncthreads();
-    fft_dr2nat<T, -1, R1, R2, 256>(C, nr, PR, 1, tw);
+    fft_dr2nat<T, -1, R1, R2, NTH>(C, nr, PR, 1, tw);
     T *o = out + size_t(t0) * 2 * N;
     double s1 = 0.0, s2 = 0.0;
-    for (int i = threadIdx.x; i < nr * N; i += 256) {
+    for (int i = threadIdx.x; i < nr * N; i += NTH) {
         const int k = i / N, c = i - k * N;
         const T a0 = amp[i], a1 = amag(C[k * PR + c]);
         *reinterpret_cast<cx<T> *>(o + size_t(k) * 2 * N + 2 * c) = cx<T>{a0, a1};
