@@ -311,12 +311,12 @@ void fft2_chip(Engine &e, const cx<T> *src, int ih, int iw, const int32_t *start
 template <class T>
 const cx<T> *tw_of(Engine &e, int n) { return static_cast<const cx<T> *>(sizeof(T) == 4 ? e.plan(n).tw32.ptr : e.plan(n).tw64.ptr); }
 
-template <class T, int R1, int R2>
+template <class T, int R1, int R2, int CB = 16, int NTH = 256>
 void os2_cols_launch(Engine &e, const cx<T> *src, int ih, int iw, const int32_t *start, const cx<double> *tab,
                      int nimg, int w, cx<T> *Y, hipStream_t st) {
-    constexpr int N = R1 * R2, CB = 16;
+    constexpr int N = R1 * R2;
     const size_t lds = (size_t(N) * (CB + 1) + N) * sizeof(cx<T>);
-    hipLaunchKernelGGL((k_os2_cols_f<T, R1, R2, CB>), dim3(idivup(w, CB), nimg), dim3(256), lds, st, src, ih, iw,
+    hipLaunchKernelGGL((k_os2_cols_f<T, R1, R2, CB, NTH>), dim3(idivup(w, CB), nimg), dim3(NTH), lds, st, src, ih, iw,
                        start, tab, w, tw_of<T>(e, N), Y);
 }
 

@@ -901,29 +901,29 @@ __device__ __forceinline__ void half_shift_dr(cx<T> *sh) {
 
 // CB chip columns per block (blockIdx.x), one chip per blockIdx.y; chip height
 // R1 * R2; Y: [nimg][R1 * R2][w]
-template <class T, int R1, int R2, int CB>
-__global__ void __launch_bounds__(256) k_os2_cols_f(const cx<T> *src, int ih, int iw, const int32_t *start,
+template <class T, int R1, int R2, int CB, int NTH = 256>
+__global__ void __launch_bounds__(NTH) k_os2_cols_f(const cx<T> *src, int ih, int iw, const int32_t *start,
                                                     const cx<double> *tab, int w, const cx<T> *tw, cx<T> *Y) {
     constexpr int N = R1 * R2, PC = CB + 1;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *C = reinterpret_cast<cx<T> *>(smem_raw);   // [N][PC]: column c at C[c + r * PC]
     cx<T> *sh = C + N * PC;
     const int win = blockIdx.y, c0 = blockIdx.x * CB, nc = min(CB, w - c0);
-    for (int i = threadIdx.x; i < N * CB; i += 256) {
+    for (int i = threadIdx.x; i < N * CB; i += NTH) {
         const int r = i / CB, c = i - r * CB;
         if (c < nc) C[r * PC + c] = chip_pixel(src, ih, iw, start, tab, win, N, w, r, c0 + c);
     }
     half_shift_dr<T, R1, R2>(sh);
     __syncthreads();
-    fft_nat2dr<T, +1, R1, R2, 256>(C, nc, 1, PC, tw);
-    for (int i = threadIdx.x; i < N * nc; i += 256) {
+    fft_nat2dr<T, +1, R1, R2, NTH>(C, nc, 1, PC, tw);
+    for (int i = threadIdx.x; i < N * nc; i += NTH) {
         const int q = i / nc, c = i - q * nc;
         C[q * PC + c] = cmul(C[q * PC + c], sh[q]);
     }
     __syncthreads();
-    fft_dr2nat<T, -1, R1, R2, 256>(C, nc, 1, PC, tw);
+    fft_dr2nat<T, -1, R1, R2, NTH>(C, nc, 1, PC, tw);
     cx<T> *y = Y + size_t(win) * N * w + c0;
-    for (int i = threadIdx.x; i < N * CB; i += 256) {
+    for (int i = threadIdx.x; i < N * CB; i += NTH) {
         const int r = i / CB, c = i - r * CB;
         if (c < nc) y[size_t(r) * w + c] = C[r * PC + c];
     }
