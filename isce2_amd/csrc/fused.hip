@@ -221,13 +221,13 @@ struct OsArgs {
 };
 
 // chip (pitch N+1), shift table, deramp row/column rotation factors, reduction scratch
-__host__ __device__ inline int os_lds_bytes(int N) { return al16(N * (N + 1) * 8 + N * 8) + 2 * N * 8 + 32 * 8; }
+__host__ __device__ inline int os_lds_bytes(int N) { return al16(N * (N + 1) * 8 + N * 8) + 2 * N * 8 + 48 * 8; }
 
-template <int R1, int R2, bool MEAN>
-__global__ void __launch_bounds__(NT, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversample(OsArgs a, const cf *tw, int w0) {
+template <int R1, int R2, bool MEAN, int NTH = NT>
+__global__ void __launch_bounds__(NTH, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversample(OsArgs a, const cf *tw, int w0) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int N = R1 * R2, PC = N + 1;
-    static_assert(2 * N <= NT, "one thread per deramp row and column factor");
+    static_assert(2 * N <= NTH, "one thread per deramp row and column factor");
     cf *C = reinterpret_cast<cf *>(smem);
     cf *hsh = C + N * PC;
     cf *rowf = reinterpret_cast<cf *>(smem + al16(N * PC * 8 + N * 8));   // e^{i r phy}
@@ -238,7 +238,7 @@ __global__ void __launch_bounds__(NT, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversam
     if (a.start) { r0 += a.start[2 * win]; c0 += a.start[2 * win + 1]; }
     auto gather = [&]() {
 #pragma unroll 4
-        for (int i = threadIdx.x; i < N * N; i += NT) {
+        for (int i = threadIdx.x; i < N * N; i += NTH) {
             const int r = i / N, c = i - r * N;
             C[r * PC + c] = pixel_cx(a.img, a.mode, (r0 + r) * a.pitch + c0 + c);
         }
@@ -246,7 +246,7 @@ __global__ void __launch_bounds__(NT, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversam
     // deramp (float2's operator*= (float2.h:71-75): the imaginary part uses the new real part);
     // the phases and their sines and cosines are FP64, the factors and the rotation FP32
     auto apply_deramp = [&]() {
-        for (int i = threadIdx.x; i < N * N; i += NT) {
+        for (int i = threadIdx.x; i < N * N; i += NTH) {
             const int r = i / N, c = i - r * N;
             const cf f = cmul(colf[c], rowf[r]);
             const cf z = C[r * PC + c];
@@ -254,15 +254,15 @@ __global__ void __launch_bounds__(NT, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversam
             C[r * PC + c] = {nx, z.y * f.x + nx * f.y};
         }
     };
-    double *psum = red + 16;             // [NT/64][2] per-wave FP64 sums of |.| and |.|^2 (MEAN)
-    static_assert(16 + 2 * (NT / 64) <= 32, "wave slots fit the reduction scratch");
-    if (MEAN && threadIdx.x < 2 * (NT / 64)) psum[threadIdx.x] = 0.0;
+    double *psum = red + 16;             // [NTH/64][2] per-wave FP64 sums of |.| and |.|^2 (MEAN)
+    static_assert(16 + 2 * (NTH / 64) <= 48, "wave slots fit the reduction scratch");
+    if (MEAN && threadIdx.x < 2 * (NTH / 64)) psum[threadIdx.x] = 0.0;
     gather();
-    for (int i = threadIdx.x; i < N; i += NT) hsh[i] = a.hsh[i];
+    for (int i = threadIdx.x; i < N; i += NTH) hsh[i] = a.hsh[i];
     __syncthreads();
     if (a.deramp == 1) {
         double ayr = 0, ayi = 0, axr = 0, axi = 0;
-        for (int i = threadIdx.x; i < N * N; i += NT) {
+        for (int i = threadIdx.x; i < N * N; i += NTH) {
             const int r = i / N, c = i - r * N;
             const cf z = C[r * PC + c];
             const double zr = z.x, zi = z.y;
@@ -275,8 +275,8 @@ __global__ void __launch_bounds__(NT, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversam
                 axr += zr * b.x + zi * b.y; axi += zi * b.x - zr * b.y;
             }
         }
-        ayr = block_sum(ayr, red); ayi = block_sum(ayi, red);
-        axr = block_sum(axr, red); axi = block_sum(axi, red);
+        ayr = block_sum<NTH>(ayr, red); ayi = block_sum<NTH>(ayi, red);
+        axr = block_sum<NTH>(axr, red); axi = block_sum<NTH>(axi, red);
         const double phy = atan2(ayi, ayr), phx = atan2(axi, axr);
         // the phase c*phx + r*phy is separable: one FP64 sincos per row and per column
         if (threadIdx.x < 2 * N) {
@@ -297,7 +297,7 @@ __global__ void __launch_bounds__(NT, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversam
     auto emit = [&](int p, int q) {      // |C| -> output pixels (2r+p, 2c+q), four columns per thread
         double s1 = 0.0, s2 = 0.0;
         float *plane = out + size_t(2 * p + q) * N * N;           // whole lines per phase
-        for (int i = threadIdx.x; i < N * N / 4; i += NT) {
+        for (int i = threadIdx.x; i < N * N / 4; i += NTH) {
             const int r = (4 * i) / N, c = 4 * i - r * N;
             float amp[4];
 #pragma unroll
@@ -324,14 +324,14 @@ __global__ void __launch_bounds__(NT, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversam
         }
     };
     auto interp_cols = [&]() {          // C <- C at half-row positions (every column)
-        fft_nat2dr<float, +1, R1, R2, NT>(C, N, 1, PC, tw);
-        for (int i = threadIdx.x; i < N * N; i += NT) {
+        fft_nat2dr<float, +1, R1, R2, NTH>(C, N, 1, PC, tw);
+        for (int i = threadIdx.x; i < N * N; i += NTH) {
             const int k = i / N, c = i - k * N;
             cf &z = C[dr_pos<R1, R2>(k) * PC + c];
             z = cmul(z, hsh[k]);
         }
         __syncthreads();
-        fft_dr2nat<float, -1, R1, R2, NT>(C, N, 1, PC, tw);
+        fft_dr2nat<float, -1, R1, R2, NTH>(C, N, 1, PC, tw);
     };
     // X: even rows, even columns; then its column interpolation: odd rows, even columns
     emit(0, 0);
@@ -346,14 +346,14 @@ __global__ void __launch_bounds__(NT, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversam
         apply_deramp();
         __syncthreads();
     }
-    fft_nat2dr<float, +1, R1, R2, NT>(C, N, PC, 1, tw);
-    for (int i = threadIdx.x; i < N * N; i += NT) {
+    fft_nat2dr<float, +1, R1, R2, NTH>(C, N, PC, 1, tw);
+    for (int i = threadIdx.x; i < N * N; i += NTH) {
         const int r = i / N, k = i - r * N;
         cf &z = C[r * PC + dr_pos<R1, R2>(k)];
         z = cmul(z, hsh[k]);
     }
     __syncthreads();
-    fft_dr2nat<float, -1, R1, R2, NT>(C, N, PC, 1, tw);
+    fft_dr2nat<float, -1, R1, R2, NTH>(C, N, PC, 1, tw);
     emit(0, 1);
     __syncthreads();
     interp_cols();
@@ -363,7 +363,7 @@ __global__ void __launch_bounds__(NT, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversam
         __syncthreads();
         double sum = 0.0, sum2 = 0.0;
 #pragma unroll
-        for (int w = 0; w < NT / 64; ++w) {
+        for (int w = 0; w < NTH / 64; ++w) {
             sum += psum[2 * w];
             sum2 += psum[2 * w + 1];
         }
