@@ -168,8 +168,9 @@ class ChipPairBatch:
 
     def _rerun_overflow(self):
         """Host-driven FP64 re-run of the near ties beyond each class's in-step capacity."""
-        mc = torch.cat([c.out["margin_coarse"] for c in self.classes]).cpu().numpy()
-        mf = torch.cat([c.out["margin_fine"] for c in self.classes]).cpu().numpy()
+        both = torch.cat([c.out["margin_coarse"] for c in self.classes] +
+                         [c.out["margin_fine"] for c in self.classes]).cpu().numpy()   # one copy back
+        mc, mf = both[:self.n], both[self.n:]
         near = ~(mc >= self.tau) | ~(mf >= self.tau)
         self.rerun_count = 0
         main = torch.cuda.current_stream(self.dev)
