@@ -9,7 +9,9 @@ with start pixels ``(i*W, 0)`` and ``(i*S, 0)``. Each class has its own engine
 context (resident plans and buffers) and a schedule chosen by the autotuner
 (fused kernels or staged pipeline, chunk, FFT radix orders).
 
-One step = every class's launches in no-sync mode; with ``graph=True`` the
+One step = every class's launches in no-sync mode, each class on a side
+stream of its own (``concurrent=True``; forked from and joined to the step's
+stream) so that small classes run beside large ones; with ``graph=True`` the
 step is captured once into a HIP graph (``torch.cuda.CUDAGraph``) and
 replayed. Under precision 1 the windows whose argmax margins fall below the
 thresholds are re-run through the FP64 pipeline inside the same step: right
@@ -83,7 +85,7 @@ class ChipPairBatch:
     """
 
     def __init__(self, groups, device=0, deramp=1, precision=1, tau=TAU_DEFAULT, graph=True,
-                 tuner: Autotuner | None = None):
+                 tuner: Autotuner | None = None, concurrent: bool = True):
         self.device = device
         self.dev = torch.device("cuda", device)
         self.precision = precision
@@ -98,6 +100,9 @@ class ChipPairBatch:
                 c.sched.apply(c.eng)
                 c.eng.setup(c.params)
             c.eng.set_precision(2 if precision == 2 else 0)
+        self.concurrent = concurrent
+        for c in self.classes:
+            c.main_stream = torch.cuda.Stream(device=self.dev) if concurrent else None
         if precision == 1:
             for c in self.classes:
                 c.rerun_eng = Engine(device)
@@ -136,7 +141,22 @@ class ChipPairBatch:
         main = torch.cuda.current_stream(self.dev)
         # largest windows first: their FP64 re-run chains, the longest, then overlap the
         # smaller classes' launches instead of trailing the step
-        for c in sorted(self.classes, key=lambda k: (-k.W, -k.hs)):
+        order = sorted(self.classes, key=lambda k: (-k.W, -k.hs))
+        if self.concurrent:
+            # every class on a stream of its own (its re-run after it on another), so
+            # the small classes' launches fill the GPU beside the large ones
+            for c in order:
+                c.main_stream.wait_stream(main)
+                with torch.cuda.stream(c.main_stream):
+                    c.launch(nosync)
+                if self.precision == 1:
+                    c.rerun_stream.wait_stream(c.main_stream)
+                    with torch.cuda.stream(c.rerun_stream):
+                        self._rerun_in_step(c)
+            for c in order:
+                main.wait_stream(c.rerun_stream if self.precision == 1 else c.main_stream)
+            return
+        for c in order:
             c.launch(nosync)
             if self.precision == 1:
                 s = c.rerun_stream

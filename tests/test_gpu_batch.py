@@ -159,3 +159,17 @@ def test_captured_context_refuses_to_grow(groups):
     with pytest.raises(Exception, match="captured"):
         c.rerun_eng.run_windows(c.ref, c.sec, *st, mode=MODE_STAGED, debug=True, nosync=True)
     b.close()
+
+
+def test_concurrent_classes_equal_sequential(groups):
+    """Classes launched on side streams of their own give the sequential step's results."""
+    a = ChipPairBatch(groups, device=0, graph=True, concurrent=True)
+    b = ChipPairBatch(groups, device=0, graph=True, concurrent=False)
+    for _ in range(2):
+        a.run()
+        b.run()
+        ra, rb = a.results(), b.results()
+        for k in ra:
+            assert np.array_equal(ra[k], rb[k], equal_nan=True), k
+    a.close()
+    b.close()
