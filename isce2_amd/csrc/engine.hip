@@ -792,9 +792,10 @@ __global__ void k_take_starts(const int32_t *rd, const int32_t *ra, const int32_
 // Windows to re-run in FP64, in order: first every window whose coarse margin
 // is below tc (or whose fine margin is below tf when no zoom starts are given:
 // has_start 0), then every other window whose fine margin is below tf (NaN
-// counts as below). cnt = {first kind, both kinds}. One workgroup of 1024
-// threads: per 1024-window slice, a ballot per wave and the wave totals give
-// each selected window its slot.
+// counts as below). cnt[0] = first kind, cnt[1] = both kinds. One workgroup of
+// 1024 threads; per slice of 4096 windows each thread takes four consecutive
+// ones, and a wave prefix sum of the per-thread counts plus the wave totals
+// gives each selected window its slot.
 __global__ void __launch_bounds__(1024) k_select_rerun(const float *mc, const float *mf, int n, float tc, float tf,
                                                        int has_start, int32_t *idx, int32_t *cnt) {
     __shared__ int wtot[16];
@@ -802,17 +803,26 @@ __global__ void __launch_bounds__(1024) k_select_rerun(const float *mc, const fl
     int filled = 0;                     // the same in every thread
     int first = 0;
     for (int pass = 0; pass < 2; ++pass) {
-        for (int c0 = 0; c0 < n; c0 += blockDim.x) {
-            const int i = c0 + threadIdx.x;
-            bool flag = false;
-            if (i < n) {
-                const bool c = !(mc[i] >= tc), f = !(mf[i] >= tf);
-                const bool full = c || (f && !has_start);
-                flag = pass == 0 ? full : (!full && f);
+        for (int c0 = 0; c0 < n; c0 += 4 * blockDim.x) {
+            const int i0 = c0 + 4 * threadIdx.x;
+            unsigned flags = 0;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int i = i0 + j;
+                if (i < n) {
+                    const bool c = !(mc[i] >= tc), f = !(mf[i] >= tf);
+                    const bool full = c || (f && !has_start);
+                    if (pass == 0 ? full : (!full && f)) flags |= 1u << j;
+                }
             }
-            const unsigned long long b = __ballot(flag);
-            const int before = __popcll(b & ((1ull << lane) - 1ull));
-            if (lane == 0) wtot[wv] = __popcll(b);
+            const int mine = __popc(flags);
+            int incl = mine;                // inclusive prefix over the wave
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const int t = __shfl_up(incl, o);
+                if (lane >= o) incl += t;
+            }
+            if (lane == 63) wtot[wv] = incl;
             __syncthreads();
             int off = 0, tot = 0;
             for (int k = 0; k < nw; ++k) {
@@ -820,7 +830,10 @@ __global__ void __launch_bounds__(1024) k_select_rerun(const float *mc, const fl
                 if (k < wv) off += t;
                 tot += t;
             }
-            if (flag) idx[filled + off + before] = i;
+            int slot = filled + off + incl - mine;
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (flags & (1u << j)) idx[slot++] = i0 + j;
             filled += tot;
             __syncthreads();
         }
@@ -867,13 +880,16 @@ void Engine::rerun_fp64(const RunArgs &a, const float *mc, const float *mf, cons
     // pipeline first, then those re-run from the zoom stage on (their coarse
     // results, SNR and covariance stay the FP32 ones); only the two counts
     // come back to the host
+    // p_count[2]: the first window whose chips leave their images (a.n if none),
+    // set by Engine::run's bounds check
     int32_t *d_idx = buf<int32_t>("p_idx", a.n);
-    int32_t *d_cnt = buf<int32_t>("p_count", 2);
+    int32_t *d_cnt = buf<int32_t>("p_count", 3);
     hipLaunchKernelGGL(k_select_rerun, dim3(1), dim3(1024), 0, a.stream, mc, mf, a.n, tau_coarse, tau_fine,
                        start32 != nullptr, d_idx, d_cnt);
-    int32_t cnt[2];
+    int32_t cnt[3];
     HIP_OK(hipMemcpyAsync(cnt, d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, a.stream));
     HIP_OK(hipStreamSynchronize(a.stream));
+    if (cnt[2] < a.n) throw Error(AMPCOR_E_RANGE, "window " + std::to_string(cnt[2]) + " reads outside its image");
     const int m_full = cnt[0], m = cnt[1];
     prec_checked += a.n;
     prec_rerun += m;
@@ -960,6 +976,21 @@ void Engine::run(const RunArgs &in) {
         hipLaunchKernelGGL(k_sanitize_starts<int32_t>, dim3(idivup(a.n, 256)), dim3(256), 0, a.stream, a.ref_d, a.ref_a,
                            a.sec_d, a.sec_a, a.n, a.ref_h, a.ref_w, a.sec_h, a.sec_w, p.window_h, p.window_w,
                            sz.search_h, sz.search_w, st4, bad);
+        a.ref_d = st4; a.ref_a = st4 + a.n; a.sec_d = st4 + 2 * size_t(a.n); a.sec_a = st4 + 3 * size_t(a.n);
+    } else if (precision == 1) {
+        // bounds checked on the device and reported with the FP64 re-run's counts
+        // (one host synchronization per call): out-of-image windows read window
+        // (0, 0) meanwhile, and the call fails with AMPCOR_E_RANGE before the re-run
+        int32_t *cnt = buf<int32_t>("p_count", 3);
+        int32_t *st4 = buf<int32_t>("ns_starts", 4 * size_t(a.n));
+        int32_t *flags = buf<int32_t>("ns_bad", a.n);
+        hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(1), 0, a.stream, cnt + 2, int(a.n));
+        hipLaunchKernelGGL(k_first_bad, dim3(idivup(a.n, 256)), dim3(256), 0, a.stream, a.ref_d, a.ref_a, a.sec_d,
+                           a.sec_a, int(a.n), a.ref_h, a.ref_w, a.sec_h, a.sec_w, p.window_h, p.window_w,
+                           sz.search_h, sz.search_w, cnt + 2);
+        hipLaunchKernelGGL(k_sanitize_starts<int32_t>, dim3(idivup(a.n, 256)), dim3(256), 0, a.stream, a.ref_d, a.ref_a,
+                           a.sec_d, a.sec_a, a.n, a.ref_h, a.ref_w, a.sec_h, a.sec_w, p.window_h, p.window_w,
+                           sz.search_h, sz.search_w, st4, flags);
         a.ref_d = st4; a.ref_a = st4 + a.n; a.sec_d = st4 + 2 * size_t(a.n); a.sec_a = st4 + 3 * size_t(a.n);
     } else {
         // bounds: every chip must lie inside its image, checked on the device (the
