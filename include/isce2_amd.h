@@ -102,7 +102,12 @@ int ampcor_setup(ampcor_ctx *ctx, const ampcor_params *params, ampcor_sizes *siz
  *   chunk : windows per launch (0 = engine default: 1024 for the staged
  *           pipeline, 32768 for the fused kernels); mode : AMPCOR_MODE_*.
  *   stream : hipStream_t (NULL = default stream). The call is asynchronous
- *   with respect to the host; synchronize the stream before reading outputs. */
+ *   with respect to the host; synchronize the stream before reading outputs.
+ * A window whose chips leave their image fails the call with AMPCOR_E_RANGE
+ * (the lowest such window is named). Under precision 0 and 2 this is checked
+ * before any kernel runs; under precision 1 it is reported with the FP64
+ * re-run's window counts (the call's one host synchronization), after the
+ * FP32 kernels have run on window (0, 0) in its place, and before the re-run. */
 int ampcor_run_windows(ampcor_ctx *ctx,
                        const void *ref_img, int64_t ref_h, int64_t ref_w, int64_t ref_pitch,
                        const void *sec_img, int64_t sec_h, int64_t sec_w, int64_t sec_pitch,
