@@ -46,7 +46,7 @@ def _case(seed):
     return wh, ww, hd, ha, deramp, real
 
 
-@pytest.mark.parametrize("seed", range(12))
+@pytest.mark.parametrize("seed", range(40))
 def test_random_config_matches_oracle(eng, scene, seed):
     wh, ww, hd, ha, deramp, real = _case(seed)
     ref, sec = scene
