@@ -132,11 +132,8 @@ __global__ void __launch_bounds__(NT, (K >= 9 ? 2 : 4)) k_coarse(CoarseArgs a, c
     // chunk's region, not loaded yet, holds them: 2 * ceil(SH/16) * SW doubles)
     static_assert(W % NORM_BLK == 0, "template height a multiple of the norm block");
     static_assert(2 * ((L + NORM_BLK - 1) / NORM_BLK) * L <= nTc * P, "block sums fit the template chunk region");
-    window_norms_blk<(L + 63) / 64>(
-        [&](int r, int c) -> double {
-            const cf v = S[(r >> 1) * P + c];
-            return (r & 1) ? v.y : v.x;
-        },
+    window_norms_blk<(L + 63) / 64, NT, W % 64 == 0 ? W / 64 : 0>(
+        [&](int r, int c) -> double { return reinterpret_cast<const float *>(S)[2 * ((r >> 1) * P + c) + (r & 1)]; },
         SH, SW, W, W, NLh, NLw, t2, nrm, reinterpret_cast<double *>(Tb));
     __syncthreads();
 
@@ -591,11 +588,8 @@ __global__ void __launch_bounds__(NTH) k_fine(FineArgs a, const cf *tw, int w0) 
         tv[k] = {0.f, 0.f};
         if ((TN % NTH == 0 || i < TN) && x < T) tv[k] = {t0[0], t0[size_t(2) * (T / 2) * (T / 2)]};
     }
-    window_norms<(P + 63) / 64, NTH>(
-        [&](int r, int c) -> double {
-            const cf v = S[(r >> 1) * PP + c];
-            return (r & 1) ? v.y : v.x;
-        },
+    window_norms<(P + 63) / 64, NTH, T % 64 == 0 ? T / 64 : 0>(
+        [&](int r, int c) -> double { return reinterpret_cast<const float *>(S)[2 * ((r >> 1) * PP + c) + (r & 1)]; },
         P, T, T, NL, NL, t2, nrm, reinterpret_cast<double *>(Tb));   // template not loaded yet
     __syncthreads();
     // template rows minus the FP64 mean, row pairs packed, zero beyond column T
@@ -773,7 +767,7 @@ __global__ void __launch_bounds__(NTH, 2) k_fine2(FineArgs a, const cf *tw, int 
             if (u0 + k < NL) Qs[(u0 + k) * H + f] = q[k];
     }
     __syncthreads();
-    window_norms_rows<(P + 63) / 64, NTH>(
+    window_norms_rows<(P + 63) / 64, NTH, T % 64 == 0 ? T / 64 : 0>(
         [&](int r, int c) -> double { return double(r < NS ? save[r * P + c] : save[(NS + r - T) * P + c]); },
         col, 1, P, T, T, NL, NL, t2, nrm);
     inverse_lags<R1, R2, NTH>(Qs, NL, NL, SR + ((NL * H + 1) & ~1), PP, tw, nrm, corr);

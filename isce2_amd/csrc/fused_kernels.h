@@ -79,7 +79,7 @@ __device__ __forceinline__ double block_sum(double v, double *sh) {
 // ---------------------------------------------------------------------------
 constexpr int NORM_BLOCKS = 4;
 
-// DPP move of an FP64 value (both halves); lanes the control leaves out read 0
+// DPP move of an FP64 value (both halves); lanes the row mask leaves out keep 0
 template <int CTRL, int ROWMASK>
 __device__ __forceinline__ double dpp_f64(double x) {
     const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(x), CTRL, ROWMASK, 0xf, false);
@@ -87,13 +87,23 @@ __device__ __forceinline__ double dpp_f64(double x) {
     return __hiloint2double(hi, lo);
 }
 
+// DPP row shift of an FP64 value with every lane written: lanes whose source
+// lies before the start of their row read 0 (bound_ctrl), so no zeroed
+// destination is needed
+template <int CTRL>
+__device__ __forceinline__ double dpp_shr_f64(double x) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+
 // inclusive wave64 prefix sum: row_shr 1, 2, 4, 8 within rows of 16 lanes, then
 // row_bcast:15 into rows 1 and 3 and row_bcast:31 into rows 2 and 3
 __device__ __forceinline__ double wave_scan_f64(double x) {
-    x += dpp_f64<0x111, 0xf>(x);
-    x += dpp_f64<0x112, 0xf>(x);
-    x += dpp_f64<0x114, 0xf>(x);
-    x += dpp_f64<0x118, 0xf>(x);
+    x += dpp_shr_f64<0x111>(x);
+    x += dpp_shr_f64<0x112>(x);
+    x += dpp_shr_f64<0x114>(x);
+    x += dpp_shr_f64<0x118>(x);
     x += dpp_f64<0x142, 0xa>(x);
     x += dpp_f64<0x143, 0xc>(x);
     return x;
@@ -110,7 +120,15 @@ __device__ __forceinline__ double lane63_f64(double x) {
 // are blocks of blk rows from row 0 down the whole search (TH a multiple of
 // blk), and a wave starts from the TH / blk blocks at or above its first lag
 // row, sliding at most blk - 1 rows. amp(r, c) is read for the slides only.
-template <int CPL, int NTH, class Amp>
+//
+// Horizontal window sums of the column sums g (lane l holds columns l + 64m):
+// * TWB = 0 (any TW): differences of the wave prefix over all SW columns,
+//   read across lanes;
+// * TWB = TW / 64 > 0: window(v) = sum_{c<TW} g[c] + sum_{j<v} (g[j+TW] - g[j]),
+//   and column j + TW sits in the same lane as column j, so each 64-column
+//   block of lag columns costs one wave scan of the differences plus one wave
+//   sum of the first TW columns per lag row, with no value read across lanes.
+template <int CPL, int NTH, int TWB = 0, class Amp>
 __device__ __forceinline__ void window_norms_rows(Amp amp, const double *part, int NB, int SW, int TH, int TW,
                                                   int NLh, int NLw, double t2, float *nrm, int blk = 0) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -150,6 +168,28 @@ __device__ __forceinline__ void window_norms_rows(Amp amp, const double *part, i
     const double inv_n = 1.0 / double(TH * TW);
     #pragma unroll 1
     for (int u = u_beg; u < u_end; ++u) {
+        if constexpr (TWB > 0) {
+            static_assert(TWB < CPL, "lanes hold a column beyond the first TW");
+            double e1 = g1[0], e2 = g2[0];
+#pragma unroll
+            for (int m = 1; m < TWB; ++m) { e1 += g1[m]; e2 += g2[m]; }
+            double c1 = lane63_f64(wave_scan_f64(e1)), c2 = lane63_f64(wave_scan_f64(e2));
+#pragma unroll
+            for (int k = 0; k + TWB <= CPL; ++k) {
+                if (64 * k >= NLw) break;
+                const double h1 = k + TWB < CPL ? g1[k + TWB] : 0.0, h2 = k + TWB < CPL ? g2[k + TWB] : 0.0;
+                const double d1 = h1 - g1[k], d2 = h2 - g2[k];
+                const double x1 = wave_scan_f64(d1), x2 = wave_scan_f64(d2);
+                const int v = lane + 64 * k;
+                if (v < NLw) {
+                    const double s1 = c1 + (x1 - d1), s2 = c2 + (x2 - d2);
+                    const double norm2 = (s2 - s1 * s1 * inv_n) * t2;
+                    nrm[u * NLw + v] = __builtin_amdgcn_rsqf(float(norm2 + double(FLT_EPSILON)));
+                }
+                c1 += lane63_f64(x1);
+                c2 += lane63_f64(x2);
+            }
+        } else {
         double p1[CPL], p2[CPL];
         double c1 = 0.0, c2 = 0.0;
 #pragma unroll
@@ -179,6 +219,7 @@ __device__ __forceinline__ void window_norms_rows(Amp amp, const double *part, i
                 nrm[u * NLw + v] = __builtin_amdgcn_rsqf(float(norm2 + double(FLT_EPSILON)));
             }
         }
+        }
         if (u + 1 < u_end) {
 #pragma unroll
             for (int m = 0; m < CPL; ++m) {
@@ -198,7 +239,7 @@ __device__ __forceinline__ void window_norms_rows(Amp amp, const double *part, i
 // doubles), so that no wave slides far to its first lag row. TH must be a
 // multiple of NORM_BLK.
 constexpr int NORM_BLK = 16;
-template <int CPL, int NTH = NT, class Amp>
+template <int CPL, int NTH = NT, int TWB = 0, class Amp>
 __device__ __forceinline__ void window_norms_blk(Amp amp, int SH, int SW, int TH, int TW, int NLh, int NLw,
                                                  double t2, float *nrm, double *part) {
     const int NB = (SH + NORM_BLK - 1) / NORM_BLK;
@@ -221,10 +262,10 @@ __device__ __forceinline__ void window_norms_blk(Amp amp, int SH, int SW, int TH
         part[(NB + b) * SW + c] = a2 + b2;
     }
     __syncthreads();
-    window_norms_rows<CPL, NTH>(amp, part, NB, SW, TH, TW, NLh, NLw, t2, nrm, NORM_BLK);
+    window_norms_rows<CPL, NTH, TWB>(amp, part, NB, SW, TH, TW, NLh, NLw, t2, nrm, NORM_BLK);
 }
 
-template <int CPL, int NTH = NT, class Amp>
+template <int CPL, int NTH = NT, int TWB = 0, class Amp>
 __device__ __forceinline__ void window_norms(Amp amp, int SW, int TH, int TW, int NLh, int NLw, double t2, float *nrm,
                              double *part) {
     constexpr int NB = NORM_BLOCKS;
@@ -248,7 +289,7 @@ __device__ __forceinline__ void window_norms(Amp amp, int SW, int TH, int TW, in
         part[(NB + b) * SW + c] = a2 + b2;
     }
     __syncthreads();
-    window_norms_rows<CPL, NTH>(amp, part, NB, SW, TH, TW, NLh, NLw, t2, nrm);
+    window_norms_rows<CPL, NTH, TWB>(amp, part, NB, SW, TH, TW, NLh, NLw, t2, nrm);
 }
 
 // ---------------------------------------------------------------------------
