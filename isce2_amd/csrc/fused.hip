@@ -639,15 +639,14 @@ __global__ void __launch_bounds__(NTH) k_fine(FineArgs a, const cf *tw, int w0) 
 // arrive, with the first and last NL - 1 rows kept for the slides.
 // ---------------------------------------------------------------------------
 struct Fine2Lds {
-    int sr, tb, save, col, nrm, corr, red, total;   // bytes
+    int sr, tb, save, nrm, corr, red, total;   // bytes
     __host__ __device__ Fine2Lds(int P, int NL, int TC) {
         const int PP = P + 1;
         const int srp = (TC + NL - 1 + 1) / 2, tcp = TC / 2;
         sr = 0;
         tb = sr + srp * PP * 8;                    // template pairs directly after the search pairs
         save = tb + tcp * PP * 8;
-        col = al16(save + 2 * (NL - 1) * P * 4);
-        nrm = col + 2 * P * 8;
+        nrm = al16(save + 2 * (NL - 1) * P * 4);
         corr = al16(nrm + NL * NL * 4);
         red = al16(corr + NL * NL * 4);
         total = red + 64 * 8;
@@ -667,12 +666,12 @@ __global__ void __launch_bounds__(NTH, 2) k_fine2(FineArgs a, const cf *tw, int 
     constexpr int NS = NL - 1;               // rows kept at each end for the slides
     constexpr int Q = P / 2, TQ = T / 2;     // polyphase plane sides
     static_assert(T % TC == 0 && TC % 2 == 0 && NS % 2 == 0 && NS == 16, "even chunks, 16x16 zoom surface");
-    static_assert(2 * P <= NTH, "one thread per column and sum");
+    static_assert(3 * P <= NTH, "one thread per column and row-pair phase");
+    static_assert(6 * P * 8 <= (TC / 2) * (P + 1) * 8, "column-sum partials fit the template chunk region");
     const Fine2Lds lay(P, NL, TC);
     cf *SR = reinterpret_cast<cf *>(smem + lay.sr);
     cf *TB = reinterpret_cast<cf *>(smem + lay.tb);
     float *save = reinterpret_cast<float *>(smem + lay.save);   // rows [0, NS) then [T, T + NS), pitch P
-    double *col = reinterpret_cast<double *>(smem + lay.col);   // column sums of s, then s^2, over rows [0, T)
     float *nrm = reinterpret_cast<float *>(smem + lay.nrm);
     float *corr = reinterpret_cast<float *>(smem + lay.corr);
     double *red = reinterpret_cast<double *>(smem + lay.red);
@@ -687,8 +686,11 @@ __global__ void __launch_bounds__(NTH, 2) k_fine2(FineArgs a, const cf *tw, int 
     auto tval = [&](int r, int x) -> float {
         return tp[size_t(2 * (r & 1) + (x & 1)) * TQ * TQ + (r >> 1) * TQ + (x >> 1)];
     };
-    const int csum_c = threadIdx.x % P, csum_k = threadIdx.x / P;    // column and sum (0: s, 1: s^2)
-    double cacc = 0.0;
+    // FP64 column sums of s and s^2 over rows [0, T): thread (column, phase)
+    // takes the row pairs whose index is phase mod 3, both rows of a pair from
+    // one load; the three partials are added in phase order by the window sums
+    const int cs_c = threadIdx.x % P, cs_ph = threadIdx.x / P;
+    double cs1 = 0.0, cs2 = 0.0;
 
     const int f = threadIdx.x % H, blk = threadIdx.x / H, u0 = blk * K;
     const bool active = (blk < NTH / H) && (u0 < NL);
@@ -734,16 +736,25 @@ __global__ void __launch_bounds__(NTH, 2) k_fine2(FineArgs a, const cf *tw, int 
             }
         }
         __syncthreads();
-        // column sums of the new rows below T; the first and last NS rows kept
-        if (csum_k < 2) {
-            for (int rr = 2 * first; rr < SROWS; ++rr) {
-                const int r = c0 + rr;
-                const cf pv = SR[(rr >> 1) * PP + csum_c];
-                const float v = (rr & 1) ? pv.y : pv.x;
-                if (r < T) cacc += csum_k ? double(v) * v : double(v);
-                if (csum_k == 0) {
-                    if (r < NS) save[r * P + csum_c] = v;
-                    else if (r >= T) save[(NS + r - T) * P + csum_c] = v;
+        // column sums of the new row pairs below T; the first and last NS rows kept
+        // (T and NS are even, so a pair never straddles either boundary)
+        if (cs_ph < 3) {
+            const int g0 = (c0 >> 1) + first, g1 = (c0 >> 1) + SRP;
+            #pragma unroll 1
+            for (int gp = g0 + (cs_ph - g0 % 3 + 3) % 3; gp < g1; gp += 3) {
+                const cf pv = SR[(gp - (c0 >> 1)) * PP + cs_c];
+                const int r = 2 * gp;
+                if (r < T) {
+                    const double x = pv.x, y = pv.y;
+                    cs1 += x + y;
+                    cs2 += fma(x, x, y * y);
+                }
+                if (r < NS) {
+                    save[r * P + cs_c] = pv.x;
+                    save[(r + 1) * P + cs_c] = pv.y;
+                } else if (r >= T) {
+                    save[(NS + r - T) * P + cs_c] = pv.x;
+                    save[(NS + r + 1 - T) * P + cs_c] = pv.y;
                 }
             }
         }
@@ -759,7 +770,12 @@ __global__ void __launch_bounds__(NTH, 2) k_fine2(FineArgs a, const cf *tw, int 
             __syncthreads();
         }
     }
-    if (csum_k < 2) col[csum_k * P + csum_c] = cacc;
+    // the template region is free once the last chunk's sums are done
+    double *part = reinterpret_cast<double *>(TB);
+    if (cs_ph < 3) {
+        part[cs_ph * P + cs_c] = cs1;
+        part[(3 + cs_ph) * P + cs_c] = cs2;
+    }
     cf *Qs = SR;
     if (active) {
 #pragma unroll
@@ -769,7 +785,7 @@ __global__ void __launch_bounds__(NTH, 2) k_fine2(FineArgs a, const cf *tw, int 
     __syncthreads();
     window_norms_rows<(P + 63) / 64, NTH, T % 64 == 0 ? T / 64 : 0>(
         [&](int r, int c) -> double { return double(r < NS ? save[r * P + c] : save[(NS + r - T) * P + c]); },
-        col, 1, P, T, T, NL, NL, t2, nrm);
+        part, 3, P, T, T, NL, NL, t2, nrm);
     inverse_lags<R1, R2, NTH>(Qs, NL, NL, SR + ((NL * H + 1) & ~1), PP, tw, nrm, corr);
     fine_surface<NTH>(a, corr, NL, SR, SR + 16 * 256, reinterpret_cast<int *>(nrm), red, win);
 }
