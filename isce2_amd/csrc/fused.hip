@@ -697,6 +697,12 @@ __global__ void __launch_bounds__(NTH, 2) k_fine2(FineArgs a, const cf *tw, int 
     cf q[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) q[k] = {0.f, 0.f};
+    // the MAC lanes all take the complex form; slot 0's (DC, Nyquist) lag sums come
+    // from one thread per lag and component in a wave the MAC leaves idle
+    constexpr int HB = ((NL + K - 1) / K * H + 63) / 64 * 64;
+    static_assert(NTH - HB >= 2 * NL, "an idle wave holds the slot-0 sums");
+    const int hid = int(threadIdx.x) - HB;
+    float dcn = 0.f;
 
     #pragma unroll 1
     for (int ch = 0; ch < NCH; ++ch) {
@@ -762,7 +768,8 @@ __global__ void __launch_bounds__(NTH, 2) k_fine2(FineArgs a, const cf *tw, int 
         // row spectra of the new search pairs and the template pairs (contiguous in LDS)
         fft_nat2dr<float, -1, R1, R2, NTH>(SR + first * PP, SRP - first + TCP, PP, 1, tw);
         split_pairs_inplace<R1, R2, NTH>(SR + first * PP, SRP - first + TCP, PP);
-        if (active) mac_rows<H, K>(q, SR, PP, TB, PP, 0, 0, TC, u0, SROWS, f);
+        if (active) mac_rows<H, K, false>(q, SR, PP, TB, PP, 0, 0, TC, u0, SROWS, f);
+        else if (hid >= 0 && hid < 2 * NL) dcn = dc_lag_sum<H>(dcn, SR, PP, TB, PP, TC, hid >> 1, SROWS, hid & 1);
         __syncthreads();
         if (ch + 1 < NCH) {
             // the last KEEP pairs are the next chunk's first
@@ -777,11 +784,12 @@ __global__ void __launch_bounds__(NTH, 2) k_fine2(FineArgs a, const cf *tw, int 
         part[(3 + cs_ph) * P + cs_c] = cs2;
     }
     cf *Qs = SR;
-    if (active) {
+    if (active && f != 0) {
 #pragma unroll
         for (int k = 0; k < K; ++k)
             if (u0 + k < NL) Qs[(u0 + k) * H + f] = q[k];
     }
+    if (hid >= 0 && hid < 2 * NL) reinterpret_cast<float *>(Qs + (hid >> 1) * H)[hid & 1] = dcn;
     __syncthreads();
     window_norms_rows<(P + 63) / 64, NTH, T % 64 == 0 ? T / 64 : 0>(
         [&](int r, int c) -> double { return double(r < NS ? save[r * P + c] : save[(NS + r - T) * P + c]); },

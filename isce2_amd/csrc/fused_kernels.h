@@ -348,12 +348,14 @@ __device__ __forceinline__ int row_slot(int r, int P) { return (r >> 1) * P + (r
 
 // ---------------------------------------------------------------------------
 // MAC over template rows [i_beg, i_end): q[k] += conj(T_i[f]) S_{i+u0+k}[f]
-// (slot 0 packs two real bins: (DC, Nyquist) multiply component-wise)
+// (slot 0 packs two real bins: (DC, Nyquist) multiply component-wise; with
+// DC = false every lane takes the complex form and the caller computes slot 0's
+// sums elsewhere: dc_lag_sum)
 // ---------------------------------------------------------------------------
-template <int H, int K>
+template <int H, int K, bool DC = true>
 __device__ __forceinline__ void mac_rows(cf (&q)[K], const cf *S, int Sp, const cf *Tsp, int Tp, int t_off,
                                          int i_beg, int i_end, int u0, int SH, int f) {
-    const bool dc = f == 0;
+    const bool dc = DC && f == 0;
     #pragma unroll 1
     for (int i0 = i_beg; i0 < i_end; i0 += 8) {
         cf sw[K + 7];
@@ -379,6 +381,21 @@ __device__ __forceinline__ void mac_rows(cf (&q)[K], const cf *S, int Sp, const 
             }
         }
     }
+}
+
+// Slot 0 of mac_rows for one lag u and one component (0: DC, 1: Nyquist):
+// acc += T_i[0].comp * S_{i+u}[0].comp over template rows [0, n), in the order and
+// with the zero rows (i + u >= SH) of mac_rows' slot-0 lane, so the sums agree bit for bit
+template <int H>
+__device__ __forceinline__ float dc_lag_sum(float acc, const cf *S, int Sp, const cf *Tsp, int Tp, int n, int u,
+                                            int SH, int comp) {
+    const float *s = reinterpret_cast<const float *>(S) + comp, *t = reinterpret_cast<const float *>(Tsp) + comp;
+    #pragma unroll 8
+    for (int i = 0; i < n; ++i) {
+        const float sv = (i + u < SH) ? s[2 * row_slot<H>(i + u, Sp)] : 0.f;
+        acc = fmaf(t[2 * row_slot<H>(i, Tp)], sv, acc);
+    }
+    return acc;
 }
 
 // Inverse of the accumulated Q[u][f] (Qs: [NLh][H]) into corr[u][v] * nrm[u][v]
