@@ -384,15 +384,17 @@ __device__ __forceinline__ void mac_rows(cf (&q)[K], const cf *S, int Sp, const 
 }
 
 // Slot 0 of mac_rows for one lag u and one component (0: DC, 1: Nyquist):
-// acc += T_i[0].comp * S_{i+u}[0].comp over template rows [0, n), in the order and
-// with the zero rows (i + u >= SH) of mac_rows' slot-0 lane, so the sums agree bit for bit
-template <int H>
-__device__ __forceinline__ float dc_lag_sum(float acc, const cf *S, int Sp, const cf *Tsp, int Tp, int n, int u,
-                                            int SH, int comp) {
+// acc += T_i[0].comp * S_{s_off+i+u}[0].comp over template rows i in [0, n) of Tsp,
+// in the order and with the zero rows (s_off + i + u >= SH) of mac_rows' slot-0
+// lane, so the sums agree bit for bit
+template <int H, int UNROLL = 8>
+__device__ __forceinline__ float dc_lag_sum(float acc, const cf *S, int Sp, const cf *Tsp, int Tp, int n, int s_off,
+                                            int u, int SH, int comp) {
     const float *s = reinterpret_cast<const float *>(S) + comp, *t = reinterpret_cast<const float *>(Tsp) + comp;
-    #pragma unroll 8
+    #pragma unroll UNROLL
     for (int i = 0; i < n; ++i) {
-        const float sv = (i + u < SH) ? s[2 * row_slot<H>(i + u, Sp)] : 0.f;
+        const int r = s_off + i + u;
+        const float sv = (r < SH) ? s[2 * row_slot<H>(r, Sp)] : 0.f;
         acc = fmaf(t[2 * row_slot<H>(i, Tp)], sv, acc);
     }
     return acc;
