@@ -792,56 +792,92 @@ __global__ void k_take_starts(const int32_t *rd, const int32_t *ra, const int32_
 // Windows to re-run in FP64, in order: first every window whose coarse margin
 // is below tc (or whose fine margin is below tf when no zoom starts are given:
 // has_start 0), then every other window whose fine margin is below tf (NaN
-// counts as below). cnt[0] = first kind, cnt[1] = both kinds. One workgroup of
-// 1024 threads; per slice of 4096 windows each thread takes four consecutive
-// ones, and a wave prefix sum of the per-thread counts plus the wave totals
-// gives each selected window its slot.
-__global__ void __launch_bounds__(1024) k_select_rerun(const float *mc, const float *mf, int n, float tc, float tf,
-                                                       int has_start, int32_t *idx, int32_t *cnt) {
-    __shared__ int wtot[16];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    int filled = 0;                     // the same in every thread
-    int first = 0;
-    for (int pass = 0; pass < 2; ++pass) {
-        for (int c0 = 0; c0 < n; c0 += 4 * blockDim.x) {
-            const int i0 = c0 + 4 * threadIdx.x;
-            unsigned flags = 0;
+// counts as below). cnt[0] = first kind, cnt[1] = both kinds. Two launches over
+// blocks of SEL_WIN windows (four consecutive ones per thread): the per-block
+// counts of both kinds, then each block's slots from the counts of the blocks
+// before it and a block prefix sum of the per-thread counts.
+constexpr int SEL_NT = 256, SEL_WIN = 4 * SEL_NT;
+
+__device__ __forceinline__ void rerun_flags(const float *mc, const float *mf, int n, float tc, float tf,
+                                            int has_start, int i0, unsigned &full, unsigned &fine) {
+    full = fine = 0;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int i = i0 + j;
-                if (i < n) {
-                    const bool c = !(mc[i] >= tc), f = !(mf[i] >= tf);
-                    const bool full = c || (f && !has_start);
-                    if (pass == 0 ? full : (!full && f)) flags |= 1u << j;
-                }
-            }
-            const int mine = __popc(flags);
-            int incl = mine;                // inclusive prefix over the wave
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const int t = __shfl_up(incl, o);
-                if (lane >= o) incl += t;
-            }
-            if (lane == 63) wtot[wv] = incl;
-            __syncthreads();
-            int off = 0, tot = 0;
-            for (int k = 0; k < nw; ++k) {
-                const int t = wtot[k];
-                if (k < wv) off += t;
-                tot += t;
-            }
-            int slot = filled + off + incl - mine;
-#pragma unroll
-            for (int j = 0; j < 4; ++j)
-                if (flags & (1u << j)) idx[slot++] = i0 + j;
-            filled += tot;
-            __syncthreads();
+    for (int j = 0; j < 4; ++j) {
+        const int i = i0 + j;
+        if (i < n) {
+            const bool c = !(mc[i] >= tc), f = !(mf[i] >= tf);
+            const bool w = c || (f && !has_start);
+            if (w) full |= 1u << j;
+            else if (f) fine |= 1u << j;
         }
-        if (pass == 0) first = filled;
     }
+}
+
+// sum of v over the workgroup (SEL_NT threads), in every thread
+__device__ __forceinline__ int sel_block_sum(int v, int *sh) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int t = 0;
+#pragma unroll
+    for (int w = 0; w < SEL_NT / 64; ++w) t += sh[w];
+    return t;
+}
+
+__global__ void __launch_bounds__(SEL_NT) k_select_count(const float *mc, const float *mf, int n, float tc, float tf,
+                                                         int has_start, int32_t *bcnt) {
+    __shared__ int sh[SEL_NT / 64];
+    unsigned full, fine;
+    rerun_flags(mc, mf, n, tc, tf, has_start, blockIdx.x * SEL_WIN + 4 * threadIdx.x, full, fine);
+    const int cf = sel_block_sum(__popc(full), sh), cq = sel_block_sum(__popc(fine), sh);
     if (threadIdx.x == 0) {
-        cnt[0] = first;
-        cnt[1] = filled;
+        bcnt[2 * blockIdx.x] = cf;
+        bcnt[2 * blockIdx.x + 1] = cq;
+    }
+}
+
+__global__ void __launch_bounds__(SEL_NT) k_select_write(const float *mc, const float *mf, int n, float tc, float tf,
+                                                         int has_start, const int32_t *bcnt, int nb, int32_t *idx,
+                                                         int32_t *cnt) {
+    __shared__ int sh[SEL_NT / 64], wf[SEL_NT / 64], wq[SEL_NT / 64];
+    // slots of this block: first kind after the earlier blocks' first kind, second
+    // kind after every first-kind window and the earlier blocks' second kind
+    int pf = 0, pq = 0, tf_ = 0, tq = 0;
+    for (int j = threadIdx.x; j < nb; j += SEL_NT) {
+        const int a = bcnt[2 * j], b = bcnt[2 * j + 1];
+        tf_ += a; tq += b;
+        if (j < int(blockIdx.x)) { pf += a; pq += b; }
+    }
+    pf = sel_block_sum(pf, sh); pq = sel_block_sum(pq, sh);
+    tf_ = sel_block_sum(tf_, sh); tq = sel_block_sum(tq, sh);
+    const int i0 = blockIdx.x * SEL_WIN + 4 * threadIdx.x;
+    unsigned full, fine;
+    rerun_flags(mc, mf, n, tc, tf, has_start, i0, full, fine);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int mf_ = __popc(full), mq = __popc(fine);
+    int xf = mf_, xq = mq;                 // inclusive prefix sums over the wave
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int a = __shfl_up(xf, o), b = __shfl_up(xq, o);
+        if (lane >= o) { xf += a; xq += b; }
+    }
+    if (lane == 63) { wf[wv] = xf; wq[wv] = xq; }
+    __syncthreads();
+    int of = 0, oq = 0;
+#pragma unroll
+    for (int w = 0; w < SEL_NT / 64; ++w)
+        if (w < wv) { of += wf[w]; oq += wq[w]; }
+    int sf = pf + of + xf - mf_, sq = tf_ + pq + oq + xq - mq;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        if (full & (1u << j)) idx[sf++] = i0 + j;
+        if (fine & (1u << j)) idx[sq++] = i0 + j;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        cnt[0] = tf_;
+        cnt[1] = tf_ + tq;
     }
 }
 
@@ -884,8 +920,12 @@ void Engine::rerun_fp64(const RunArgs &a, const float *mc, const float *mf, cons
     // set by Engine::run's bounds check
     int32_t *d_idx = buf<int32_t>("p_idx", a.n);
     int32_t *d_cnt = buf<int32_t>("p_count", 3);
-    hipLaunchKernelGGL(k_select_rerun, dim3(1), dim3(1024), 0, a.stream, mc, mf, a.n, tau_coarse, tau_fine,
-                       start32 != nullptr, d_idx, d_cnt);
+    const int nb = idivup(a.n, SEL_WIN);
+    int32_t *d_bcnt = buf<int32_t>("p_bcount", 2 * size_t(nb));
+    hipLaunchKernelGGL(k_select_count, dim3(nb), dim3(SEL_NT), 0, a.stream, mc, mf, a.n, tau_coarse, tau_fine,
+                       start32 != nullptr, d_bcnt);
+    hipLaunchKernelGGL(k_select_write, dim3(nb), dim3(SEL_NT), 0, a.stream, mc, mf, a.n, tau_coarse, tau_fine,
+                       start32 != nullptr, d_bcnt, nb, d_idx, d_cnt);
     int32_t cnt[3];
     HIP_OK(hipMemcpyAsync(cnt, d_cnt, sizeof(cnt), hipMemcpyDeviceToHost, a.stream));
     HIP_OK(hipStreamSynchronize(a.stream));
