@@ -197,8 +197,10 @@ __device__ __forceinline__ void dft_r(cx<T> *x) {
 template <int R1, int R2>
 __host__ __device__ __forceinline__ int dr_pos(int k) { return k / R1 + R2 * (k % R1); }
 
+// post (optional): bin k is multiplied by post[k] as it is stored
 template <class T, int DIR, int R1, int R2, int NTH>
-__device__ __forceinline__ void fft_nat2dr(cx<T> *buf, int nsig, int ss, int es, const cx<T> *tw) {
+__device__ __forceinline__ void fft_nat2dr(cx<T> *buf, int nsig, int ss, int es, const cx<T> *tw,
+                                           const cx<T> *post = nullptr) {
     constexpr int n = R1 * R2;
     #pragma unroll 1
     for (int t = threadIdx.x; t < nsig * R2; t += NTH) {
@@ -223,6 +225,10 @@ __device__ __forceinline__ void fft_nat2dr(cx<T> *buf, int nsig, int ss, int es,
 #pragma unroll
         for (int n2 = 0; n2 < R2; ++n2) x[n2] = b[n2 * es];
         dft_r<T, DIR, R2>(x);
+        if (post) {
+#pragma unroll
+            for (int k2 = 0; k2 < R2; ++k2) x[k2] = cmul(x[k2], post[k1 + R1 * k2]);
+        }
 #pragma unroll
         for (int k2 = 0; k2 < R2; ++k2) b[k2 * es] = x[k2];
     }

@@ -347,14 +347,19 @@ __global__ void __launch_bounds__(NTH, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversa
             }
         }
     };
+    // the half-sample shift multiplies each bin as the forward transform stores it,
+    // except in the 64-VGPR reference kernel, where its own pass spills less
+    constexpr bool FOLD = !(N == 64 && MEAN);
     auto interp_cols = [&]() {          // C <- C at half-row positions (every column)
-        fft_nat2dr<float, +1, R1, R2, NTH>(C, N, 1, PC, tw);
-        for (int i = threadIdx.x; i < N * N; i += NTH) {
-            const int k = i / N, c = i - k * N;
-            cf &z = C[dr_pos<R1, R2>(k) * PC + c];
-            z = cmul(z, hsh[k]);
+        fft_nat2dr<float, +1, R1, R2, NTH>(C, N, 1, PC, tw, FOLD ? hsh : nullptr);
+        if (!FOLD) {
+            for (int i = threadIdx.x; i < N * N; i += NTH) {
+                const int k = i / N, c = i - k * N;
+                cf &z = C[dr_pos<R1, R2>(k) * PC + c];
+                z = cmul(z, hsh[k]);
+            }
+            __syncthreads();
         }
-        __syncthreads();
         fft_dr2nat<float, -1, R1, R2, NTH>(C, N, 1, PC, tw);
     };
     // X: even rows, even columns; then its column interpolation: odd rows, even columns
@@ -370,13 +375,15 @@ __global__ void __launch_bounds__(NTH, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversa
         apply_deramp();
         __syncthreads();
     }
-    fft_nat2dr<float, +1, R1, R2, NTH>(C, N, PC, 1, tw);
-    for (int i = threadIdx.x; i < N * N; i += NTH) {
-        const int r = i / N, k = i - r * N;
-        cf &z = C[r * PC + dr_pos<R1, R2>(k)];
-        z = cmul(z, hsh[k]);
+    fft_nat2dr<float, +1, R1, R2, NTH>(C, N, PC, 1, tw, FOLD ? hsh : nullptr);
+    if (!FOLD) {
+        for (int i = threadIdx.x; i < N * N; i += NTH) {
+            const int r = i / N, k = i - r * N;
+            cf &z = C[r * PC + dr_pos<R1, R2>(k)];
+            z = cmul(z, hsh[k]);
+        }
+        __syncthreads();
     }
-    __syncthreads();
     fft_dr2nat<float, -1, R1, R2, NTH>(C, N, PC, 1, tw);
     emit(0, 1);
     __syncthreads();
