@@ -210,9 +210,9 @@ __device__ __forceinline__ void fft_nat2dr(cx<T> *buf, int nsig, int ss, int es,
 #pragma unroll
         for (int n1 = 0; n1 < R1; ++n1) x[n1] = b[(R2 * n1) * es];
         dft_r<T, DIR, R1>(x);
+        // no branch on n2 == 0 (tw[0] is exactly 1): the R1 - 1 table loads go out together
 #pragma unroll
-        for (int k1 = 1; k1 < R1; ++k1)
-            if (n2) x[k1] = cmul(x[k1], twid<T, DIR>(tw, n2 * k1));
+        for (int k1 = 1; k1 < R1; ++k1) x[k1] = cmul(x[k1], twid<T, DIR>(tw, n2 * k1));
 #pragma unroll
         for (int k1 = 0; k1 < R1; ++k1) b[(R2 * k1) * es] = x[k1];
     }
@@ -247,8 +247,7 @@ __device__ __forceinline__ void fft_dr2nat(cx<T> *buf, int nsig, int ss, int es,
         for (int k2 = 0; k2 < R2; ++k2) x[k2] = b[k2 * es];
         dft_r<T, DIR, R2>(x);
 #pragma unroll
-        for (int m2 = 1; m2 < R2; ++m2)
-            if (k1) x[m2] = cmul(x[m2], twid<T, DIR>(tw, m2 * k1));
+        for (int m2 = 1; m2 < R2; ++m2) x[m2] = cmul(x[m2], twid<T, DIR>(tw, m2 * k1));
 #pragma unroll
         for (int m2 = 0; m2 < R2; ++m2) b[m2 * es] = x[m2];
     }

@@ -179,7 +179,8 @@ __device__ void stockham_pass(const cx<T> *src, cx<T> *dst, const cx<T> *tw, int
 #pragma unroll
         for (int r = 0; r < R; ++r) {
             cx<T> v = in[(j + r * m) * es];
-            if (r > 0 && k > 0) v = cmul(v, twid<T, DIR>(tw, r * step));
+            // no branch on k == 0 (tw[0] is exactly 1): the table loads go out together
+            if (r > 0) v = cmul(v, twid<T, DIR>(tw, r * step));
             x[r] = v;
         }
         dft_radix<T, DIR, R>(x, tw, n);

@@ -24,7 +24,7 @@ __host__ __device__ inline int al16(int x) { return (x + 15) & ~15; }
 __host__ __device__ constexpr int coarse_tc(int W) { return W > 32 ? 32 : W; }
 
 struct CoarseLds {
-    int spec, nrm, red, total, y_off, corr_off;   // bytes (spec, nrm, red, total); cf units (y_off, corr_off)
+    int spec, nrm, red, tw, total, y_off, corr_off;   // bytes (spec, nrm, red, tw, total); cf units (y_off, corr_off)
     __host__ __device__ CoarseLds(int W, int L, int SH, int NLh, int NLw) {
         const int P = L + 1, H = L / 2;
         const int nS = (SH + 1) / 2, nTc = coarse_tc(W) / 2;
@@ -37,7 +37,8 @@ struct CoarseLds {
         spec = 0;
         nrm = al16(spec + cf_units * 8);
         red = al16(nrm + NLh * NLw * 4);
-        total = red + 32 * 8;
+        tw = red + 32 * 8;                            // the row transform's twiddles
+        total = tw + L * 8;
     }
 };
 
@@ -56,7 +57,7 @@ struct CoarseArgs {
 // DCH_ITEMS each); the host picks it only when the lag blocks leave a wave idle
 constexpr int DCH_ITEMS = 3;
 template <int W, int R1, int R2, int K, bool DCH>
-__global__ void __launch_bounds__(NT, (K >= 9 ? 2 : 4)) k_coarse(CoarseArgs a, const cf *tw, int w0) {
+__global__ void __launch_bounds__(NT, (K >= 9 ? 2 : 4)) k_coarse(CoarseArgs a, const cf *tw_g, int w0) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int L = R1 * R2, H = L / 2, P = L + 1, TC = coarse_tc(W), nTc = TC / 2;
     static_assert(W % TC == 0, "template chunks cover the window");
@@ -71,6 +72,10 @@ __global__ void __launch_bounds__(NT, (K >= 9 ? 2 : 4)) k_coarse(CoarseArgs a, c
     float *corr = reinterpret_cast<float *>(S + lay.corr_off);
     double *red = reinterpret_cast<double *>(smem + lay.red);
     const int win = w0 + xcd_window(blockIdx.x, gridDim.x);
+    // twiddles read from LDS (the transforms' table loads wait one at a time)
+    cf *twl = reinterpret_cast<cf *>(smem + lay.tw);
+    for (int i = threadIdx.x; i < L; i += NT) twl[i] = tw_g[i];
+    const cf *tw = twl;
 
     // search amplitudes, row pairs packed as (row 2j, row 2j+1), and the
     // template's FP64 sum and sum of squares, gathers unrolled for memory parallelism
@@ -244,11 +249,11 @@ struct OsArgs {
     double *mean;                     // [n][2]: mean and sum of squared deviations of |.| (MEAN kernels)
 };
 
-// chip (pitch N+1), shift table, deramp row/column rotation factors, reduction scratch
-__host__ __device__ inline int os_lds_bytes(int N) { return al16(N * (N + 1) * 8 + N * 8) + 2 * N * 8 + 48 * 8; }
+// chip (pitch N+1), shift table, deramp row/column rotation factors, reduction scratch, twiddles
+__host__ __device__ inline int os_lds_bytes(int N) { return al16(N * (N + 1) * 8 + N * 8) + 2 * N * 8 + 48 * 8 + N * 8; }
 
 template <int R1, int R2, bool MEAN, int NTH = NT>
-__global__ void __launch_bounds__(NTH, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversample(OsArgs a, const cf *tw, int w0) {
+__global__ void __launch_bounds__(NTH, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversample(OsArgs a, const cf *tw_g, int w0) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int N = R1 * R2, PC = N + 1;
     static_assert(2 * N <= NTH, "one thread per deramp row and column factor");
@@ -257,6 +262,8 @@ __global__ void __launch_bounds__(NTH, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversa
     cf *rowf = reinterpret_cast<cf *>(smem + al16(N * PC * 8 + N * 8));   // e^{i r phy}
     cf *colf = rowf + N;                                                  // e^{i c phx}
     double *red = reinterpret_cast<double *>(colf + N);
+    cf *twl = reinterpret_cast<cf *>(red + 48);                           // twiddles, read from LDS
+    const cf *tw = twl;
     const int win = w0 + xcd_window(blockIdx.x, gridDim.x);
     int64_t r0 = a.sd[win], c0 = a.sa[win];
     if (a.start) { r0 += a.start[2 * win]; c0 += a.start[2 * win + 1]; }
@@ -282,7 +289,10 @@ __global__ void __launch_bounds__(NTH, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversa
     static_assert(16 + 2 * (NTH / 64) <= 48, "wave slots fit the reduction scratch");
     if (MEAN && threadIdx.x < 2 * (NTH / 64)) psum[threadIdx.x] = 0.0;
     gather();
-    for (int i = threadIdx.x; i < N; i += NTH) hsh[i] = a.hsh[i];
+    for (int i = threadIdx.x; i < N; i += NTH) {
+        hsh[i] = a.hsh[i];
+        twl[i] = tw_g[i];
+    }
     __syncthreads();
     if (a.deramp == 1) {
         double ayr = 0, ayi = 0, axr = 0, axi = 0;
@@ -673,7 +683,7 @@ __global__ void __launch_bounds__(NTH) k_fine(FineArgs a, const cf *tw, int w0) 
 // arrive, with the first and last NL - 1 rows kept for the slides.
 // ---------------------------------------------------------------------------
 struct Fine2Lds {
-    int sr, tb, save, nrm, corr, red, total;   // bytes
+    int sr, tb, save, nrm, corr, red, tw, total;   // bytes
     __host__ __device__ Fine2Lds(int P, int NL, int TC) {
         const int PP = P + 1;
         const int srp = (TC + NL - 1 + 1) / 2, tcp = TC / 2;
@@ -683,12 +693,13 @@ struct Fine2Lds {
         nrm = al16(save + 2 * (NL - 1) * P * 4);
         corr = al16(nrm + NL * NL * 4);
         red = al16(corr + NL * NL * 4);
-        total = red + 64 * 8;
+        tw = red + 64 * 8;                         // the row transform's twiddles
+        total = tw + P * 8;
     }
 };
 
 template <int T, int R1, int R2, int K, int NTH, int TC>
-__global__ void __launch_bounds__(NTH, 2) k_fine2(FineArgs a, const cf *tw, int w0) {
+__global__ void __launch_bounds__(NTH, 2) k_fine2(FineArgs a, const cf *tw_g, int w0) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int P = R1 * R2, L = P, PP = L + 1, H = L / 2;
     constexpr int NL = P - T + 1;            // lags per axis (17)
@@ -709,6 +720,10 @@ __global__ void __launch_bounds__(NTH, 2) k_fine2(FineArgs a, const cf *tw, int 
     float *nrm = reinterpret_cast<float *>(smem + lay.nrm);
     float *corr = reinterpret_cast<float *>(smem + lay.corr);
     double *red = reinterpret_cast<double *>(smem + lay.red);
+    // twiddles read from LDS (the transforms' table loads wait one at a time)
+    cf *twl = reinterpret_cast<cf *>(smem + lay.tw);
+    for (int i = threadIdx.x; i < P; i += NTH) twl[i] = tw_g[i];
+    const cf *tw = twl;
     const int win = w0 + xcd_window(blockIdx.x, gridDim.x);
     const float *tp = a.tamp + size_t(win) * T * T;
     const float *sg = a.samp + size_t(win) * P * P;
