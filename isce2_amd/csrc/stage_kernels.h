@@ -330,6 +330,8 @@ __global__ void __launch_bounds__(NTH) k_rowspec_f(const T *img, int nimg, int h
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *C = reinterpret_cast<cx<T> *>(smem_raw);   // [G][PR]
     __shared__ T rowmu[G];                             // the pair's image mean, subtracted on load when given
+    __shared__ cx<T> twl[L];                           // twiddles, read from LDS
+    for (int i = threadIdx.x; i < L; i += NTH) twl[i] = tw[i];
     const int np = (h + 1) / 2;
     const int t0 = blockIdx.x * G;
     const int nr = min(G, nimg * np - t0);
@@ -349,7 +351,7 @@ __global__ void __launch_bounds__(NTH) k_rowspec_f(const T *img, int nimg, int h
         C[k * PR + x] = v;
     }
     __syncthreads();
-    fft_nat2dr<T, -1, R1, R2, NTH>(C, nr, PR, 1, tw);
+    fft_nat2dr<T, -1, R1, R2, NTH>(C, nr, PR, 1, twl);
     for (int i = threadIdx.x; i < nr * H1; i += NTH) {
         const int k = i / H1, kk = i - k * H1;
         const int t = t0 + k, win = t / np, j = t - win * np;
@@ -887,17 +889,9 @@ __global__ void __launch_bounds__(256) k_os2_rows(const cx<T> *src, int ih, int 
 }
 
 // The same two passes for chip sides N = R1 * R2 with the transforms in place
-// (fft_nat2dr, then fft_dr2nat: the spectrum stays in digit-reversed order,
-// where the shift factors are applied), one chip line per LDS row of odd
-// pitch. Shift factors in that order:
-template <class T, int R1, int R2>
-__device__ __forceinline__ void half_shift_dr(cx<T> *sh) {
-    constexpr int N = R1 * R2;
-    for (int q = threadIdx.x; q < N; q += blockDim.x) {
-        const int k1 = q / R2, k2 = q - k1 * R2;     // storage q holds bin k1 + R1 k2
-        sh[q] = half_shift<T>(k1 + R1 * k2, N);
-    }
-}
+// (fft_nat2dr, which applies the shift factors as it stores each bin, then
+// fft_dr2nat: the spectrum stays in digit-reversed order), one chip line per
+// LDS row of odd pitch.
 
 // CB chip columns per block (blockIdx.x), one chip per blockIdx.y; chip height
 // R1 * R2; Y: [nimg][R1 * R2][w]
@@ -907,21 +901,21 @@ __global__ void __launch_bounds__(NTH) k_os2_cols_f(const cx<T> *src, int ih, in
     constexpr int N = R1 * R2, PC = CB + 1;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *C = reinterpret_cast<cx<T> *>(smem_raw);   // [N][PC]: column c at C[c + r * PC]
-    cx<T> *sh = C + N * PC;
+    cx<T> *sh = C + N * PC;                            // shift factors in bin order
+    __shared__ cx<T> twl[N];                           // twiddles, read from LDS
     const int win = blockIdx.y, c0 = blockIdx.x * CB, nc = min(CB, w - c0);
     for (int i = threadIdx.x; i < N * CB; i += NTH) {
         const int r = i / CB, c = i - r * CB;
         if (c < nc) C[r * PC + c] = chip_pixel(src, ih, iw, start, tab, win, N, w, r, c0 + c);
     }
-    half_shift_dr<T, R1, R2>(sh);
-    __syncthreads();
-    fft_nat2dr<T, +1, R1, R2, NTH>(C, nc, 1, PC, tw);
-    for (int i = threadIdx.x; i < N * nc; i += NTH) {
-        const int q = i / nc, c = i - q * nc;
-        C[q * PC + c] = cmul(C[q * PC + c], sh[q]);
+    for (int k = threadIdx.x; k < N; k += NTH) {
+        sh[k] = half_shift<T>(k, N);
+        twl[k] = tw[k];
     }
     __syncthreads();
-    fft_dr2nat<T, -1, R1, R2, NTH>(C, nc, 1, PC, tw);
+    // the shift multiplies each bin as the forward transform stores it
+    fft_nat2dr<T, +1, R1, R2, NTH>(C, nc, 1, PC, twl, sh);
+    fft_dr2nat<T, -1, R1, R2, NTH>(C, nc, 1, PC, twl);
     cx<T> *y = Y + size_t(win) * N * w + c0;
     for (int i = threadIdx.x; i < N * CB; i += NTH) {
         const int r = i / CB, c = i - r * CB;
@@ -940,8 +934,9 @@ __global__ void __launch_bounds__(NTH) k_os2_rows_f(const cx<T> *src, int ih, in
     constexpr int N = R1 * R2, PR = N + 1;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *C = reinterpret_cast<cx<T> *>(smem_raw);   // [G][PR]
-    cx<T> *sh = C + G * PR;
+    cx<T> *sh = C + G * PR;                            // shift factors in bin order
     T *amp = reinterpret_cast<T *>(sh + N);            // [G][N]
+    __shared__ cx<T> twl[N];                           // twiddles, read from LDS
     const int t0 = blockIdx.x * G;
     const int nr = min(G, nimg * 2 * h - t0);
     for (int i = threadIdx.x; i < nr * N; i += NTH) {
@@ -951,15 +946,14 @@ __global__ void __launch_bounds__(NTH) k_os2_rows_f(const cx<T> *src, int ih, in
         C[k * PR + c] = v;
         amp[i] = amag(v);
     }
-    half_shift_dr<T, R1, R2>(sh);
-    __syncthreads();
-    fft_nat2dr<T, +1, R1, R2, NTH>(C, nr, PR, 1, tw);
-    for (int i = threadIdx.x; i < nr * N; i += NTH) {
-        const int k = i / N, q = i - k * N;
-        C[k * PR + q] = cmul(C[k * PR + q], sh[q]);
+    for (int k = threadIdx.x; k < N; k += NTH) {
+        sh[k] = half_shift<T>(k, N);
+        twl[k] = tw[k];
     }
     __syncthreads();
-    fft_dr2nat<T, -1, R1, R2, NTH>(C, nr, PR, 1, tw);
+    // the shift multiplies each bin as the forward transform stores it
+    fft_nat2dr<T, +1, R1, R2, NTH>(C, nr, PR, 1, twl, sh);
+    fft_dr2nat<T, -1, R1, R2, NTH>(C, nr, PR, 1, twl);
     T *o = out + size_t(t0) * 2 * N;
     double s1 = 0.0, s2 = 0.0;
     for (int i = threadIdx.x; i < nr * N; i += NTH) {
