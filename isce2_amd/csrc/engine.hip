@@ -606,10 +606,10 @@ void Engine::run_staged_t(const RunArgs &a) {
         // grouping), unless that leaves too few workgroups to fill the GPU (the
         // FP64 re-run of a few hundred windows)
         if (NL > 9 && n * idivup(H1, 64) >= 2048)
-            hipLaunchKernelGGL((k_mac_lags<T, 17>), dim3(idivup(H1, 64), idivup(NL, 17), n), dim3(64), 0, st, tspec,
+            hipLaunchKernelGGL((k_mac_lags<T, 17, 2>), dim3(idivup(H1, 64), idivup(NL, 17), n), dim3(128), 0, st, tspec,
                                th, sspec, sh, NL, L, rq, T(1.0 / double(L)));
         else
-            hipLaunchKernelGGL((k_mac_lags<T, 9>), dim3(idivup(H1, 64), idivup(NL, 9), n), dim3(64), 0, st, tspec,
+            hipLaunchKernelGGL((k_mac_lags<T, 9, 4>), dim3(idivup(H1, 64), idivup(NL, 9), n), dim3(256), 0, st, tspec,
                                th, sspec, sh, NL, L, rq, T(1.0 / double(L)));
         hipLaunchKernelGGL(k_lagrows_real<T>, dim3(idivup(n * NL, g)), dim3(256), lds, st, rq, n * NL, g, d, NLw, out);
     };

@@ -385,29 +385,54 @@ __global__ void __launch_bounds__(256) k_lagrows_real(const cx<T> *q, int nrows,
 }
 
 // q[u][k] = scale * sum_t conj(T_t[k]) S_{t+u}[k] for lags u0 <= u < u0 + K (u < nl),
-// written as full Hermitian rows of length L for the inverse row FFT. One
-// thread per (bin, lag group, window); it walks the search rows once, keeping
-// the template rows of its K lags in a sliding register window.
-template <class T, int K>
-__global__ void __launch_bounds__(64) k_mac_lags(const cx<T> *tspec, int th, const cx<T> *sspec, int sh, int nl,
-                                                 int L, cx<T> *q, T scale) {
-    const int H1 = L / 2 + 1, k = blockIdx.x * blockDim.x + threadIdx.x, u0 = blockIdx.y * K, win = blockIdx.z;
-    if (k >= H1) return;
-    const cx<T> *ts = tspec + size_t(win) * th * H1 + k;
-    const cx<T> *ss = sspec + size_t(win) * sh * H1 + k;
+// written as full Hermitian rows of length L for the inverse row FFT. A workgroup
+// of NQ waves per (64 bins, lag group, window): wave w walks the w-th contiguous
+// part of the search rows, keeping the template rows of its K lags in a sliding
+// register window, so each thread's chain of row loads is 1/NQ of the whole;
+// the NQ partial sums are added in wave order through LDS.
+template <class T, int K, int NQ>
+__global__ void __launch_bounds__(64 * NQ) k_mac_lags(const cx<T> *tspec, int th, const cx<T> *sspec, int sh, int nl,
+                                                      int L, cx<T> *q, T scale) {
+    __shared__ cx<T> part[NQ > 1 ? NQ - 1 : 1][K][64];
+    const int H1 = L / 2 + 1, lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int k = blockIdx.x * 64 + lane, u0 = blockIdx.y * K, win = blockIdx.z;
+    const int r_end = min(sh, u0 + K - 1 + th), per = (r_end - u0 + NQ - 1) / NQ;
+    const int ra = u0 + w * per, rb = min(r_end, ra + per);
     cx<T> acc[K], tw[K];
 #pragma unroll
     for (int j = 0; j < K; ++j) acc[j] = tw[j] = cx<T>{T(0), T(0)};
-    const int r_end = min(sh, u0 + K - 1 + th);
-    #pragma unroll 2
-    for (int r = u0; r < r_end; ++r) {
+    if (k < H1) {
+        const cx<T> *ts = tspec + size_t(win) * th * H1 + k;
+        const cx<T> *ss = sspec + size_t(win) * sh * H1 + k;
+        // the template rows the lags meet just before row ra: tw[m] = T_{ra - u0 - 1 - m}
 #pragma unroll
-        for (int j = K - 1; j > 0; --j) tw[j] = tw[j - 1];     // tw[j] = T_{r - u0 - j}
-        tw[0] = r - u0 < th ? ts[size_t(r - u0) * H1] : cx<T>{T(0), T(0)};
-        const cx<T> sv = ss[size_t(r) * H1];
+        for (int m = 0; m + 1 < K; ++m) {
+            const int t = ra - u0 - 1 - m;
+            if (t >= 0 && t < th) tw[m] = ts[size_t(t) * H1];
+        }
+        #pragma unroll 2
+        for (int r = ra; r < rb; ++r) {
 #pragma unroll
-        for (int j = 0; j < K; ++j) acc[j] = cmac_conj(acc[j], tw[j], sv);
+            for (int j = K - 1; j > 0; --j) tw[j] = tw[j - 1];     // tw[j] = T_{r - u0 - j}
+            tw[0] = r - u0 < th ? ts[size_t(r - u0) * H1] : cx<T>{T(0), T(0)};
+            const cx<T> sv = ss[size_t(r) * H1];
+#pragma unroll
+            for (int j = 0; j < K; ++j) acc[j] = cmac_conj(acc[j], tw[j], sv);
+        }
     }
+    if (NQ > 1) {
+        if (w > 0) {
+#pragma unroll
+            for (int j = 0; j < K; ++j) part[w - 1][j][lane] = acc[j];
+        }
+        __syncthreads();
+        if (w > 0) return;
+#pragma unroll
+        for (int p = 0; p + 1 < NQ; ++p)
+#pragma unroll
+            for (int j = 0; j < K; ++j) acc[j] = acc[j] + part[p][j][lane];
+    }
+    if (k >= H1) return;
 #pragma unroll
     for (int j = 0; j < K; ++j) {
         if (u0 + j < nl) {   // no early exit: the unrolled loop keeps acc[] in registers
