@@ -302,38 +302,42 @@ __device__ __forceinline__ void window_norms(Amp amp, int SW, int TH, int TW, in
 template <int R1, int R2, int NTH = NT>
 __device__ __forceinline__ void split_pairs_inplace(cf *buf, int nsig, int P) {
     constexpr int L = R1 * R2, H = L / 2, U = 4;
-    constexpr int SIG_PER_ROUND = (NTH * U) / H;
-    static_assert(SIG_PER_ROUND >= 1, "a round must hold at least one signal");
-    const int total = nsig * H;
+    // each thread keeps one bin for the whole call (its storage indices computed
+    // once): bins 1..H-1 on the first SPR (H-1) threads, bin 0 on the next SPR,
+    // so only the wave that holds the bin-0 threads takes both paths; a round
+    // covers U SPR whole signals
+    constexpr int SPR = NTH / H, NG = SPR * (H - 1);
+    static_assert(SPR >= 1, "a pass must hold at least one signal");
+    const int t = threadIdx.x;
+    const bool gen = t < NG, act = t < NG + SPR;
+    const int jj = gen ? t / (H - 1) : t - NG;
+    const int k = gen ? 1 + t % (H - 1) : 0;
+    const int pk = dr_pos<R1, R2>(k), pc = dr_pos<R1, R2>(gen ? L - k : H);
     #pragma unroll 1
-    for (int j0 = 0; j0 < nsig; j0 += SIG_PER_ROUND) {
-        const int base = j0 * H;
-        const int end = min(total, base + SIG_PER_ROUND * H);
+    for (int j0 = 0; j0 < nsig; j0 += U * SPR) {
         cf av[U], bv[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int it = base + u * NTH + threadIdx.x;
-            if (it < end) {
-                const int j = it / H, k = it - j * H;
+            const int j = j0 + jj + u * SPR;
+            if (act && j < nsig) {
                 const cf *z = buf + j * P;
-                if (k == 0) {
-                    const cf z0 = z[0], zh = z[dr_pos<R1, R2>(H)];
-                    av[u] = {z0.x, zh.x};
-                    bv[u] = {z0.y, zh.y};
-                } else {
-                    const cf zk = z[dr_pos<R1, R2>(k)], zc = conj(z[dr_pos<R1, R2>(L - k)]);
+                const cf zk = z[pk], zp = z[pc];
+                if (gen) {
+                    const cf zc = conj(zp);
                     av[u] = (zk + zc) * 0.5f;
                     const cf d = (zk - zc) * 0.5f;
                     bv[u] = {d.y, -d.x};
+                } else {
+                    av[u] = {zk.x, zp.x};
+                    bv[u] = {zk.y, zp.y};
                 }
             }
         }
         __syncthreads();
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int it = base + u * NTH + threadIdx.x;
-            if (it < end) {
-                const int j = it / H, k = it - j * H;
+            const int j = j0 + jj + u * SPR;
+            if (act && j < nsig) {
                 buf[j * P + k] = av[u];
                 buf[j * P + H + k] = bv[u];
             }
