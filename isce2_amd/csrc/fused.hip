@@ -711,6 +711,7 @@ __global__ void __launch_bounds__(NTH, 2) k_fine2(FineArgs a, const cf *tw_g, in
     constexpr int NS = NL - 1;               // rows kept at each end for the slides
     constexpr int Q = P / 2, TQ = T / 2;     // polyphase plane sides
     static_assert(T % TC == 0 && TC % 2 == 0 && NS % 2 == 0 && NS == 16, "even chunks, 16x16 zoom surface");
+    static_assert(NCH > 1 && NS <= 2 * SRP && TC + NS <= 2 * SRP, "the kept rows lie in the first and last chunks");
     static_assert(3 * P <= NTH, "one thread per column and row-pair phase");
     static_assert(6 * P * 8 <= (TC / 2) * (P + 1) * 8, "column-sum partials fit the template chunk region");
     const Fine2Lds lay(P, NL, TC);
@@ -791,26 +792,28 @@ __global__ void __launch_bounds__(NTH, 2) k_fine2(FineArgs a, const cf *tw_g, in
             }
         }
         __syncthreads();
-        // column sums of the new row pairs below T; the first and last NS rows kept
-        // (T and NS are even, so a pair never straddles either boundary)
+        // column sums of the new row pairs below T (T is even: no pair straddles it)
         if (cs_ph < 3) {
-            const int g0 = (c0 >> 1) + first, g1 = (c0 >> 1) + SRP;
-            #pragma unroll 1
-            for (int gp = g0 + (cs_ph - g0 % 3 + 3) % 3; gp < g1; gp += 3) {
-                const cf pv = SR[(gp - (c0 >> 1)) * PP + cs_c];
-                const int r = 2 * gp;
-                if (r < T) {
-                    const double x = pv.x, y = pv.y;
-                    cs1 += x + y;
-                    cs2 += fma(x, x, y * y);
-                }
-                if (r < NS) {
-                    save[r * P + cs_c] = pv.x;
-                    save[(r + 1) * P + cs_c] = pv.y;
-                } else if (r >= T) {
-                    save[(NS + r - T) * P + cs_c] = pv.x;
-                    save[(NS + r + 1 - T) * P + cs_c] = pv.y;
-                }
+            const int g0 = (c0 >> 1) + first, g1 = min((c0 >> 1) + SRP, T / 2);
+            int gp = g0 + (cs_ph - g0 % 3 + 3) % 3;
+            const cf *src = SR + (gp - (c0 >> 1)) * PP + cs_c;
+            #pragma unroll 2
+            for (; gp < g1; gp += 3, src += 3 * PP) {
+                const cf pv = *src;
+                const double x = pv.x, y = pv.y;
+                cs1 += x + y;
+                cs2 += fma(x, x, y * y);
+            }
+        }
+        // the first NS rows (chunk 0's first NS/2 pairs) and the last NS (rows
+        // [T, P), the last chunk's) kept for the slides
+        if (ch == 0 || ch == NCH - 1) {
+            const int pr0 = ch == 0 ? 0 : T / 2 - (c0 >> 1), sr0 = ch == 0 ? 0 : NS;
+            for (int i = threadIdx.x; i < (NS / 2) * P; i += NTH) {
+                const int j = i / P, c = i - j * P;
+                const cf pv = SR[(pr0 + j) * PP + c];
+                save[(sr0 + 2 * j) * P + c] = pv.x;
+                save[(sr0 + 2 * j + 1) * P + c] = pv.y;
             }
         }
         __syncthreads();
