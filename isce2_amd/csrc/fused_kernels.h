@@ -354,34 +354,40 @@ __device__ __forceinline__ int row_slot(int r, int P) { return (r >> 1) * P + (r
 // MAC over template rows [i_beg, i_end): q[k] += conj(T_i[f]) S_{i+u0+k}[f]
 // (slot 0 packs two real bins: (DC, Nyquist) multiply component-wise; with
 // DC = false every lane takes the complex form and the caller computes slot 0's
-// sums elsewhere: dc_lag_sum)
+// sums elsewhere: dc_lag_sum). Rows sit in pairs (row r at slot (r >> 1) P +
+// (r & 1) H); i_beg and t_off are even and the range a multiple of 8, so a
+// thread's search rows keep the parity of u0 and every load of a block is a
+// fixed offset from one of two addresses. A search row at or past SH meets only
+// lags u0 + k >= SH - (i_end - 1 - t_off), past the last lag the callers keep,
+// and the callers' LDS continues past their search rows, so such rows are read
+// without a bound check and only reach sums that are discarded.
 // ---------------------------------------------------------------------------
 template <int H, int K, bool DC = true>
 __device__ __forceinline__ void mac_rows(cf (&q)[K], const cf *S, int Sp, const cf *Tsp, int Tp, int t_off,
                                          int i_beg, int i_end, int u0, int SH, int f) {
     const bool dc = DC && f == 0;
+    const int p = u0 & 1;
+    const int e0 = p ? H : 0, e1 = p ? Sp : H;      // slot offset of even / odd block rows
+    (void)SH;
     #pragma unroll 1
     for (int i0 = i_beg; i0 < i_end; i0 += 8) {
+        const cf *sb = S + ((i0 + u0) >> 1) * Sp + f;
+        const cf *tb = Tsp + ((i0 - t_off) >> 1) * Tp + f;
         cf sw[K + 7];
 #pragma unroll
-        for (int m = 0; m < K + 7; ++m) {
-            const int row = i0 + u0 + m;
-            sw[m] = (row < SH) ? S[row_slot<H>(row, Sp) + f] : cf{0.f, 0.f};
-        }
+        for (int m = 0; m < K + 7; ++m) sw[m] = sb[(m >> 1) * Sp + ((m & 1) ? e1 : e0)];
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            if (i0 + i < i_end) {
-                const cf tv = Tsp[row_slot<H>(i0 + i - t_off, Tp) + f];
-                // one branch-free form for every lane: conj(t) s, or for slot 0 the
-                // component-wise product (t.x s.x, t.y s.y)
-                const float c0 = tv.x, c1 = dc ? 0.f : tv.y, c2 = dc ? tv.y : tv.x, c3 = dc ? 0.f : -tv.y;
-                // two chained FMAs per component (the accumulator is the addend)
+            const cf tv = tb[(i >> 1) * Tp + (i & 1) * H];
+            // one branch-free form for every lane: conj(t) s, or for slot 0 the
+            // component-wise product (t.x s.x, t.y s.y)
+            const float c0 = tv.x, c1 = dc ? 0.f : tv.y, c2 = dc ? tv.y : tv.x, c3 = dc ? 0.f : -tv.y;
+            // two chained FMAs per component (the accumulator is the addend)
 #pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    const cf s = sw[i + k];
-                    q[k].x = fmaf(c1, s.y, fmaf(c0, s.x, q[k].x));
-                    q[k].y = fmaf(c3, s.x, fmaf(c2, s.y, q[k].y));
-                }
+            for (int k = 0; k < K; ++k) {
+                const cf s = sw[i + k];
+                q[k].x = fmaf(c1, s.y, fmaf(c0, s.x, q[k].x));
+                q[k].y = fmaf(c3, s.x, fmaf(c2, s.y, q[k].y));
             }
         }
     }
