@@ -52,11 +52,7 @@ struct CoarseArgs {
     float *margin;                 // [n] top-1 minus top-2 of the normalized surface, or null
 };
 
-// DCH: the MAC lanes all take the complex form and slot 0's (DC, Nyquist) lag
-// sums come from threads the MAC leaves idle, one per lag and component (up to
-// DCH_ITEMS each); the host picks it only when the lag blocks leave a wave idle
-constexpr int DCH_ITEMS = 3;
-template <int W, int R1, int R2, int K, bool DCH>
+template <int W, int R1, int R2, int K>
 __global__ void __launch_bounds__(NT, (K >= 9 ? 2 : 4)) k_coarse(CoarseArgs a, const cf *tw_g, int w0) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int L = R1 * R2, H = L / 2, P = L + 1, TC = coarse_tc(W), nTc = TC / 2;
@@ -151,10 +147,6 @@ __global__ void __launch_bounds__(NT, (K >= 9 ? 2 : 4)) k_coarse(CoarseArgs a, c
     cf q[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) q[k] = {0.f, 0.f};
-    // a helper keeps its sums in the registers of its (unused) MAC accumulators;
-    // its index is recomputed where needed, so that nothing more stays live
-    auto helper = [&]() { return int(threadIdx.x) - (((NLh + K - 1) / K * H + 63) & ~63); };
-    static_assert(DCH_ITEMS <= 2 * K, "helper sums fit the accumulators");
     #pragma unroll 1
     for (int c0 = 0; c0 < W; c0 += TC) {
         // template rows [c0, c0 + TC) minus the FP64 mean, packed in pairs, from
@@ -180,33 +172,14 @@ __global__ void __launch_bounds__(NT, (K >= 9 ? 2 : 4)) k_coarse(CoarseArgs a, c
             fft_nat2dr<float, -1, R1, R2, NT>(Tb, nTc, P, 1, tw);
             split_pairs_inplace<R1, R2>(Tb, nTc, P);
         }
-        if (active) mac_rows<H, K, !DCH>(q, S, P, Tb, P, c0, c0, c0 + TC, u0, SH, f);
-        else if (DCH && helper() >= 0) {
-            const int hid = helper(), nh = NT - (int(threadIdx.x) - hid);
-#pragma unroll
-            for (int j = 0; j < DCH_ITEMS; ++j) {
-                const int it = hid + j * nh;
-                if (it < 2 * NLh) {
-                    float &acc = (j & 1) ? q[j >> 1].y : q[j >> 1].x;
-                    acc = dc_lag_sum<H, 2>(acc, S, P, Tb, P, TC, c0, it >> 1, SH, it & 1);
-                }
-            }
-        }
+        if (active) mac_rows<H, K>(q, S, P, Tb, P, c0, c0, c0 + TC, u0, SH, f);
         __syncthreads();
     }
     cf *Qs = S;
     if (active) {
 #pragma unroll
         for (int k = 0; k < K; ++k)
-            if (u0 + k < NLh && (!DCH || f != 0)) Qs[(u0 + k) * H + f] = q[k];
-    }
-    if (DCH && helper() >= 0) {
-        const int hid = helper(), nh = NT - (int(threadIdx.x) - hid);
-#pragma unroll
-        for (int j = 0; j < DCH_ITEMS; ++j) {
-            const int it = hid + j * nh;
-            if (it < 2 * NLh) reinterpret_cast<float *>(Qs + (it >> 1) * H)[it & 1] = (j & 1) ? q[j >> 1].y : q[j >> 1].x;
-        }
+            if (u0 + k < NLh) Qs[(u0 + k) * H + f] = q[k];
     }
     __syncthreads();
     inverse_lags<R1, R2>(Qs, NLh, NLw, S + lay.y_off, P, tw, nrm, corr);
@@ -862,25 +835,15 @@ int pick_k(int nl, int blocks) {
     return 0;
 }
 
-template <int W, int R1, int R2, bool DCH>
-void launch_coarse_k(int K, const CoarseArgs &a, const cf *tw, int n, int w0, size_t lds, hipStream_t st) {
-    switch (K) {
-        case 2: hipLaunchKernelGGL((k_coarse<W, R1, R2, 2, DCH>), dim3(n), dim3(NT), lds, st, a, tw, w0); break;
-        case 3: hipLaunchKernelGGL((k_coarse<W, R1, R2, 3, DCH>), dim3(n), dim3(NT), lds, st, a, tw, w0); break;
-        case 5: hipLaunchKernelGGL((k_coarse<W, R1, R2, 5, DCH>), dim3(n), dim3(NT), lds, st, a, tw, w0); break;
-        case 6: hipLaunchKernelGGL((k_coarse<W, R1, R2, 6, DCH>), dim3(n), dim3(NT), lds, st, a, tw, w0); break;
-        default: hipLaunchKernelGGL((k_coarse<W, R1, R2, 9, DCH>), dim3(n), dim3(NT), lds, st, a, tw, w0); break;
-    }
-}
-
-// the slot-0 helper form when the lag blocks leave a whole wave idle and its
-// threads cover the 2 NLh sums in at most DCH_ITEMS rounds
 template <int W, int R1, int R2>
-void launch_coarse(int K, int NLh, const CoarseArgs &a, const cf *tw, int n, int w0, size_t lds, hipStream_t st) {
-    constexpr int H = R1 * R2 / 2;
-    const int hb = ((NLh + K - 1) / K * H + 63) & ~63, nh = NT - hb;
-    if (nh >= 64 && 2 * NLh <= DCH_ITEMS * nh) launch_coarse_k<W, R1, R2, true>(K, a, tw, n, w0, lds, st);
-    else launch_coarse_k<W, R1, R2, false>(K, a, tw, n, w0, lds, st);
+void launch_coarse(int K, const CoarseArgs &a, const cf *tw, int n, int w0, size_t lds, hipStream_t st) {
+    switch (K) {
+        case 2: hipLaunchKernelGGL((k_coarse<W, R1, R2, 2>), dim3(n), dim3(NT), lds, st, a, tw, w0); break;
+        case 3: hipLaunchKernelGGL((k_coarse<W, R1, R2, 3>), dim3(n), dim3(NT), lds, st, a, tw, w0); break;
+        case 5: hipLaunchKernelGGL((k_coarse<W, R1, R2, 5>), dim3(n), dim3(NT), lds, st, a, tw, w0); break;
+        case 6: hipLaunchKernelGGL((k_coarse<W, R1, R2, 6>), dim3(n), dim3(NT), lds, st, a, tw, w0); break;
+        default: hipLaunchKernelGGL((k_coarse<W, R1, R2, 9>), dim3(n), dim3(NT), lds, st, a, tw, w0); break;
+    }
 }
 
 }  // namespace fused
@@ -965,9 +928,9 @@ void Engine::launch_fused(const FusedTables &tb, const FusedLaunch &c, int n, in
     const size_t lds_f2 = Fine2Lds(P, P - T2 + 1, 32).total;
     {
         timed("k_coarse", n, st, [&] {
-            if (W == 64) launch_coarse<64, 8, 16>(K1, sz.raw_corr_h, ca, tw1, n, w0, lds_c, st);
-            else if (L1 == 128) launch_coarse<32, 8, 16>(K1, sz.raw_corr_h, ca, tw1, n, w0, lds_c, st);
-            else launch_coarse<32, 8, 8>(K1, sz.raw_corr_h, ca, tw1, n, w0, lds_c, st);
+            if (W == 64) launch_coarse<64, 8, 16>(K1, ca, tw1, n, w0, lds_c, st);
+            else if (L1 == 128) launch_coarse<32, 8, 16>(K1, ca, tw1, n, w0, lds_c, st);
+            else launch_coarse<32, 8, 8>(K1, ca, tw1, n, w0, lds_c, st);
         });
         OsArgs oref{c.ref, c.ref_pitch, mode, c.ref_d, c.ref_a, nullptr, p.deramp, h_ref, tamp, tmean};
         OsArgs osec{c.sec, c.sec_pitch, mode, c.sec_d, c.sec_a, c.start, p.deramp, h_zoom, samp, nullptr};
