@@ -240,28 +240,27 @@ __global__ void __launch_bounds__(NTH, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversa
     const int win = w0 + xcd_window(blockIdx.x, gridDim.x);
     int64_t r0 = a.sd[win], c0 = a.sa[win];
     if (a.start) { r0 += a.start[2 * win]; c0 += a.start[2 * win + 1]; }
-    auto gather = [&]() {
+    // deramp of pixel (r, c) (float2's operator*= (float2.h:71-75): the imaginary part
+    // uses the new real part); the phases and their sines and cosines are FP64, the
+    // factors and the rotation FP32
+    auto deramp_px = [&](cf z, int r, int c) -> cf {
+        const cf f = cmul(colf[c], rowf[r]);
+        const float nx = z.x * f.x - z.y * f.y;
+        return {nx, z.y * f.x + nx * f.y};
+    };
+    // chip into LDS; rot: deramped as it loads (the factors are in place)
+    auto gather = [&](bool rot) {
 #pragma unroll 4
         for (int i = threadIdx.x; i < N * N; i += NTH) {
             const int r = i / N, c = i - r * N;
-            C[r * PC + c] = pixel_cx(a.img, a.mode, (r0 + r) * a.pitch + c0 + c);
-        }
-    };
-    // deramp (float2's operator*= (float2.h:71-75): the imaginary part uses the new real part);
-    // the phases and their sines and cosines are FP64, the factors and the rotation FP32
-    auto apply_deramp = [&]() {
-        for (int i = threadIdx.x; i < N * N; i += NTH) {
-            const int r = i / N, c = i - r * N;
-            const cf f = cmul(colf[c], rowf[r]);
-            const cf z = C[r * PC + c];
-            const float nx = z.x * f.x - z.y * f.y;
-            C[r * PC + c] = {nx, z.y * f.x + nx * f.y};
+            const cf z = pixel_cx(a.img, a.mode, (r0 + r) * a.pitch + c0 + c);
+            C[r * PC + c] = rot ? deramp_px(z, r, c) : z;
         }
     };
     double *psum = red + 16;             // [NTH/64][2] per-wave FP64 sums of |.| and |.|^2 (MEAN)
     static_assert(16 + 2 * (NTH / 64) <= 48, "wave slots fit the reduction scratch");
     if (MEAN && threadIdx.x < 2 * (NTH / 64)) psum[threadIdx.x] = 0.0;
-    gather();
+    gather(false);
     for (int i = threadIdx.x; i < N; i += NTH) {
         hsh[i] = a.hsh[i];
         twl[i] = tw_g[i];
@@ -293,15 +292,15 @@ __global__ void __launch_bounds__(NTH, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversa
             (threadIdx.x < N ? rowf : colf)[k] = {float(cs), float(sn)};
         }
         __syncthreads();
-        apply_deramp();
-        __syncthreads();
     }
     float *out = a.out + size_t(win) * 4 * N * N;
     // FP64 sums of |.| and |.|^2: each emit reduces its own over the wave and
     // adds them to the wave's LDS slot, so no FP64 accumulator stays live
     // through the transforms (the reference-chip kernel is held to 64 VGPRs)
     static_assert(N % 4 == 0, "four output columns per store");
-    auto emit = [&](int p, int q) {      // |C| -> output pixels (2r+p, 2c+q), four columns per thread
+    // |C| -> output pixels (2r+p, 2c+q), four columns per thread; rot: C is deramped
+    // in place first (each thread rewrites the pixels it reads)
+    auto emit = [&](int p, int q, bool rot) {
         double s1 = 0.0, s2 = 0.0;
         float *plane = out + size_t(2 * p + q) * N * N;           // whole lines per phase
         for (int i = threadIdx.x; i < N * N / 4; i += NTH) {
@@ -309,7 +308,11 @@ __global__ void __launch_bounds__(NTH, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversa
             float amp[4];
 #pragma unroll
             for (int m = 0; m < 4; ++m) {
-                const cf v = C[r * PC + c + m];
+                cf v = C[r * PC + c + m];
+                if (rot) {
+                    v = deramp_px(v, r, c + m);
+                    C[r * PC + c + m] = v;
+                }
                 amp[m] = fast_sqrt(v.x * v.x + v.y * v.y);
                 if (MEAN) {
                     s1 += amp[m];
@@ -346,18 +349,14 @@ __global__ void __launch_bounds__(NTH, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversa
         fft_dr2nat<float, -1, R1, R2, NTH>(C, N, 1, PC, tw);
     };
     // X: even rows, even columns; then its column interpolation: odd rows, even columns
-    emit(0, 0);
+    emit(0, 0, a.deramp == 1);
     __syncthreads();
     interp_cols();
-    emit(1, 0);
+    emit(1, 0, false);
     __syncthreads();
     // Y = X interpolated along rows: even rows, odd columns; then its columns: odd rows, odd columns
-    gather();
+    gather(a.deramp == 1);
     __syncthreads();
-    if (a.deramp == 1) {
-        apply_deramp();
-        __syncthreads();
-    }
     fft_nat2dr<float, +1, R1, R2, NTH>(C, N, PC, 1, tw, FOLD ? hsh : nullptr);
     if (!FOLD) {
         for (int i = threadIdx.x; i < N * N; i += NTH) {
@@ -368,10 +367,10 @@ __global__ void __launch_bounds__(NTH, (R1 * R2 == 64 && MEAN) ? 8 : 1) k_oversa
         __syncthreads();
     }
     fft_dr2nat<float, -1, R1, R2, NTH>(C, N, PC, 1, tw);
-    emit(0, 1);
+    emit(0, 1, false);
     __syncthreads();
     interp_cols();
-    emit(1, 1);
+    emit(1, 1, false);
     if (MEAN) {
         // FP64 mean and sum of squared deviations (sum2 - sum^2 / count)
         __syncthreads();
