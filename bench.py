@@ -30,8 +30,8 @@ FP32_PEAK_TFLOPS = 157.3
 # HBM bytes per window of each fused kernel, from rocprofv3 FETCH_SIZE + WRITE_SIZE (KiB per dispatch of a
 # 31248-window launch) scaled by a 1 GiB device copy's counts (FETCH x2, WRITE x1); re-measure after kernel changes
 HBM_BYTES_PER_WINDOW = {k: v * 1024 / 31248 for k, v in {
-    "k_fine": 2266558.7 * 2 + 681.4, "k_coarse": 651665.8 * 2 + 1467.2,
-    "k_oversample_ref": 503017.5 * 2 + 2562843.8, "k_oversample_search": 560162.7 * 2 + 2531088.0}.items()}
+    "k_fine": 2266579.0 * 2 + 698.5, "k_coarse": 652045.4 * 2 + 1466.8,
+    "k_oversample_ref": 453804.7 * 2 + 2500350.4, "k_oversample_search": 560834.7 * 2 + 2531088.0}.items()}
 HBM_PEAK_GBS = 8000.0
 
 
