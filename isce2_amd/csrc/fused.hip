@@ -57,6 +57,9 @@ __global__ void __launch_bounds__(NT, (K >= 9 ? 2 : 4)) k_coarse(CoarseArgs a, c
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int L = R1 * R2, H = L / 2, P = L + 1, TC = coarse_tc(W), nTc = TC / 2;
     static_assert(W % TC == 0, "template chunks cover the window");
+    // mac_rows reads up to K - 2 rows past the search (only discarded lags use them):
+    // they land in the template chunk right after the search pairs
+    static_assert(nTc >= K / 2 + 1, "the template chunk follows the search rows");
     static_assert(2 * NORM_BLOCKS * L <= nTc * P, "window-sum scratch fits the template chunk region");
     const int SH = W + 2 * a.hs_d, SW = W + 2 * a.hs_a;
     const int NLh = SH - W + 1, NLw = SW - W + 1;
@@ -568,6 +571,8 @@ __global__ void __launch_bounds__(NTH) k_fine(FineArgs a, const cf *tw, int w0) 
     constexpr int Z = NL - 1;          // 16 (trimmed surface)
     constexpr int SO = 256;            // oversampled surface side
     static_assert(Z == 16, "fused surface stage assumes a 16x16 zoom surface");
+    // mac_rows reads up to K - 2 rows past the search; the template spectra follow it
+    static_assert(T / 2 >= K / 2 + 1, "the template region follows the search rows");
     const FineLds lay(P, NL, T);
     cf *S = reinterpret_cast<cf *>(smem + lay.spec);
     cf *Tb = reinterpret_cast<cf *>(smem + lay.tchunk);
@@ -684,6 +689,8 @@ __global__ void __launch_bounds__(NTH, 2) k_fine2(FineArgs a, const cf *tw_g, in
     constexpr int Q = P / 2, TQ = T / 2;     // polyphase plane sides
     static_assert(T % TC == 0 && TC % 2 == 0 && NS % 2 == 0 && NS == 16, "even chunks, 16x16 zoom surface");
     static_assert(NCH > 1 && NS <= 2 * SRP && TC + NS <= 2 * SRP, "the kept rows lie in the first and last chunks");
+    // mac_rows reads up to K - 2 rows past a chunk's search rows; the template pairs follow them
+    static_assert(TCP >= K / 2 + 1, "the template chunk follows the search rows");
     static_assert(3 * P <= NTH, "one thread per column and row-pair phase");
     static_assert(6 * P * 8 <= (TC / 2) * (P + 1) * 8, "column-sum partials fit the template chunk region");
     const Fine2Lds lay(P, NL, TC);
