@@ -357,10 +357,11 @@ __device__ __forceinline__ int row_slot(int r, int P) { return (r >> 1) * P + (r
 // sums elsewhere: dc_lag_sum). Rows sit in pairs (row r at slot (r >> 1) P +
 // (r & 1) H); i_beg and t_off are even and the range a multiple of 8, so a
 // thread's search rows keep the parity of u0 and every load of a block is a
-// fixed offset from one of two addresses. A search row at or past SH meets only
-// lags u0 + k >= SH - (i_end - 1 - t_off), past the last lag the callers keep,
-// and the callers' LDS continues past their search rows, so such rows are read
-// without a bound check and only reach sums that are discarded.
+// fixed offset from one of two addresses. Template row i meets search row
+// i + u0 + k, so a search row at or past SH meets only lags u0 + k >=
+// SH - (i_end - 1), past the last lag the callers keep (SH - template height + 1
+// lags), and the callers' LDS continues past their search rows: such rows are
+// read without a bound check and only reach sums that are discarded.
 // ---------------------------------------------------------------------------
 template <int H, int K, bool DC = true>
 __device__ __forceinline__ void mac_rows(cf (&q)[K], const cf *S, int Sp, const cf *Tsp, int Tp, int t_off,
@@ -395,8 +396,9 @@ __device__ __forceinline__ void mac_rows(cf (&q)[K], const cf *S, int Sp, const 
 
 // Slot 0 of mac_rows for one lag u and one component (0: DC, 1: Nyquist):
 // acc += T_i[0].comp * S_{s_off+i+u}[0].comp over template rows i in [0, n) of Tsp,
-// in the order and with the zero rows (s_off + i + u >= SH) of mac_rows' slot-0
-// lane, so the sums agree bit for bit
+// in mac_rows' order, so for the lags a caller keeps (whose search rows all lie
+// below SH; rows at or past it read as zero here) the sums agree bit for bit
+// with mac_rows' slot-0 lane
 template <int H, int UNROLL = 8>
 __device__ __forceinline__ float dc_lag_sum(float acc, const cf *S, int Sp, const cf *Tsp, int Tp, int n, int s_off,
                                             int u, int SH, int comp) {
